@@ -1,0 +1,134 @@
+"""Throughput of the Fock-build hot path on one MI355X.
+
+Workload: direct RHF/cc-pVDZ on a seeded (H2O)16 cluster (4x2x2 lattice, 2.9 A
+spacing, PCG64 seed 2, uniform rotations, +-0.1 A jitter), 400 Cartesian /
+384 spherical functions, Schwarz threshold 1e-12, FP64 throughout.  A step is
+one Fock build: J and K of one density over every surviving shell quartet,
+with the density and the results resident in HBM.  value = Fock builds/s.
+Also reported: one full SCF solve (SAD guess, ADIIS->CDIIS, spherical
+orthogonaliser, tol 1e-10) as SCF solves/s with its iteration count, the
+per-step latency percentiles, the roofline of the dominant shell-quartet class
+kernel and of the whole build, and the oracle's CPU time on a bounded sample.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "hf-rs_amd")
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("hf_rs_amd", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["hf_rs_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(mol, text, stride, threads, density):
+    """Oracle direct J/K (the reference's McMurchie-Davidson algorithm, FP64,
+    OpenMP) on the bra pairs ij % stride == 0, scaled by algorithmic FLOPs."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ob = O.Basis.build(text, mol.symbols, mol.xyz)
+    total, _, _ = ob.fock_flops()
+    sample, _, _ = ob.fock_flops(stride, 0)
+    _, _, sec = ob.direct_jk(density, stride, 0)
+    est = sec * total / sample
+    return {"value": 1.0 / est, "unit": "Fock builds/s", "cores": threads, "kind": "port",
+            "sample": "oracle direct J/K over bra shell pairs ij %% %d == 0 (%.1f%% of the algorithmic FLOPs), "
+                      "%.1f s on %d threads, scaled to the full build" % (stride, 100 * sample / total, sec,
+                                                                          threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-stride", type=int, default=24)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scf", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this engine's workloads are single-GPU (see DESIGN.md, sharding); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    hf = load_pkg()
+    sys.path.insert(0, PKG)
+    import molecules as M
+    if hf.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    mol = M.water_cluster((4, 2, 2), seed=2)
+    text = M.basis_text("cc-pvdz")
+    basis = hf.Basis.from_g94(text, mol.symbols, mol.xyz)
+    t0 = time.perf_counter()
+    ctx = hf.Context(basis)
+    setup_s = time.perf_counter() - t0
+    info = ctx.info()
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal((basis.n, 40)) * 0.2
+    dens = c @ c.T  # symmetric positive semidefinite; work does not depend on values
+    ctx.set_density(dens)
+    for _ in range(args.warmup):
+        ctx.jk_resident()
+    ctx.profile(True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        ctx.jk_resident()  # returns after the build completes on the device
+        lat.append(time.perf_counter() - s0)
+    elapsed = time.perf_counter() - t0
+    ctx.profile(False)  # keeps the accumulated statistics until the next enable
+    stats = ctx.class_stats()
+    total_flops, prim_q = ctx.fock_flops()
+    value = args.steps / elapsed
+    ms_classes = sum(s["ms"] for s in stats) / args.steps
+    dom = max(stats, key=lambda s: s["ms"])
+    dom_ms = dom["ms"] / max(1, dom["launches"])
+    dom_tflops = dom["flops"] / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
+    whole_tflops = total_flops / (ms_classes * 1e-3) / 1e12
+    line = {
+        "metric": "Fock builds/s (direct FP64 J+K, |dE|<=1e-8 Eh vs FP64 oracle)",
+        "value": value, "unit": "Fock builds/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded water cluster geometry)",
+        "config": {"workload": "direct RHF/cc-pVDZ (H2O)16, 4x2x2 lattice 2.9 A, seed 2, Schwarz 1e-12",
+                   "n_cart": basis.n, "n_sph": 384, "shell_quartets": info["n_quartets"],
+                   "primitive_quartets": prim_q, "parallelism": "single GPU"},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "setup_s": setup_s,
+        "roofline": {"bound": "fp64-valu", "kernel": "k_jk<%d,%d,%d,%d>" % dom["l"],
+                     "achieved": dom_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": dom_tflops / PEAK_FP64_TFLOPS, "traffic": None,
+                     "flops_per_launch": dom["flops"], "ms_per_launch": dom_ms},
+        "fock_build_roofline": {"achieved": whole_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                "frac": whole_tflops / PEAK_FP64_TFLOPS, "algorithmic_flops": total_flops,
+                                "class_kernel_ms": ms_classes},
+        "classes": [{"l": s["l"], "ms": s["ms"] / max(1, s["launches"]), "gflop": s["flops"] / 1e9,
+                     "quartets": s["quartets"]} for s in stats if s["quartets"]],
+    }
+    if not args.no_scf:
+        r = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
+        line["scf"] = {"solves_per_s": 1.0 / r["seconds_total"], "seconds": r["seconds_total"],
+                       "iterations": r["iterations"], "converged": r["converged"], "fock_builds": r["fock_builds"],
+                       "seconds_fock": r["seconds_fock"], "energy": r["total_energy"]}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(mol, text, args.cpu_stride, args.cpu_threads, dens)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

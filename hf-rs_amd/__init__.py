@@ -1,0 +1,292 @@
+"""hf_rs_amd -- Python view of the MI355X Fock-build engine (libhfg.so).
+
+Mirrors the hf-rs calls that reach the Fock-build path: build_basis
+(src/basis.rs:402-452), compute_shell_quartet (src/shell_integrals.rs:138),
+compute_eri_tensor (src/scf.rs:275), contract_jk / build_fock_matrix
+(src/scf.rs:337-369) and the RHF driver run_scf_in_space_with_convergence
+(src/scf.rs:410-627), through the C ABI of include/hfg.h.  There is no CPU
+fallback: importing works without a GPU, but every compute call needs the
+compiled library and a visible device, and raises otherwise.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhfg.so")
+_LIB = None
+
+HFG_OK, HFG_INVALID_INPUT, HFG_INTEGRAL, HFG_DEVICE, HFG_NOT_CONVERGED, HFG_UNSUPPORTED = range(6)
+
+
+class HfError(RuntimeError):
+    """Mirror of hf-rs HfError (src/error.rs:5-15); .code is the C status."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+class BasisView(C.Structure):
+    _fields_ = [("n_functions", C.c_int32), ("atom_index", C.c_void_p), ("center", C.c_void_p),
+                ("lxyz", C.c_void_p), ("prim_offset", C.c_void_p), ("exponent", C.c_void_p),
+                ("coefficient", C.c_void_p)]
+
+
+class ScfOptions(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("energy_tol", C.c_double), ("residual_tol", C.c_double),
+                ("density_tol", C.c_double), ("conv_algo", C.c_int32), ("spherical", C.c_int32),
+                ("verbose", C.c_int32)]
+
+
+class ScfResult(C.Structure):
+    _fields_ = [("total_energy", C.c_double), ("electronic_energy", C.c_double),
+                ("nuclear_repulsion", C.c_double), ("iterations", C.c_int32), ("converged", C.c_int32),
+                ("n_occ", C.c_int32), ("fock_builds", C.c_int32), ("seconds_total", C.c_double),
+                ("seconds_fock", C.c_double)]
+
+
+EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_destroy", "hfg_ctx_info",
+            "hfg_schwarz", "hfg_shell_quartet", "hfg_eri_packed", "hfg_contract_jk", "hfg_build_fock",
+            "hfg_contract_jk_device", "hfg_scf_rhf", "hfg_basis_from_g94", "hfg_basis_free",
+            "hfg_basis_get_view", "hfg_set_density", "hfg_jk_resident", "hfg_get_jk", "hfg_profile",
+            "hfg_class_stats", "hfg_fock_flops", "hfg_md_flops"]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libhfg.so is not built: run __graft_entry__.build() (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        L.hfg_last_error.restype = C.c_char_p
+        vp, i32, dbl = C.c_void_p, C.c_int, C.c_double
+        L.hfg_ctx_create.argtypes = [C.POINTER(BasisView), C.POINTER(vp)]
+        L.hfg_ctx_destroy.argtypes = [vp]
+        L.hfg_ctx_info.argtypes = [vp] + [C.POINTER(C.c_int64)] * 4
+        L.hfg_schwarz.argtypes = [vp, vp]
+        L.hfg_shell_quartet.argtypes = [vp, i32, i32, i32, i32, vp]
+        L.hfg_eri_packed.argtypes = [vp, vp, C.c_size_t]
+        L.hfg_contract_jk.argtypes = [vp, vp, vp, vp]
+        L.hfg_build_fock.argtypes = [vp, vp, vp, vp]
+        L.hfg_contract_jk_device.argtypes = [vp, vp, vp, vp]
+        L.hfg_scf_rhf.argtypes = [C.POINTER(BasisView), vp, i32, i32, C.POINTER(ScfOptions), C.POINTER(ScfResult),
+                                  vp, vp, vp]
+        L.hfg_basis_from_g94.argtypes = [C.c_char_p, C.c_char_p, vp, i32, C.POINTER(vp)]
+        L.hfg_basis_free.argtypes = [vp]
+        L.hfg_basis_get_view.argtypes = [vp, C.POINTER(BasisView)]
+        L.hfg_set_density.argtypes = [vp, vp]
+        L.hfg_jk_resident.argtypes = [vp]
+        L.hfg_get_jk.argtypes = [vp, vp, vp]
+        L.hfg_profile.argtypes = [vp, i32]
+        L.hfg_class_stats.argtypes = [vp, i32, vp, C.POINTER(dbl), C.POINTER(C.c_int64), C.POINTER(dbl),
+                                      C.POINTER(C.c_int64)]
+        L.hfg_fock_flops.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl)]
+        L.hfg_md_flops.argtypes = [i32] * 4
+        L.hfg_md_flops.restype = dbl
+        _LIB = L
+    return _LIB
+
+
+def _check(status):
+    if status != HFG_OK:
+        raise HfError(status, lib().hfg_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def md_flops(la, lb, lc, ld):
+    return lib().hfg_md_flops(la, lb, lc, ld)
+
+
+def device_count():
+    return lib().hfg_device_count()
+
+
+class Basis:
+    """Cartesian basis as flat arrays, one row per ContractedGaussian."""
+
+    KEYS = ("atom_index", "center", "lxyz", "prim_offset", "exponent", "coefficient")
+
+    def __init__(self, arrays):
+        self.a = {
+            "atom_index": np.ascontiguousarray(arrays["atom_index"], dtype=np.int32),
+            "center": np.ascontiguousarray(np.asarray(arrays["center"], dtype=np.float64).reshape(-1, 3)),
+            "lxyz": np.ascontiguousarray(np.asarray(arrays["lxyz"], dtype=np.int32).reshape(-1, 3)),
+            "prim_offset": np.ascontiguousarray(arrays["prim_offset"], dtype=np.int32),
+            "exponent": np.ascontiguousarray(arrays["exponent"], dtype=np.float64),
+            "coefficient": np.ascontiguousarray(arrays["coefficient"], dtype=np.float64),
+        }
+        self.n = len(self.a["atom_index"])
+        self.view = BasisView(self.n, *[_p(self.a[k]) for k in self.KEYS])
+
+    @classmethod
+    def from_g94(cls, text, symbols, xyz_bohr):
+        xyz = np.ascontiguousarray(np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3))
+        h = C.c_void_p()
+        st = lib().hfg_basis_from_g94(text.encode(), " ".join(symbols).encode(), _p(xyz), len(symbols), C.byref(h))
+        if st != HFG_OK:
+            raise HfError(st, "basis construction failed")
+        v = BasisView()
+        _check(lib().hfg_basis_get_view(h, C.byref(v)))
+        n = v.n_functions
+
+        def arr(ptr, count, ct):
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(count,)).copy()
+
+        po = arr(v.prim_offset, n + 1, C.c_int32)
+        out = dict(atom_index=arr(v.atom_index, n, C.c_int32), center=arr(v.center, 3 * n, C.c_double),
+                   lxyz=arr(v.lxyz, 3 * n, C.c_int32), prim_offset=po,
+                   exponent=arr(v.exponent, int(po[-1]), C.c_double),
+                   coefficient=arr(v.coefficient, int(po[-1]), C.c_double))
+        lib().hfg_basis_free(h)
+        return cls(out)
+
+
+class Context:
+    """One molecule on one GPU: primitive pairs, Schwarz bounds, quartet lists."""
+
+    def __init__(self, basis):
+        self.basis = basis
+        self.h = C.c_void_p()
+        _check(lib().hfg_ctx_create(C.byref(basis.view), C.byref(self.h)))
+        self.n = basis.n
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value and _LIB is not None:
+            _LIB.hfg_ctx_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        v = [C.c_int64() for _ in range(4)]
+        _check(lib().hfg_ctx_info(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("n_functions", "n_shells", "n_pairs", "n_quartets"), (x.value for x in v)))
+
+    def schwarz(self):
+        q = np.zeros(self.info()["n_pairs"])
+        _check(lib().hfg_schwarz(self.h, _p(q)))
+        return q
+
+    def shell_quartet(self, a, b, c, d, dims):
+        out = np.zeros(int(np.prod(dims)))
+        _check(lib().hfg_shell_quartet(self.h, a, b, c, d, _p(out)))
+        return out.reshape(dims)
+
+    def eri_packed(self):
+        np_ = self.n * (self.n + 1) // 2
+        out = np.zeros(np_ * (np_ + 1) // 2)
+        _check(lib().hfg_eri_packed(self.h, _p(out), out.size))
+        return out
+
+    def contract_jk(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        j = np.zeros((self.n, self.n))
+        k = np.zeros((self.n, self.n))
+        _check(lib().hfg_contract_jk(self.h, _p(p), _p(j), _p(k)))
+        return j, k
+
+    def build_fock(self, h, p):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        f = np.zeros((self.n, self.n))
+        _check(lib().hfg_build_fock(self.h, _p(h), _p(p), _p(f)))
+        return f
+
+    def set_density(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        _check(lib().hfg_set_density(self.h, _p(p)))
+
+    def jk_resident(self):
+        """J/K of the resident density; results stay in HBM."""
+        _check(lib().hfg_jk_resident(self.h))
+
+    def get_jk(self):
+        j = np.zeros((self.n, self.n))
+        k = np.zeros((self.n, self.n))
+        _check(lib().hfg_get_jk(self.h, _p(j), _p(k)))
+        return j, k
+
+    def profile(self, on=True):
+        _check(lib().hfg_profile(self.h, int(on)))
+
+    def class_stats(self):
+        out = []
+        for c in range(21):
+            l4 = np.zeros(4, np.int32)
+            fl, ms = C.c_double(), C.c_double()
+            q, nl = C.c_int64(), C.c_int64()
+            _check(lib().hfg_class_stats(self.h, c, _p(l4), C.byref(fl), C.byref(q), C.byref(ms), C.byref(nl)))
+            out.append(dict(cls=c, l=tuple(int(x) for x in l4), flops=fl.value, quartets=q.value, ms=ms.value,
+                            launches=nl.value))
+        return out
+
+    def fock_flops(self):
+        f, pq = C.c_double(), C.c_double()
+        _check(lib().hfg_fock_flops(self.h, C.byref(f), C.byref(pq)))
+        return f.value, pq.value
+
+    def contract_jk_device(self, d_p, d_j, d_k):
+        """J/K on HBM-resident buffers (device pointers as ints, e.g. torch data_ptr())."""
+        _check(lib().hfg_contract_jk_device(self.h, C.c_void_p(d_p), C.c_void_p(d_j), C.c_void_p(d_k)))
+
+
+CONV = {"adiissoscf": 0, "default": 0, "cdiis": 1, "diis": 1, "pulay": 1, "adiis": 2, "adiiscdiis": 3}
+
+
+def scf_rhf(basis, zxyz, charge=0, max_iter=100, tol=1e-10, residual=None, density=None, conv="adiissoscf",
+            spherical=False, verbose=False, want_matrices=False):
+    """RHF with the derived tolerances of run_scf_in_space (src/scf.rs:384-400)."""
+    if residual is None:
+        residual = max(math.sqrt(tol) * 1e-2, 1e-11)
+    if density is None:
+        density = min(max(math.sqrt(tol) * 0.1, 1e-12), 1e-10)
+    key = conv.replace("-", "").replace("_", "").lower()
+    opt = ScfOptions(max_iter, tol, residual, density, CONV[key], int(spherical), int(verbose))
+    res = ScfResult()
+    zxyz = np.ascontiguousarray(zxyz, dtype=np.float64)
+    n = basis.n
+    p = np.zeros((n, n)) if want_matrices else None
+    f = np.zeros((n, n)) if want_matrices else None
+    eps = np.zeros(n) if want_matrices else None
+    st = lib().hfg_scf_rhf(C.byref(basis.view), _p(zxyz), len(zxyz), charge, C.byref(opt), C.byref(res),
+                           _p(p) if p is not None else None, _p(f) if f is not None else None,
+                           _p(eps) if eps is not None else None)
+    if st not in (HFG_OK, HFG_NOT_CONVERGED):
+        _check(st)
+    out = {k: getattr(res, k) for k, _ in ScfResult._fields_}
+    out["converged"] = bool(res.converged)
+    if want_matrices:
+        out.update(density=p, fock=f, orbital_energies=eps)
+    return out
+
+
+class Molecule:
+    """Small mirror of hf_rs::Molecule (src/api.rs:437-638) for the RHF path."""
+
+    def __init__(self, symbols, positions_bohr, basis="sto-3g", charge=0, spherical=False):
+        from . import molecules as M
+        self.mol = M.Molecule(symbols, positions_bohr, charge)
+        self.basis = Basis.from_g94(M.basis_text(basis), symbols, positions_bohr)
+        self.charge = charge
+        self.spherical = spherical
+
+    def scf(self, **kw):
+        kw.setdefault("spherical", self.spherical)
+        return scf_rhf(self.basis, self.mol.zxyz, charge=self.charge, **kw)
+
+
+def run(method, symbols, positions_bohr, basis="sto-3g", charge=0, spherical=False, **kw):
+    """Mirror of api::run (src/api.rs:1729-1738) for method "hf"/"rhf"."""
+    if method.lower() not in ("hf", "rhf", "scf"):
+        raise HfError(HFG_INVALID_INPUT, "only the RHF path is provided by this engine")
+    r = Molecule(symbols, positions_bohr, basis, charge, spherical).scf(**kw)
+    if not r["converged"]:
+        raise HfError(HFG_NOT_CONVERGED, "SCF did not converge")
+    return r

@@ -1,0 +1,304 @@
+// capi.cpp -- the extern "C" boundary declared in include/hfg.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/hfg.h"
+#include "engine.h"
+#include "orient.h"
+
+using namespace hfg;
+
+namespace {
+thread_local std::string g_err;
+int set_err(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+}  // namespace
+
+struct hfg_ctx {
+  std::unique_ptr<Engine> e;
+};
+
+#define GUARD(...)                                         \
+  try {                                                    \
+    __VA_ARGS__                                            \
+  } catch (HfgError & ex) {                                \
+    return set_err(ex.code, ex.msg);                       \
+  } catch (std::bad_alloc&) {                              \
+    return set_err(HFG_DEVICE, "host allocation failed");  \
+  } catch (std::exception & ex) {                          \
+    return set_err(HFG_INTEGRAL, ex.what());               \
+  }
+
+static int nc_of(const Engine& e, int pair) {
+  const PairInfo& p = e.pairs[pair];
+  return ((p.la + 1) * (p.la + 2) / 2) * ((p.lb + 1) * (p.lb + 2) / 2);
+}
+
+namespace hfg {
+int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
+            hfg_scf_result* res, double* p_out, double* f_out, double* eps_out);
+}
+
+extern "C" {
+
+const char* hfg_last_error(void) { return g_err.c_str(); }
+
+int hfg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int hfg_ctx_create(const hfg_basis_view* basis, hfg_ctx** out) {
+  if (!basis || !out) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({
+    auto* c = new hfg_ctx;
+    try {
+      c->e.reset(new Engine(*basis));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+    return HFG_OK;
+  })
+}
+
+void hfg_ctx_destroy(hfg_ctx* ctx) { delete ctx; }
+
+int hfg_ctx_info(const hfg_ctx* ctx, int64_t* nf, int64_t* ns, int64_t* np, int64_t* nq) {
+  if (!ctx) return set_err(HFG_INVALID_INPUT, "null context");
+  if (nf) *nf = ctx->e->n;
+  if (ns) *ns = (int64_t)ctx->e->shells.size();
+  if (np) *np = (int64_t)ctx->e->pairs.size();
+  if (nq) *nq = ctx->e->n_quartets;
+  return HFG_OK;
+}
+
+int hfg_schwarz(hfg_ctx* ctx, double* q) {
+  if (!ctx || !q) return set_err(HFG_INVALID_INPUT, "null argument");
+  std::memcpy(q, ctx->e->Q.data(), ctx->e->Q.size() * sizeof(double));
+  return HFG_OK;
+}
+
+int hfg_shell_quartet(hfg_ctx* ctx, int sa, int sb, int sc, int sd, double* out) {
+  if (!ctx || !out) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({
+    Engine& e = *ctx->e;
+    const int ns = (int)e.shells.size();
+    for (int s : {sa, sb, sc, sd})
+      if (s < 0 || s >= ns) return set_err(HFG_INVALID_INPUT, "one shell per quartet slot is required");
+    Orientation o = orient(e.shells, sa, sb, sc, sd);
+    auto pid = [](int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; };
+    // kernel-orientation pairs must match the stored orientation of each pair
+    int bra = pid(o.k[0], o.k[1]), ket = pid(o.k[2], o.k[3]);
+    const PairInfo& pb = e.pairs[bra];
+    const PairInfo& pk = e.pairs[ket];
+    // stored pair orientation puts the higher-L shell first; equal L keeps
+    // hf-rs order (a >= b).  Re-derive the slot mapping from it.
+    Orientation o2 = o;
+    o2.k[0] = pb.sh_a;
+    o2.k[1] = pb.sh_b;
+    o2.k[2] = pk.sh_a;
+    o2.k[3] = pk.sh_b;
+    if (o2.k[0] != o.k[0]) std::swap(o2.pos[0], o2.pos[1]);
+    if (o2.k[2] != o.k[2]) std::swap(o2.pos[2], o2.pos[3]);
+    size_t nt = 1;
+    for (int j = 0; j < 4; ++j) nt *= e.shells[o2.k[j]].scale.size();
+    std::vector<double> blk(nt);
+    e.blocks({BlockTask{bra, ket, 0}}, {o.cls}, blk);
+    unorient(e.shells, o2, blk.data(), out);
+    return HFG_OK;
+  })
+}
+
+int hfg_eri_packed(hfg_ctx* ctx, double* out, size_t len) {
+  if (!ctx || !out) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({
+    Engine& e = *ctx->e;
+    const size_t n = e.n, np = n * (n + 1) / 2;
+    if (len != np * (np + 1) / 2) return set_err(HFG_INVALID_INPUT, "packed length must be np(np+1)/2");
+    std::memset(out, 0, len * sizeof(double));
+    const size_t npair = e.pairs.size();
+    auto cidx = [](size_t i, size_t j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
+    // batches of surviving quartets ij >= kl (src/shell_integrals.rs:225-235)
+    size_t ij = 0, kl = 0;
+    const size_t budget = 1 << 24;  // doubles per batch
+    while (ij < npair) {
+      std::vector<BlockTask> tasks;
+      std::vector<int> cls;
+      std::vector<std::pair<int, int>> which;
+      long long off = 0;
+      while (ij < npair && (size_t)off < budget) {
+        if (e.Q[ij] * e.Q[kl] > e.screen) {
+          const PairInfo& pb = e.pairs[ij];
+          const PairInfo& pk = e.pairs[kl];
+          int pcb = pair_class(pb.la, pb.lb), pck = pair_class(pk.la, pk.lb);
+          int b = (int)ij, k = (int)kl;
+          if (pck > pcb) {
+            std::swap(b, k);
+            std::swap(pcb, pck);
+          }
+          tasks.push_back({b, k, off});
+          cls.push_back(quartet_class(pcb, pck));
+          which.push_back({b, k});
+          off += (long long)nc_of(e, b) * nc_of(e, k);
+        }
+        if (++kl > ij) {
+          ++ij;
+          kl = 0;
+        }
+      }
+      // group by class for the launches
+      std::vector<size_t> ord(tasks.size());
+      for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+      std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return cls[x] < cls[y]; });
+      std::vector<BlockTask> t2;
+      std::vector<int> c2;
+      for (size_t i : ord) {
+        t2.push_back(tasks[i]);
+        c2.push_back(cls[i]);
+      }
+      std::vector<double> blk(off);
+      e.blocks(t2, c2, blk);
+      for (const BlockTask& t : t2) {
+        const PairInfo& pb = e.pairs[t.bra];
+        const PairInfo& pk = e.pairs[t.ket];
+        const int na = (int)e.shells[pb.sh_a].scale.size(), nb = (int)e.shells[pb.sh_b].scale.size();
+        const int nc_ = (int)e.shells[pk.sh_a].scale.size(), nd = (int)e.shells[pk.sh_b].scale.size();
+        const double* v = blk.data() + t.off;
+        for (int i = 0; i < na; ++i)
+          for (int j = 0; j < nb; ++j) {
+            const size_t mu = pb.ao_a + i, nu = pb.ao_b + j;
+            const double fab = e.ao_scale[mu] * e.ao_scale[nu];
+            for (int k = 0; k < nc_; ++k)
+              for (int l = 0; l < nd; ++l) {
+                const size_t la = pk.ao_a + k, si = pk.ao_b + l;
+                out[cidx(cidx(mu, nu), cidx(la, si))] =
+                    fab * e.ao_scale[la] * e.ao_scale[si] * v[((i * nb + j) * nc_ + k) * nd + l];
+              }
+          }
+      }
+    }
+    return HFG_OK;
+  })
+}
+
+int hfg_contract_jk_device(hfg_ctx* ctx, const double* d_p, double* d_j, double* d_k) {
+  if (!ctx || !d_p) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({
+    ctx->e->jk_device(d_p, d_j, d_k);
+    ctx->e->sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_contract_jk(hfg_ctx* ctx, const double* p, double* j, double* k) {
+  if (!ctx || !p || !j || !k) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({
+    Engine& e = *ctx->e;
+    const size_t nn = (size_t)e.n * e.n;
+    for (size_t x = 0; x < nn; ++x)
+      if (!std::isfinite(p[x])) return set_err(HFG_INVALID_INPUT, "non-finite density");
+    HIP_CHECK(hipMemcpyAsync(e.d_D.p, p, nn * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    e.jk_device(e.d_D.as<double>(), e.d_J.as<double>(), e.d_K.as<double>());
+    HIP_CHECK(hipMemcpyAsync(j, e.d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(k, e.d_K.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_build_fock(hfg_ctx* ctx, const double* h, const double* p, double* f) {
+  if (!ctx || !h || !p || !f) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({
+    Engine& e = *ctx->e;
+    const size_t nn = (size_t)e.n * e.n;
+    HIP_CHECK(hipMemcpyAsync(e.d_D.p, p, nn * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(e.d_H.p, h, nn * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    e.jk_device(e.d_D.as<double>(), nullptr, nullptr, e.d_H.as<double>(), e.d_F.as<double>());
+    HIP_CHECK(hipMemcpyAsync(f, e.d_F.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_set_density(hfg_ctx* ctx, const double* p) {
+  if (!ctx || !p) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({
+    Engine& e = *ctx->e;
+    HIP_CHECK(hipMemcpyAsync(e.d_D.p, p, (size_t)e.n * e.n * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    e.sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_jk_resident(hfg_ctx* ctx) {
+  if (!ctx) return set_err(HFG_INVALID_INPUT, "null context");
+  GUARD({
+    Engine& e = *ctx->e;
+    e.jk_device(e.d_D.as<double>(), e.d_J.as<double>(), e.d_K.as<double>());
+    e.sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_get_jk(hfg_ctx* ctx, double* j, double* k) {
+  if (!ctx) return set_err(HFG_INVALID_INPUT, "null context");
+  GUARD({
+    Engine& e = *ctx->e;
+    const size_t nn = (size_t)e.n * e.n * sizeof(double);
+    if (j) HIP_CHECK(hipMemcpyAsync(j, e.d_J.p, nn, hipMemcpyDeviceToHost, e.stream));
+    if (k) HIP_CHECK(hipMemcpyAsync(k, e.d_K.p, nn, hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_profile(hfg_ctx* ctx, int enable) {
+  if (!ctx) return set_err(HFG_INVALID_INPUT, "null context");
+  GUARD({
+    ctx->e->set_profile(enable != 0);
+    return HFG_OK;
+  })
+}
+
+int hfg_class_stats(hfg_ctx* ctx, int cls, int32_t* l4, double* flops, int64_t* quartets, double* ms,
+                    int64_t* launches) {
+  if (!ctx || cls < 0 || cls >= HFG_NCLASS) return set_err(HFG_INVALID_INPUT, "class index out of range");
+  Engine& e = *ctx->e;
+  int l[4];
+  class_angular(cls, l);
+  if (l4)
+    for (int k = 0; k < 4; ++k) l4[k] = l[k];
+  if (flops) *flops = e.class_flops[cls];
+  if (quartets) *quartets = e.class_quartets[cls];
+  if (ms) *ms = e.class_ms[cls];
+  if (launches) *launches = e.class_launches[cls];
+  return HFG_OK;
+}
+
+int hfg_fock_flops(hfg_ctx* ctx, double* flops, double* prim_quartets) {
+  if (!ctx) return set_err(HFG_INVALID_INPUT, "null context");
+  double f = 0;
+  for (double x : ctx->e->class_flops) f += x;
+  if (flops) *flops = f;
+  if (prim_quartets) *prim_quartets = ctx->e->prim_quartets;
+  return HFG_OK;
+}
+
+double hfg_md_flops(int la, int lb, int lc, int ld) { return md_flops(la, lb, lc, ld); }
+
+int hfg_scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
+                hfg_scf_result* res, double* p_out, double* f_out, double* eps_out) {
+  if (!basis || !zxyz || !opt || !res) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({ return scf_rhf(basis, zxyz, natom, charge, opt, res, p_out, f_out, eps_out); })
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// engine.h -- per-molecule device state of the Fock-build engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "classes.h"
+#include "eri_device.h"
+#include "host_prep.h"
+
+namespace hfg {
+
+// Shell pair as the kernels read it (kernel orientation, see PairInfo).
+struct PairDev {
+  int p0, p1, ao_a, ao_b, sh_a, sh_b, pad0, pad1;
+  double AB[3], pad2;
+};
+
+// One wave of work: bra pair, first ket pair and number of ket pairs (<= 64);
+// all ket pairs share one primitive count.
+struct JKTask {
+  int bra, ket0, nket, pad;
+};
+
+struct BlockTask {
+  int bra, ket;
+  long long off;
+};
+
+#define HIP_CHECK(expr)                                                                           \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess)                                                                         \
+      throw ::hfg::HfgError{HFG_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)};     \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void alloc(size_t b) {
+    free();
+    if (b) HIP_CHECK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return (T*)p; }
+  ~DevBuf() { free(); }
+};
+
+double md_flops(int la, int lb, int lc, int ld);
+
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n = 0;
+  std::vector<Shell> shells;
+  std::vector<PairInfo> pairs;  // hf-rs pair order
+  PrimPairHost pp;
+  std::vector<double> Q;        // Schwarz bound per pair
+  std::vector<double> ao_scale; // per AO component factor
+  int64_t n_quartets = 0;
+  double screen = 1e-12;        // Q_ab*Q_cd > screen (src/shell_integrals.rs:184,226)
+
+  DevBuf d_pp, d_pairs, d_jkpairs, d_boys, d_scale, d_tasks, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
+  std::vector<size_t> task_begin;  // per class, into d_tasks (HFG_NCLASS+1 entries)
+  std::vector<double> class_flops;      // algorithmic FLOPs per launch
+  std::vector<int64_t> class_quartets;  // quartets per launch
+  double prim_quartets = 0;
+  bool profile = false;
+  std::vector<double> class_ms;
+  std::vector<int64_t> class_launches;
+  std::vector<hipEvent_t> events;  // 2 per class
+
+  explicit Engine(const hfg_basis_view& b);
+  ~Engine();
+  PrimPairs pp_view() const;
+
+  void compute_schwarz();
+  void build_tasks();
+  // Blocks of the listed quartets (kernel orientation, unscaled) into host out.
+  void blocks(const std::vector<BlockTask>& tasks, const std::vector<int>& cls, std::vector<double>& out);
+  // J/K from the device density d_D into d_J, d_K (and F = H + J - K/2 when H given).
+  void jk_device(const double* dP, double* dJ, double* dK, const double* dH = nullptr, double* dF = nullptr);
+  void sync();
+  void set_profile(bool on);
+};
+
+void class_angular(int c, int* l4);
+
+}  // namespace hfg

@@ -1,0 +1,121 @@
+// eri_device.h -- integral arithmetic shared by the HIP kernels.
+//
+// Compiled by hipcc for gfx950.  The same text also compiles as plain C++
+// (HFG_HOST_CHECK defined) so that the CPU test-suite can compare the
+// generated recurrences with the oracle before any GPU run; that host build is
+// a test artefact, never a fallback of the engine.
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+#ifdef HFG_HOST_CHECK
+#define HFG_DEV inline
+#else
+#include <hip/hip_runtime.h>
+#define HFG_DEV __host__ __device__ __forceinline__
+#endif
+
+namespace hfg {
+
+constexpr double TWO_PI_2_5 = 34.986836655249726164477;  // 2 pi^(5/2)
+constexpr double HALF_SQRT_PI = 0.88622692545275801365;
+
+// ---------------------------------------------------------------------------
+// Boys function F_m(T), m = 0..M.
+//
+// Table: rows T_r = r/8 (r = 0..BOYS_ROWS-1, T up to 50), each row holding
+// exp(-T_r) followed by F_0(T_r) .. F_{BOYS_ORDERS-1}(T_r).  F_M is a 9-term
+// Taylor expansion about the nearest row, d^k F_m/dT^k = (-1)^k F_{m+k}, with
+// |dT| <= 1/16 so the truncation is below F_M * (1/16)^9/9! ~ 5e-17 relative;
+// lower orders follow by the stable downward recursion
+// F_{m-1} = (2T F_m + e^{-T})/(2m-1), e^{-T} = e^{-T_r} e^{-dT}.  For T >= 50,
+// F_0 = sqrt(pi/T)/2 (erf(sqrt T) = 1 to 3e-24) and the upward recursion
+// F_m = ((2m-1) F_{m-1} - e^{-T})/(2T) is stable.
+// ---------------------------------------------------------------------------
+constexpr int BOYS_TAYLOR = 9;
+constexpr int BOYS_MAXM = 24;                       // highest order served
+constexpr int BOYS_ORDERS = BOYS_MAXM + BOYS_TAYLOR;  // F_0 .. F_32 stored
+constexpr int BOYS_ROW = BOYS_ORDERS + 1;           // + exp(-T_r)
+constexpr double BOYS_INV_STEP = 8.0;
+constexpr double BOYS_TMAX = 50.0;
+constexpr int BOYS_ROWS = 402;
+
+template <int M>
+HFG_DEV void boys_eval(double T, const double* __restrict__ tab, double* __restrict__ F) {
+  if (T < BOYS_TMAX) {
+    const int r = (int)(T * BOYS_INV_STEP + 0.5);
+    const double dT = T - r * (1.0 / BOYS_INV_STEP);
+    const double* row = tab + r * BOYS_ROW;
+    // F_M(T) = sum_k F_{M+k}(T_r) (-dT)^k / k!   (Horner)
+    double acc = row[1 + M + BOYS_TAYLOR - 1];
+#pragma unroll
+    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = row[1 + M + k - 1] + acc * (-dT) * (1.0 / k);
+    F[M] = acc;
+    if (M > 0) {
+      // e^{-dT} by its Taylor series, |dT| <= 1/16
+      double e = 1.0;
+#pragma unroll
+      for (int k = 9; k >= 1; --k) e = 1.0 + e * (-dT) * (1.0 / k);
+      const double emt = row[0] * e;
+      const double t2 = 2.0 * T;
+#pragma unroll
+      for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + emt) * (1.0 / (2 * m + 1));
+    }
+  } else {
+    const double inv_t = 1.0 / T;
+    F[0] = HALF_SQRT_PI * sqrt(inv_t);
+    if (M > 0) {
+      const double emt = exp(-T);
+      const double h = 0.5 * inv_t;
+#pragma unroll
+      for (int m = 1; m <= M; ++m) F[m] = ((2 * m - 1) * F[m - 1] - emt) * h;
+    }
+  }
+}
+
+// Runtime-order variant for the general path.
+HFG_DEV void boys_eval_rt(int M, double T, const double* __restrict__ tab, double* __restrict__ F) {
+  if (T < BOYS_TMAX) {
+    const int r = (int)(T * BOYS_INV_STEP + 0.5);
+    const double dT = T - r * (1.0 / BOYS_INV_STEP);
+    const double* row = tab + r * BOYS_ROW;
+    double acc = row[1 + M + BOYS_TAYLOR - 1];
+    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = row[1 + M + k - 1] + acc * (-dT) * (1.0 / k);
+    F[M] = acc;
+    double e = 1.0;
+    for (int k = 9; k >= 1; --k) e = 1.0 + e * (-dT) * (1.0 / k);
+    const double emt = row[0] * e, t2 = 2.0 * T;
+    for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + emt) / (2 * m + 1);
+  } else {
+    const double inv_t = 1.0 / T;
+    F[0] = HALF_SQRT_PI * sqrt(inv_t);
+    const double emt = exp(-T), h = 0.5 * inv_t;
+    for (int m = 1; m <= M; ++m) F[m] = ((2 * m - 1) * F[m - 1] - emt) * h;
+  }
+}
+
+// Primitive-pair data, structure of arrays.  For a pair of shells (A, B) in
+// kernel orientation (L_A >= L_B): p = alpha+beta, P = (alpha A + beta B)/p,
+// PA = P - A and K = c_A c_B exp(-alpha beta |AB|^2 / p) / p.
+struct PrimPairs {
+  const double* __restrict__ p;
+  const double* __restrict__ Px;
+  const double* __restrict__ Py;
+  const double* __restrict__ Pz;
+  const double* __restrict__ PAx;
+  const double* __restrict__ PAy;
+  const double* __restrict__ PAz;
+  const double* __restrict__ K;
+};
+
+template <int LA, int LB, int LC, int LD>
+HFG_DEV void eri_block(const PrimPairs& bp, int b0, int b1, const PrimPairs& kp, int k0, int k1, const double AB[3],
+                       const double CD[3], const double* __restrict__ boys_tab, double* __restrict__ I);
+
+constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
+
+}  // namespace hfg
+
+namespace hfg {
+#include "gen/eri_classes.inc"
+}

@@ -1,0 +1,271 @@
+"""Generate straight-line Head-Gordon-Pople ERI code for each shell-quartet class.
+
+For a class (la lb | lc ld) with la >= lb, lc >= ld the generated device
+function evaluates, per primitive quartet, the Obara-Saika vertical recurrence
+
+  [a+1_i|c]^m = PA_i [a|c]^m + WP_i [a|c]^{m+1}
+              + a_i/(2p) ([a-1_i|c]^m - q/(p+q) [a-1_i|c]^{m+1})          (c = 0)
+  [a|c+1_i]^m = QC_i [a|c]^m + WQ_i [a|c]^{m+1}
+              + c_i/(2q) ([a|c-1_i]^m - p/(p+q) [a|c-1_i]^{m+1})
+              + a_i/(2(p+q)) [a-1_i|c]^{m+1}
+
+from [0|0]^m = 2 pi^{5/2} K_ab K_cd / (p q sqrt(p+q)) F_m(T), accumulates the
+contracted [e0|f0] for la<=|e|<=la+lb and lc<=|f|<=lc+ld in registers, and after
+the primitive loop applies the horizontal recurrences
+
+  (a, b+1_i| = (a+1_i, b| + AB_i (a, b|        |c, d+1_i) = |c+1_i, d) + CD_i |c, d)
+
+to obtain the Cartesian block (ab|cd) in the hf-rs component order
+(src/basis.rs:170-188, 430-436).  The algorithm differs from the reference's
+McMurchie-Davidson evaluation (src/integrals.rs:259-346, 905-971) on purpose:
+primitive work is limited to the vertical step and the horizontal step runs
+once per contracted quartet.
+
+Usage: python gen_eri.py OUTDIR   (writes eri_classes.inc)
+"""
+import itertools
+import os
+import sys
+
+LMAX = 2
+
+
+def cart(L):
+    """Cartesian components in hf-rs order."""
+    if L == 2:
+        return [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)]
+    return [(lx, ly, L - lx - ly) for lx in range(L, -1, -1) for ly in range(L - lx, -1, -1)]
+
+
+def ncart(L):
+    return (L + 1) * (L + 2) // 2
+
+
+def sub(t, i, k=1):
+    u = list(t)
+    u[i] -= k
+    return tuple(u)
+
+
+def add(t, i, k=1):
+    u = list(t)
+    u[i] += k
+    return tuple(u)
+
+
+def tag(t):
+    return "%d%d%d" % t
+
+
+class Emitter:
+    def __init__(self):
+        self.lines = []
+        self.nflops = 0
+
+    def emit(self, s):
+        self.lines.append(s)
+
+
+def choose_axis(t, other):
+    """Axis to lower in t: a component that is 1 drops the second-order term;
+    ties broken towards the axis where `other` is zero (drops the coupling term)."""
+    best, score = None, None
+    for i in range(3):
+        if t[i] == 0:
+            continue
+        s = (t[i] - 1 > 0) * 2 + (other[i] > 0)
+        if score is None or s < score:
+            best, score = i, s
+    return best
+
+
+class VRR:
+    """Memoised generation of [a|c]^m nodes inside the primitive loop."""
+
+    def __init__(self, em):
+        self.em = em
+        self.memo = {}
+
+    def node(self, a, c, m):
+        key = (a, c, m)
+        if key in self.memo:
+            return self.memo[key]
+        if any(x < 0 for x in a) or any(x < 0 for x in c):
+            return None
+        em = self.em
+        name = "v%s_%s_%d" % (tag(a), tag(c), m)
+        if sum(a) == 0 and sum(c) == 0:
+            name = "F%d" % m
+            self.memo[key] = name
+            return name
+        terms = []
+        if sum(c) > 0:
+            i = choose_axis(c, a)
+            c1 = sub(c, i)
+            t0 = self.node(a, c1, m)
+            t1 = self.node(a, c1, m + 1)
+            ax = "xyz"[i]
+            terms.append("QC%s*%s" % (ax, t0))
+            terms.append("WQ%s*%s" % (ax, t1))
+            if c1[i] > 0:
+                c2 = sub(c1, i)
+                u0 = self.node(a, c2, m)
+                u1 = self.node(a, c2, m + 1)
+                terms.append("%d.0*oo2q*(%s - rq*%s)" % (c1[i], u0, u1))
+                em.nflops += 4
+            if a[i] > 0:
+                w = self.node(sub(a, i), c1, m + 1)
+                terms.append("%d.0*oo2s*%s" % (a[i], w))
+                em.nflops += 3
+        else:
+            i = choose_axis(a, c)
+            a1 = sub(a, i)
+            t0 = self.node(a1, c, m)
+            t1 = self.node(a1, c, m + 1)
+            ax = "xyz"[i]
+            terms.append("PA%s*%s" % (ax, t0))
+            terms.append("WP%s*%s" % (ax, t1))
+            if a1[i] > 0:
+                a2 = sub(a1, i)
+                u0 = self.node(a2, c, m)
+                u1 = self.node(a2, c, m + 1)
+                terms.append("%d.0*oo2p*(%s - rp*%s)" % (a1[i], u0, u1))
+                em.nflops += 4
+        em.nflops += 3
+        em.emit("      const double %s = %s;" % (name, " + ".join(terms)))
+        self.memo[key] = name
+        return name
+
+
+def levels(lo, hi):
+    out = []
+    for L in range(lo, hi + 1):
+        out += cart(L)
+    return out
+
+
+def gen_class(la, lb, lc, ld):
+    """Return C++ source of eri_<la><lb><lc><ld>(...)."""
+    M = la + lb + lc + ld
+    em = Emitter()
+    E = levels(la, la + lb)
+    Fk = levels(lc, lc + ld)
+    fname = "eri_%d%d%d%d" % (la, lb, lc, ld)
+    nab, ncd = ncart(la) * ncart(lb), ncart(lc) * ncart(ld)
+    out = []
+    out.append("// (%s%s|%s%s): %d contracted [e0|f0] accumulators" % ("spdfg"[la], "spdfg"[lb], "spdfg"[lc],
+                                                                        "spdfg"[ld], len(E) * len(Fk)))
+    out.append("template <> HFG_DEV void eri_block<%d,%d,%d,%d>(" % (la, lb, lc, ld))
+    out.append("    const PrimPairs& bp, int b0, int b1, const PrimPairs& kp, int k0, int k1,")
+    out.append("    const double AB[3], const double CD[3], const double* __restrict__ boys_tab, double* __restrict__ I) {")
+    acc = {}
+    for e in E:
+        for f in Fk:
+            acc[(e, f)] = "c%s_%s" % (tag(e), tag(f))
+    out.append("  double " + ", ".join("%s = 0.0" % v for v in acc.values()) + ";")
+    out.append("  for (int ib = b0; ib < b1; ++ib) {")
+    out.append("    const double p = bp.p[ib], Px = bp.Px[ib], Py = bp.Py[ib], Pz = bp.Pz[ib];")
+    out.append("    const double PAx = bp.PAx[ib], PAy = bp.PAy[ib], PAz = bp.PAz[ib], Kp = bp.K[ib];")
+    if la + lb > 0:
+        out.append("    const double oo2p = 0.5 / p;")
+    out.append("    for (int ik = k0; ik < k1; ++ik) {")
+    out.append("      const double q = kp.p[ik];")
+    out.append("      const double PQx = Px - kp.Px[ik], PQy = Py - kp.Py[ik], PQz = Pz - kp.Pz[ik];")
+    out.append("      const double inv_s = 1.0 / (p + q);")
+    out.append("      const double rho = p * q * inv_s;")
+    out.append("      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    out.append("      const double pref = TWO_PI_2_5 * Kp * kp.K[ik] * sqrt(inv_s);")
+    out.append("      double Fm[%d];" % (M + 1))
+    out.append("      boys_eval<%d>(T, boys_tab, Fm);" % M)
+    for m in range(M + 1):
+        out.append("      const double F%d = pref * Fm[%d];" % (m, m))
+    if M > 0:
+        if la + lb > 0:
+            out.append("      const double WPx = -q * inv_s * PQx, WPy = -q * inv_s * PQy, WPz = -q * inv_s * PQz;")
+            out.append("      const double rp = q * inv_s;")
+        if lc + ld > 0:
+            out.append("      const double QCx = kp.PAx[ik], QCy = kp.PAy[ik], QCz = kp.PAz[ik];")
+            out.append("      const double WQx = p * inv_s * PQx, WQy = p * inv_s * PQy, WQz = p * inv_s * PQz;")
+            out.append("      const double oo2q = 0.5 / q, rq = p * inv_s, oo2s = 0.5 * inv_s;")
+    vrr = VRR(em)
+    for e in E:
+        for f in Fk:
+            n = vrr.node(e, f, 0)
+            em.emit("      %s += %s;" % (acc[(e, f)], n))
+    out += em.lines
+    out.append("    }")
+    out.append("  }")
+    # Horizontal recurrences on the contracted values.
+    hl = []
+    memo = {}
+
+    def bra(a, b, f):
+        # (a b | f0)
+        key = ("b", a, b, f)
+        if key in memo:
+            return memo[key]
+        if sum(b) == 0:
+            memo[key] = acc[(a, f)]
+            return memo[key]
+        i = choose_axis(b, (0, 0, 0))
+        b1 = sub(b, i)
+        x = bra(add(a, i), b1, f)
+        y = bra(a, b1, f)
+        name = "hb%s_%s_%s" % (tag(a), tag(b), tag(f))
+        hl.append("  const double %s = %s + AB[%d] * %s;" % (name, x, i, y))
+        memo[key] = name
+        return name
+
+    def ket(a, b, c, d):
+        key = ("k", a, b, c, d)
+        if key in memo:
+            return memo[key]
+        if sum(d) == 0:
+            return bra(a, b, c)
+        i = choose_axis(d, (0, 0, 0))
+        d1 = sub(d, i)
+        x = ket(a, b, add(c, i), d1)
+        y = ket(a, b, c, d1)
+        name = "hk%s%s_%s_%s" % (tag(a), tag(b), tag(c), tag(d))
+        hl.append("  const double %s = %s + CD[%d] * %s;" % (name, x, i, y))
+        memo[key] = name
+        return name
+
+    idx = 0
+    for a in cart(la):
+        for b in cart(lb):
+            for c in cart(lc):
+                for d in cart(ld):
+                    v = ket(a, b, c, d)
+                    hl.append("  I[%d] = %s;" % (idx, v))
+                    idx += 1
+    out += hl
+    out.append("}")
+    return "\n".join(out) + "\n", em.nflops
+
+
+def classes():
+    pcs = [(a, b) for a in range(LMAX + 1) for b in range(a + 1)]
+    out = []
+    for i, (la, lb) in enumerate(pcs):
+        for (lc, ld) in pcs[:i + 1]:
+            out.append((la, lb, lc, ld))
+    return out
+
+
+def main(outdir):
+    os.makedirs(outdir, exist_ok=True)
+    parts = ["// Generated by gen_eri.py -- do not edit.\n"]
+    report = []
+    for c in classes():
+        src, fl = gen_class(*c)
+        parts.append(src)
+        report.append("%s %d" % ("".join(map(str, c)), fl))
+    with open(os.path.join(outdir, "eri_classes.inc"), "w") as f:
+        f.write("\n".join(parts))
+    with open(os.path.join(outdir, "eri_classes.txt"), "w") as f:
+        f.write("\n".join(report) + "\n")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(os.path.abspath(__file__)), "gen"))

@@ -1,0 +1,755 @@
+// scf.cpp -- restricted Hartree-Fock driver with device-resident state.
+//
+// Follows run_scf_in_space_with_convergence (src/scf.rs:410-627): SAD guess
+// (src/sad.rs), loop {F = H + J - K/2 on the GPU, E = 1/2 sum P(H+F),
+// e = FPS - SPF (projected by X when the orthogonaliser is rectangular),
+// converged when dE < tol, max|e| < residual and rms(dP) < density,
+// ADIIS/CDIIS extrapolation (src/diis.rs), eigh(X^T F X), P = 2 C_occ C_occ^T},
+// then a final diagonalisation of the true Fock matrix and a last Fock build.
+// F, P, S, H, X, the DIIS histories and the eigensolver live in HBM; FP64
+// GEMMs go to rocBLAS and the symmetric eigensolver to rocSOLVER (dsyevd).
+// Only scalars (energies, norms, DIIS inner products) come back to the host.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+
+#include "engine.h"
+#include "scf_host.h"
+
+namespace hfg {
+
+#define RB_CHECK(expr)                                                                   \
+  do {                                                                                   \
+    rocblas_status _s = (expr);                                                          \
+    if (_s != rocblas_status_success)                                                    \
+      throw HfgError{HFG_DEVICE, std::string(#expr) + ": " + rocblas_status_to_string(_s)}; \
+  } while (0)
+
+__global__ void k_antisym(const double* a, double* out, int n) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * n) return;
+  const int r = (int)(idx / n), c = (int)(idx % n);
+  out[idx] = a[idx] - a[(size_t)c * n + r];
+}
+__global__ void k_sub(const double* a, const double* b, double* out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] - b[i];
+}
+__global__ void k_zero(double* a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Small dense host algebra for the atomic (SAD) problems, n <= ~40.
+// ---------------------------------------------------------------------------
+typedef std::vector<double> HMat;
+static HMat hmm(const HMat& a, const HMat& b, int m, int k, int n, bool ta = false, bool tb = false) {
+  HMat c((size_t)m * n, 0.0);
+  for (int i = 0; i < m; ++i)
+    for (int p = 0; p < k; ++p) {
+      const double x = ta ? a[(size_t)p * m + i] : a[(size_t)i * k + p];
+      if (x == 0.0) continue;
+      for (int j = 0; j < n; ++j) c[(size_t)i * n + j] += x * (tb ? b[(size_t)j * k + p] : b[(size_t)p * n + j]);
+    }
+  return c;
+}
+// Cyclic Jacobi with ascending eigenvalues (vectors in columns).
+static void jacobi(int n, HMat a, std::vector<double>& w, HMat& v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0, tot = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) (i == j ? tot : off) += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+    if (off <= 1e-30 * (tot + off) || off < 1e-300) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double theta = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+          a[(size_t)k * n + p] = c * akp - s * akq;
+          a[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+          a[(size_t)p * n + k] = c * apk - s * aqk;
+          a[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+          v[(size_t)k * n + p] = c * vkp - s * vkq;
+          v[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return a[(size_t)x * n + x] < a[(size_t)y * n + y]; });
+  w.resize(n);
+  HMat v2((size_t)n * n);
+  for (int c = 0; c < n; ++c) {
+    w[c] = a[(size_t)ord[c] * n + ord[c]];
+    for (int r = 0; r < n; ++r) v2[(size_t)r * n + c] = v[(size_t)r * n + ord[c]];
+  }
+  v = v2;
+}
+static bool host_orthogonalizer(int n, const HMat& s, double cut, HMat& x, int& m) {
+  std::vector<double> w;
+  HMat v;
+  jacobi(n, s, w, v);
+  if (w[0] < -cut) return false;
+  std::vector<int> keep;
+  for (int i = 0; i < n; ++i)
+    if (w[i] >= cut) keep.push_back(i);
+  m = (int)keep.size();
+  if (!m) return false;
+  x.assign((size_t)n * m, 0.0);
+  for (int c = 0; c < m; ++c)
+    for (int r = 0; r < n; ++r) x[(size_t)r * m + c] = v[(size_t)r * n + keep[c]] / std::sqrt(w[keep[c]]);
+  return true;
+}
+static bool solve_small(HMat a, std::vector<double> b, int n, std::vector<double>& x) {
+  for (int i = 0; i < n; ++i) {
+    int piv = i;
+    for (int k = i + 1; k < n; ++k)
+      if (std::fabs(a[(size_t)k * n + i]) > std::fabs(a[(size_t)piv * n + i])) piv = k;
+    if (std::fabs(a[(size_t)piv * n + i]) < 1e-28) return false;
+    if (piv != i) {
+      std::swap(b[i], b[piv]);
+      for (int c = 0; c < n; ++c) std::swap(a[(size_t)i * n + c], a[(size_t)piv * n + c]);
+    }
+    for (int j = i + 1; j < n; ++j) {
+      const double f = a[(size_t)j * n + i] / a[(size_t)i * n + i];
+      for (int k = i; k < n; ++k) a[(size_t)j * n + k] -= f * a[(size_t)i * n + k];
+      b[j] -= f * b[i];
+    }
+  }
+  x.assign(n, 0.0);
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int k = i + 1; k < n; ++k) s -= a[(size_t)i * n + k] * x[k];
+    x[i] = s / a[(size_t)i * n + i];
+  }
+  return true;
+}
+
+// Projected-gradient minimisation of 1/2 c^T Q c + b^T c on the probability
+// simplex, fixed step 1/L with L the Gershgorin bound of sym(Q)
+// (src/diis.rs:195-266); projection by the sort-and-threshold rule.
+static void simplex_project(std::vector<double>& v) {
+  std::vector<double> u = v;
+  std::sort(u.begin(), u.end(), std::greater<double>());
+  double css = 0, theta = 0;
+  for (size_t k = 0; k < u.size(); ++k) {
+    css += u[k];
+    const double t = (css - 1.0) / (double)(k + 1);
+    if (u[k] - t > 0) theta = t;
+  }
+  for (double& x : v) x = std::max(x - theta, 0.0);
+}
+static std::vector<double> simplex_minimise(const HMat& q, const std::vector<double>& b) {
+  const int m = (int)b.size();
+  double lip = 1e-12;
+  for (int i = 0; i < m; ++i) {
+    double s = 0;
+    for (int j = 0; j < m; ++j) s += std::fabs(0.5 * (q[(size_t)i * m + j] + q[(size_t)j * m + i]));
+    lip = std::max(lip, s);
+  }
+  std::vector<double> c(m, 0.0), nx(m);
+  c[m - 1] = 1.0;
+  simplex_project(c);
+  for (int it = 0; it < 5000; ++it) {
+    for (int i = 0; i < m; ++i) {
+      double g = b[i];
+      for (int j = 0; j < m; ++j) g += 0.5 * (q[(size_t)i * m + j] + q[(size_t)j * m + i]) * c[j];
+      nx[i] = c[i] - g / lip;
+    }
+    simplex_project(nx);
+    double d = 0;
+    for (int i = 0; i < m; ++i) d = std::max(d, std::fabs(nx[i] - c[i]));
+    c = nx;
+    if (d < 1e-14) break;
+  }
+  return c;
+}
+
+// Fractional occupations with equal shares inside degenerate groups (|de| < 1e-6).
+static std::vector<double> frac_occ(const std::vector<double>& e, double nel) {
+  if (nel < 0 || nel > 2.0 * e.size() + 1e-9) throw HfgError{HFG_INVALID_INPUT, "atomic electron count exceeds basis capacity"};
+  std::vector<double> occ(e.size(), 0.0);
+  double left = nel;
+  size_t i = 0;
+  while (i < e.size() && left > 1e-12) {
+    size_t j = i + 1;
+    while (j < e.size() && std::fabs(e[j] - e[i]) < 1e-6) ++j;
+    const double pop = std::min(left, 2.0 * (j - i));
+    for (size_t k = i; k < j; ++k) occ[k] = pop / (j - i);
+    left -= pop;
+    i = j;
+  }
+  return occ;
+}
+
+// ---------------------------------------------------------------------------
+// SAD: fractional-occupation atomic RHF per element (src/sad.rs:56-134), with
+// the two-electron part from the GPU engine on the atom's own basis.
+// ---------------------------------------------------------------------------
+struct AtomicDensity {
+  HMat p;
+  double energy = 0;
+};
+
+static AtomicDensity atomic_hf(const std::vector<Shell>& sh_in, double z) {
+  std::vector<Shell> sh = sh_in;
+  int n = 0;
+  for (auto& s : sh) {
+    s.c[0] = s.c[1] = s.c[2] = 0.0;
+    s.atom = 0;
+    s.ao = n;
+    n += (int)s.scale.size();
+  }
+  std::vector<std::array<double, 4>> atoms{{z, 0, 0, 0}};
+  HMat S, T, V;
+  one_electron(sh, n, atoms, S, T, V);
+  HMat H(S.size());
+  for (size_t i = 0; i < H.size(); ++i) H[i] = T[i] + V[i];
+  HMat X;
+  int m;
+  if (!host_orthogonalizer(n, S, 1e-8, X, m)) throw HfgError{HFG_INTEGRAL, "atomic overlap is singular"};
+  // device engine over the atomic basis
+  std::vector<int32_t> atom(n, 0), lx(3 * n), po(n + 1);
+  std::vector<double> cen(3 * n, 0.0), ex, co;
+  int f = 0;
+  for (auto& s : sh) {
+    std::vector<std::array<int, 3>> cmp;
+    if (s.L == 2)
+      cmp = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    else
+      for (int a = s.L; a >= 0; --a)
+        for (int b = s.L - a; b >= 0; --b) cmp.push_back({a, b, s.L - a - b});
+    for (size_t c = 0; c < cmp.size(); ++c, ++f) {
+      for (int k = 0; k < 3; ++k) lx[3 * f + k] = cmp[c][k];
+      po[f] = (int32_t)ex.size();
+      for (int k = 0; k < s.K; ++k) {
+        ex.push_back(s.e[k]);
+        co.push_back(s.r[k] * s.scale[c]);
+      }
+    }
+  }
+  po[n] = (int32_t)ex.size();
+  hfg_basis_view view{n, atom.data(), cen.data(), lx.data(), po.data(), ex.data(), co.data()};
+  std::unique_ptr<Engine> eng;
+  const bool hydrogenic = std::fabs(z - 1.0) < 1e-12;
+  if (!hydrogenic) eng.reset(new Engine(view));
+  const size_t nn = (size_t)n * n;
+  auto fock = [&](const HMat& P) {
+    if (hydrogenic) return H;
+    HMat F(nn);
+    HIP_CHECK(hipMemcpyAsync(eng->d_D.p, P.data(), nn * 8, hipMemcpyHostToDevice, eng->stream));
+    HIP_CHECK(hipMemcpyAsync(eng->d_H.p, H.data(), nn * 8, hipMemcpyHostToDevice, eng->stream));
+    eng->jk_device(eng->d_D.as<double>(), nullptr, nullptr, eng->d_H.as<double>(), eng->d_F.as<double>());
+    HIP_CHECK(hipMemcpyAsync(F.data(), eng->d_F.p, nn * 8, hipMemcpyDeviceToHost, eng->stream));
+    eng->sync();
+    return F;
+  };
+  auto density = [&](const HMat& F, std::vector<double>& eps) {
+    HMat fp = hmm(hmm(X, F, m, n, n, true), X, m, n, m);
+    HMat cp;
+    jacobi(m, fp, eps, cp);
+    HMat C = hmm(X, cp, n, m, m);
+    auto occ = frac_occ(eps, z);
+    HMat w = C;
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < m; ++c) w[(size_t)r * m + c] *= std::sqrt(occ[c]);
+    return hmm(w, w, n, m, n, false, true);
+  };
+  auto commutator = [&](const HMat& F, const HMat& P) {
+    HMat a = hmm(hmm(F, P, n, n, n), S, n, n, n), b = hmm(hmm(S, P, n, n, n), F, n, n, n);
+    for (size_t i = 0; i < a.size(); ++i) a[i] -= b[i];
+    return a;
+  };
+  auto energy = [&](const HMat& P, const HMat& F) {
+    double e = 0;
+    for (size_t i = 0; i < nn; ++i) e += P[i] * (H[i] + F[i]);
+    return 0.5 * e;
+  };
+  std::vector<double> eps;
+  HMat P = density(H, eps);
+  double prev = INFINITY;
+  std::vector<HMat> fh, eh;
+  for (int it = 1; it <= 200; ++it) {
+    HMat F = fock(P);
+    const double e = energy(P, F);
+    HMat next = density(F, eps);
+    HMat err = commutator(F, P);
+    double emax = 0, rms = 0;
+    for (double v : err) emax = std::max(emax, std::fabs(v));
+    for (size_t i = 0; i < nn; ++i) rms += (P[i] - next[i]) * (P[i] - next[i]);
+    rms = std::sqrt(rms / nn);
+    if (std::fabs(e - prev) < 1e-11 && rms < 1e-9 && emax < 1e-8) {
+      HMat Fn = fock(next);
+      return {next, energy(next, Fn)};
+    }
+    prev = e;
+    if (fh.size() >= 8) {
+      fh.erase(fh.begin());
+      eh.erase(eh.begin());
+    }
+    fh.push_back(F);
+    eh.push_back(err);
+    HMat fu = F;
+    const int k = (int)eh.size();
+    if (k >= 2) {
+      HMat B((size_t)(k + 1) * (k + 1), 0.0);
+      for (int i = 0; i < k; ++i) {
+        for (int j = 0; j < k; ++j) {
+          double d = 0;
+          for (size_t x = 0; x < nn; ++x) d += eh[i][x] * eh[j][x];
+          B[(size_t)i * (k + 1) + j] = d;
+        }
+        B[(size_t)i * (k + 1) + k] = B[(size_t)k * (k + 1) + i] = -1.0;
+      }
+      std::vector<double> rhs(k + 1, 0.0), c;
+      rhs[k] = -1.0;
+      if (solve_small(B, rhs, k + 1, c)) {
+        fu.assign(nn, 0.0);
+        for (int i = 0; i < k; ++i)
+          for (size_t x = 0; x < nn; ++x) fu[x] += c[i] * fh[i][x];
+      }
+    }
+    next = density(fu, eps);
+    if (it <= 5)
+      for (size_t i = 0; i < nn; ++i) P[i] = 0.5 * P[i] + 0.5 * next[i];
+    else
+      P = next;
+  }
+  throw HfgError{HFG_NOT_CONVERGED, "fractional atomic HF did not converge"};
+}
+
+static std::string shell_key(const std::vector<Shell>& sh, double z) {
+  std::ostringstream os;
+  os.precision(17);
+  os << z;
+  for (auto& s : sh) {
+    os << "|" << s.L << ":" << s.K;
+    for (int k = 0; k < s.K; ++k) os << "," << s.e[k] << "," << s.r[k];
+    for (double c : s.scale) os << ";" << c;
+  }
+  return os.str();
+}
+
+static HMat sad_guess(const std::vector<Shell>& sh, int n, const std::vector<std::array<double, 4>>& atoms, int charge,
+                      const HMat& S) {
+  static std::mutex mu;
+  static std::map<std::string, AtomicDensity> cache;  // src/sad.rs:136-214
+  HMat P((size_t)n * n, 0.0);
+  double target = -charge;
+  for (auto& a : atoms) target += a[0];
+  for (size_t a = 0; a < atoms.size(); ++a) {
+    std::vector<Shell> loc;
+    for (auto& s : sh)
+      if (s.atom == (int)a) loc.push_back(s);
+    if (loc.empty()) throw HfgError{HFG_INVALID_INPUT, "atom without basis functions"};
+    const std::string key = shell_key(loc, atoms[a][0]);
+    AtomicDensity ad;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = cache.find(key);
+      if (it != cache.end()) ad = it->second;
+    }
+    if (ad.p.empty()) {
+      ad = atomic_hf(loc, atoms[a][0]);
+      std::lock_guard<std::mutex> lk(mu);
+      cache[key] = ad;
+    }
+    std::vector<int> idx;
+    for (auto& s : loc)
+      for (size_t c = 0; c < s.scale.size(); ++c) idx.push_back(s.ao + (int)c);
+    const int m = (int)idx.size();
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) P[(size_t)idx[i] * n + idx[j]] = ad.p[(size_t)i * m + j];
+  }
+  double el = 0;
+  for (size_t i = 0; i < P.size(); ++i) el += P[i] * S[i];
+  if (!(el > 1e-12)) throw HfgError{HFG_INTEGRAL, "zero SAD population"};
+  for (double& x : P) x *= target / el;
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+// Device linear algebra, row-major matrices on top of column-major rocBLAS.
+// ---------------------------------------------------------------------------
+struct Blas {
+  rocblas_handle h = nullptr;
+  hipStream_t s;
+  explicit Blas(hipStream_t st) : s(st) {
+    RB_CHECK(rocblas_create_handle(&h));
+    RB_CHECK(rocblas_set_stream(h, st));
+    RB_CHECK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+  }
+  ~Blas() {
+    if (h) rocblas_destroy_handle(h);
+  }
+  // C (MxN) = alpha op(A) op(B) + beta C, all row-major.
+  void gemm(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int lda, const double* B, int ldb,
+            double beta, double* C, int ldc) {
+    RB_CHECK(rocblas_dgemm(h, tb ? rocblas_operation_transpose : rocblas_operation_none,
+                           ta ? rocblas_operation_transpose : rocblas_operation_none, N, M, K, &alpha, B, ldb, A, lda,
+                           &beta, C, ldc));
+  }
+  double dot(size_t n, const double* x, const double* y) {
+    double r = 0;
+    RB_CHECK(rocblas_ddot(h, (rocblas_int)n, x, 1, y, 1, &r));
+    return r;
+  }
+  double nrm2(size_t n, const double* x) {
+    double r = 0;
+    RB_CHECK(rocblas_dnrm2(h, (rocblas_int)n, x, 1, &r));
+    return r;
+  }
+  double amax(size_t n, const double* x) {
+    rocblas_int i = 0;
+    RB_CHECK(rocblas_idamax(h, (rocblas_int)n, x, 1, &i));
+    double v = 0;
+    HIP_CHECK(hipMemcpyAsync(&v, x + (i - 1), 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return std::fabs(v);
+  }
+};
+
+struct Dev {
+  DevBuf b;
+  double* p = nullptr;
+  void alloc(size_t n) {
+    b.alloc(n * sizeof(double));
+    p = b.as<double>();
+  }
+};
+
+static void upload(double* d, const double* h, size_t n, hipStream_t s) {
+  HIP_CHECK(hipMemcpyAsync(d, h, n * 8, hipMemcpyHostToDevice, s));
+}
+static void download(double* h, const double* d, size_t n, hipStream_t s) {
+  HIP_CHECK(hipMemcpyAsync(h, d, n * 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Symmetric eigensolver: eigenvalues ascending into w, eigenvectors as
+// columns of the row-major result V (computed as V^T by rocSOLVER's
+// column-major output and transposed in the following GEMM by the caller).
+struct Eigh {
+  Dev work, e;
+  DevBuf info;
+  int n = 0;
+  void run(Blas& bl, double* a_colmajor_vecs, double* w, int m) {
+    if (n != m) {
+      e.alloc(m);
+      info.alloc(sizeof(rocblas_int));
+      n = m;
+    }
+    RB_CHECK(rocsolver_dsyevd(bl.h, rocblas_evect_original, rocblas_fill_lower, m, a_colmajor_vecs, m, w, e.p,
+                              info.as<rocblas_int>()));
+  }
+};
+
+int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
+            hfg_scf_result* res, double* p_out, double* f_out, double* eps_out) {
+  using clk = std::chrono::steady_clock;
+  const auto t_start = clk::now();
+  std::memset(res, 0, sizeof *res);
+  const double tol = opt->energy_tol;
+  if (opt->max_iter <= 0 || basis->n_functions <= 0 || !(tol > 0) || !(opt->residual_tol > 0) ||
+      !(opt->density_tol > 0))
+    throw HfgError{HFG_INVALID_INPUT, "SCF requires a nonempty basis, positive tolerance and iteration limit"};
+  std::vector<std::array<double, 4>> atoms(natom);
+  double zsum = 0;
+  for (int a = 0; a < natom; ++a) {
+    for (int k = 0; k < 4; ++k) atoms[a][k] = zxyz[4 * a + k];
+    zsum += atoms[a][0];
+  }
+  const int nel = (int)std::lround(zsum) - charge;
+  if (nel <= 0) throw HfgError{HFG_INVALID_INPUT, "invalid electron count"};
+  if (nel % 2) throw HfgError{HFG_INVALID_INPUT, "RHF requires an even number of electrons; use UHF for open shells"};
+  const int nocc = nel / 2;
+  const int n = basis->n_functions;
+  if (nocc > n) throw HfgError{HFG_INVALID_INPUT, "more occupied orbitals than basis functions"};
+
+  Engine eng(*basis);
+  hipStream_t st = eng.stream;
+  Blas bl(st);
+  const size_t nn = (size_t)n * n;
+  double enuc = 0;
+  for (int i = 0; i < natom; ++i)
+    for (int j = i + 1; j < natom; ++j) {
+      const double dx = atoms[i][1] - atoms[j][1], dy = atoms[i][2] - atoms[j][2], dz = atoms[i][3] - atoms[j][3];
+      enuc += atoms[i][0] * atoms[j][0] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  HMat S, T, V;
+  one_electron(eng.shells, n, atoms, S, T, V);
+  HMat H(nn);
+  for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
+
+  // Orthogonaliser X = Tr * U_kept s^{-1/2} of S' = Tr^T S Tr (src/ao.rs:58-67).
+  HMat Tr;
+  int m0 = n;
+  if (opt->spherical) spherical_transform(eng.shells, n, Tr, m0);
+  Dev dS, dH, dX, dTmp, dTmp2, dF, dP, dPprev, dErr, dW, dC, dFu;
+  dS.alloc(nn);
+  dH.alloc(nn);
+  dTmp.alloc(nn);
+  dTmp2.alloc(nn);
+  dF.alloc(nn);
+  dP.alloc(nn);
+  dPprev.alloc(nn);
+  dErr.alloc(nn);
+  dW.alloc(n);
+  dC.alloc(nn);
+  dFu.alloc(nn);
+  upload(dS.p, S.data(), nn, st);
+  upload(dH.p, H.data(), nn, st);
+  Eigh eig;
+  int m = 0;
+  {
+    // S' (m0 x m0) on the device
+    Dev dTr, dSp;
+    dSp.alloc((size_t)m0 * m0);
+    if (opt->spherical) {
+      dTr.alloc((size_t)n * m0);
+      upload(dTr.p, Tr.data(), (size_t)n * m0, st);
+      bl.gemm(false, false, n, m0, n, 1.0, dS.p, n, dTr.p, m0, 0.0, dTmp.p, m0);        // S Tr
+      bl.gemm(true, false, m0, m0, n, 1.0, dTr.p, m0, dTmp.p, m0, 0.0, dSp.p, m0);      // Tr^T S Tr
+    } else {
+      HIP_CHECK(hipMemcpyAsync(dSp.p, dS.p, nn * 8, hipMemcpyDeviceToDevice, st));
+    }
+    Dev dw;
+    dw.alloc(m0);
+    eig.run(bl, dSp.p, dw.p, m0);
+    std::vector<double> w(m0);
+    download(w.data(), dw.p, m0, st);
+    if (w[0] < -1e-8) throw HfgError{HFG_INTEGRAL, "overlap has negative eigenvalue"};
+    int first = 0;
+    while (first < m0 && w[first] < 1e-8) ++first;
+    m = m0 - first;
+    if (m <= 0) throw HfgError{HFG_INTEGRAL, "overlap has zero retained rank"};
+    if (nocc > m) throw HfgError{HFG_INVALID_INPUT, "more occupied orbitals than retained overlap rank"};
+    // U (row-major m0 x m0, columns = vectors) is the transpose of dSp's row-major view.
+    HMat U((size_t)m0 * m0);
+    download(U.data(), dSp.p, (size_t)m0 * m0, st);
+    HMat Xp((size_t)m0 * m);
+    for (int c = 0; c < m; ++c)
+      for (int r = 0; r < m0; ++r) Xp[(size_t)r * m + c] = U[(size_t)(first + c) * m0 + r] / std::sqrt(w[first + c]);
+    HMat X = opt->spherical ? hmm(Tr, Xp, n, m0, m) : Xp;
+    dX.alloc((size_t)n * m);
+    upload(dX.p, X.data(), (size_t)n * m, st);
+  }
+  const int tb = 256;
+  auto grid = [&](size_t cnt) { return dim3((unsigned)((cnt + tb - 1) / tb)); };
+  double t_fock = 0;
+  int builds = 0;
+  auto fock = [&](const double* P, double* F) {
+    const auto t0 = clk::now();
+    eng.jk_device(P, nullptr, nullptr, dH.p, F);
+    HIP_CHECK(hipStreamSynchronize(st));
+    t_fock += std::chrono::duration<double>(clk::now() - t0).count();
+    ++builds;
+  };
+  // diagonalise X^T F X, C = X C', P = 2 C_occ C_occ^T; eigenvalues into eps
+  std::vector<double> eps(m);
+  auto diag_density = [&](const double* F, double* P, bool want_eps) {
+    bl.gemm(false, false, n, m, n, 1.0, F, n, dX.p, m, 0.0, dTmp.p, m);     // F X
+    bl.gemm(true, false, m, m, n, 1.0, dX.p, m, dTmp.p, m, 0.0, dTmp2.p, m);  // X^T F X
+    eig.run(bl, dTmp2.p, dW.p, m);
+    // C = X V where the row-major view of dTmp2 is V^T
+    bl.gemm(false, true, n, m, m, 1.0, dX.p, m, dTmp2.p, m, 0.0, dC.p, m);
+    bl.gemm(false, true, n, n, nocc, 2.0, dC.p, m, dC.p, m, 0.0, P, n);
+    if (want_eps) download(eps.data(), dW.p, m, st);
+  };
+  auto energy = [&](const double* P, const double* F) { return 0.5 * (bl.dot(nn, P, dH.p) + bl.dot(nn, P, F)); };
+
+  // Initial density: SAD, else core Hamiltonian (src/scf.rs:452-471).  The
+  // reference also diagonalises the SAD Fock matrix for the orbital-rotation
+  // accelerators only; the Fock-extrapolation family used here never reads it.
+  bool have_sad = true;
+  HMat P0;
+  try {
+    P0 = sad_guess(eng.shells, n, atoms, charge, S);
+  } catch (HfgError&) {
+    have_sad = false;
+  }
+  if (have_sad)
+    upload(dP.p, P0.data(), nn, st);
+  else
+    diag_density(dH.p, dP.p, false);
+
+  // DIIS histories (device), src/diis.rs:72-137 and 289-359
+  const int MAXS = 8;
+  const int mm2 = m * m;
+  Dev cF, cE, aF, aP, dG, dCoef;
+  cF.alloc(MAXS * nn);
+  cE.alloc((size_t)MAXS * mm2);
+  aF.alloc(MAXS * nn);
+  aP.alloc(MAXS * nn);
+  dG.alloc(MAXS * MAXS);
+  dCoef.alloc(MAXS);
+  hipLaunchKernelGGL(k_zero, grid(MAXS * nn), dim3(tb), 0, st, cF.p, MAXS * nn);
+  hipLaunchKernelGGL(k_zero, grid(MAXS * nn), dim3(tb), 0, st, aF.p, MAXS * nn);
+  hipLaunchKernelGGL(k_zero, grid(MAXS * nn), dim3(tb), 0, st, aP.p, MAXS * nn);
+  std::vector<int> cslot, aslot;  // logical order -> slot
+  std::vector<double> aE;
+  std::map<std::pair<int, int>, double> edot;  // slot-pair error inner products
+  auto take_slot = [&](std::vector<int>& order) {
+    int s;
+    if ((int)order.size() >= MAXS) {
+      s = order.front();
+      order.erase(order.begin());
+    } else {
+      s = (int)order.size();
+      for (int c = 0; c < MAXS; ++c)
+        if (std::find(order.begin(), order.end(), c) == order.end()) {
+          s = c;
+          break;
+        }
+    }
+    order.push_back(s);
+    return s;
+  };
+  auto combine = [&](const Dev& hist, const std::vector<int>& order, const std::vector<double>& c, double* out) {
+    std::vector<double> cs(MAXS, 0.0);
+    for (size_t i = 0; i < order.size(); ++i) cs[order[i]] = c[i];
+    upload(dCoef.p, cs.data(), MAXS, st);
+    const double one = 1.0, zero = 0.0;
+    RB_CHECK(rocblas_dgemv(bl.h, rocblas_operation_none, (rocblas_int)nn, MAXS, &one, hist.p, (rocblas_int)nn,
+                           dCoef.p, 1, &zero, out, 1));
+  };
+
+  double prev_e = 0;
+  bool conv = false;
+  int iters = 0;
+  HIP_CHECK(hipMemcpyAsync(dPprev.p, dP.p, nn * 8, hipMemcpyDeviceToDevice, st));
+  for (int it = 1; it <= opt->max_iter; ++it) {
+    iters = it;
+    fock(dP.p, dF.p);
+    const double tot = energy(dP.p, dF.p) + enuc;
+    const double de = std::fabs(tot - prev_e);
+    prev_e = tot;
+    // e = FPS - SPF = FPS - (FPS)^T
+    bl.gemm(false, false, n, n, n, 1.0, dF.p, n, dP.p, n, 0.0, dTmp.p, n);
+    bl.gemm(false, false, n, n, n, 1.0, dTmp.p, n, dS.p, n, 0.0, dTmp2.p, n);
+    hipLaunchKernelGGL(k_antisym, grid(nn), dim3(tb), 0, st, dTmp2.p, dErr.p, n);
+    const double* err = dErr.p;
+    size_t elen = nn;
+    if (m < n) {
+      bl.gemm(false, false, n, m, n, 1.0, dErr.p, n, dX.p, m, 0.0, dTmp.p, m);
+      bl.gemm(true, false, m, m, n, 1.0, dX.p, m, dTmp.p, m, 0.0, dTmp2.p, m);
+      err = dTmp2.p;
+      elen = mm2;
+    }
+    const double en = bl.amax(elen, err);
+    hipLaunchKernelGGL(k_sub, grid(nn), dim3(tb), 0, st, dP.p, dPprev.p, dTmp.p, nn);
+    const double rms = bl.nrm2(nn, dTmp.p) / std::sqrt((double)nn);
+    if (opt->verbose) fprintf(stderr, "  hfg iter %3d  E=%.12f  dE=%.3e  |e|=%.3e\n", it, tot, de, en);
+    if (it > 1 && de < tol && en < opt->residual_tol && rms < opt->density_tol) {
+      conv = true;
+      break;
+    }
+    HIP_CHECK(hipMemcpyAsync(dPprev.p, dP.p, nn * 8, hipMemcpyDeviceToDevice, st));
+    bool use_cdiis;
+    if (opt->conv_algo == 1)
+      use_cdiis = true;
+    else if (opt->conv_algo == 2)
+      use_cdiis = false;
+    else
+      use_cdiis = !(en > (opt->conv_algo == 3 ? 0.08 : 0.05) || it <= 2);
+    bool ok = false;
+    if (use_cdiis) {
+      const int s = take_slot(cslot);
+      HIP_CHECK(hipMemcpyAsync(cF.p + s * nn, dF.p, nn * 8, hipMemcpyDeviceToDevice, st));
+      HIP_CHECK(hipMemcpyAsync(cE.p + (size_t)s * mm2, err, elen * 8, hipMemcpyDeviceToDevice, st));
+      for (auto it2 = edot.begin(); it2 != edot.end();)
+        if (it2->first.first == s || it2->first.second == s)
+          it2 = edot.erase(it2);
+        else
+          ++it2;
+      for (int o : cslot) edot[{std::min(s, o), std::max(s, o)}] = bl.dot(elen, cE.p + (size_t)s * mm2, cE.p + (size_t)o * mm2);
+      const int k = (int)cslot.size();
+      if (k >= 2) {
+        HMat B((size_t)(k + 1) * (k + 1), 0.0);
+        for (int i = 0; i < k; ++i) {
+          for (int j = 0; j < k; ++j)
+            B[(size_t)i * (k + 1) + j] = edot[{std::min(cslot[i], cslot[j]), std::max(cslot[i], cslot[j])}];
+          B[(size_t)i * (k + 1) + k] = B[(size_t)k * (k + 1) + i] = -1.0;
+        }
+        std::vector<double> rhs(k + 1, 0.0), c;
+        rhs[k] = -1.0;
+        if (solve_small(B, rhs, k + 1, c)) {
+          c.resize(k);
+          combine(cF, cslot, c, dFu.p);
+          ok = true;
+        }
+      }
+    } else {
+      const int s = take_slot(aslot);
+      if ((int)aE.size() >= MAXS) aE.erase(aE.begin());
+      aE.push_back(tot);
+      HIP_CHECK(hipMemcpyAsync(aF.p + s * nn, dF.p, nn * 8, hipMemcpyDeviceToDevice, st));
+      HIP_CHECK(hipMemcpyAsync(aP.p + s * nn, dP.p, nn * 8, hipMemcpyDeviceToDevice, st));
+      const int k = (int)aslot.size();
+      if (k >= 2) {
+        // G[s][t] = P_s . F_t over all slots; the column-major MAXS x MAXS
+        // result of P^T F is G^T in row-major terms.
+        const double one = 1.0, zero = 0.0;
+        RB_CHECK(rocblas_dgemm(bl.h, rocblas_operation_transpose, rocblas_operation_none, MAXS, MAXS, (rocblas_int)nn,
+                               &one, aP.p, (rocblas_int)nn, aF.p, (rocblas_int)nn, &zero, dG.p, MAXS));
+        std::vector<double> G(MAXS * MAXS);
+        download(G.data(), dG.p, MAXS * MAXS, st);
+        auto g = [&](int i, int j) { return G[(size_t)aslot[j] * MAXS + aslot[i]]; };  // P_i . F_j
+        const int last = k - 1;
+        HMat Q((size_t)k * k);
+        std::vector<double> b(k);
+        for (int i = 0; i < k; ++i) {
+          b[i] = g(i, last) - g(last, last);
+          for (int j = 0; j < k; ++j) Q[(size_t)i * k + j] = g(i, j) - g(i, last) - g(last, j) + g(last, last);
+        }
+        auto c = simplex_minimise(Q, b);
+        combine(aF, aslot, c, dFu.p);
+        ok = true;
+      }
+    }
+    diag_density(ok ? dFu.p : dF.p, dP.p, false);
+  }
+  if (!conv) fock(dP.p, dF.p);
+  diag_density(dF.p, dP.p, true);
+  fock(dP.p, dF.p);
+  const double fe = energy(dP.p, dF.p);
+  res->total_energy = fe + enuc;
+  res->electronic_energy = fe;
+  res->nuclear_repulsion = enuc;
+  res->iterations = iters;
+  res->converged = conv;
+  res->n_occ = nocc;
+  res->fock_builds = builds;
+  res->seconds_fock = t_fock;
+  if (p_out) download(p_out, dP.p, nn, st);
+  if (f_out) download(f_out, dF.p, nn, st);
+  if (eps_out) std::memcpy(eps_out, eps.data(), m * sizeof(double));
+  HIP_CHECK(hipStreamSynchronize(st));
+  res->seconds_total = std::chrono::duration<double>(clk::now() - t_start).count();
+  return conv ? HFG_OK : HFG_NOT_CONVERGED;
+}
+
+}  // namespace hfg

@@ -1,0 +1,143 @@
+/* hfg.h -- C ABI of the MI355X Fock-build engine for hf-rs.
+ *
+ * Plain C types only: caller-owned, row-major FP64 buffers, int status codes
+ * (0 = success) and a thread-local message in hfg_last_error().  One context
+ * holds one basis on one GPU, with its own HIP stream; calls on one context
+ * are not reentrant, calls on different contexts are.
+ *
+ * Each entry point names the hf-rs function (Rust, src/...) whose body it
+ * replaces when hf-rs is built against this library.
+ */
+#ifndef HFG_H
+#define HFG_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, mirroring hf-rs's HfError variants (src/error.rs:5-15). */
+enum {
+  HFG_OK = 0,
+  HFG_INVALID_INPUT = 1, /* HfError::InvalidInput */
+  HFG_INTEGRAL = 2,      /* HfError::Integral (numerical failure) */
+  HFG_DEVICE = 3,        /* HIP runtime / device failure */
+  HFG_NOT_CONVERGED = 4, /* SCF did not converge (src/scf.rs:600-604) */
+  HFG_UNSUPPORTED = 5    /* angular momentum beyond the compiled classes */
+};
+
+/* A basis as hf-rs holds it: one row per Cartesian ContractedGaussian
+ * (src/basis.rs:89-98), shells contiguous in hf-rs component order
+ * (src/basis.rs:170-188, 430-436), coefficients already normalised. */
+typedef struct {
+  int32_t n_functions;
+  const int32_t* atom_index;  /* [n] */
+  const double* center;       /* [3n] bohr */
+  const int32_t* lxyz;        /* [3n] */
+  const int32_t* prim_offset; /* [n+1] */
+  const double* exponent;     /* [prim_offset[n]] */
+  const double* coefficient;  /* [prim_offset[n]] */
+} hfg_basis_view;
+
+typedef struct hfg_ctx hfg_ctx;
+
+/* Message of the last failing call on this thread. */
+const char* hfg_last_error(void);
+/* Number of visible GPUs (0 when none); never allocates device memory. */
+int hfg_device_count(void);
+
+/* Build the per-molecule state: shells, primitive-pair data in HBM, Schwarz
+ * bounds and the screened shell-quartet work lists.  Replaces the shell-pair
+ * and bound set-up of compute_shell_eri_tensor (src/shell_integrals.rs:156-173);
+ * the shell grouping and its errors follow shell_ranges (src/ao.rs:133-164). */
+int hfg_ctx_create(const hfg_basis_view* basis, hfg_ctx** out);
+void hfg_ctx_destroy(hfg_ctx* ctx);
+
+/* Sizes: functions, shells, shell pairs, surviving unique shell quartets. */
+int hfg_ctx_info(const hfg_ctx* ctx, int64_t* n_functions, int64_t* n_shells, int64_t* n_pairs,
+                 int64_t* n_quartets);
+/* Schwarz bounds Q_ab = sqrt(max (ab|ab)) in hf-rs pair order a>=b
+ * (src/shell_integrals.rs:165-173); length n_shells*(n_shells+1)/2. */
+int hfg_schwarz(hfg_ctx* ctx, double* q);
+
+/* One contracted Cartesian block [a][b][c][d] of shells (sa,sb,sc,sd).
+ * Replaces compute_shell_quartet (src/shell_integrals.rs:138-151). */
+int hfg_shell_quartet(hfg_ctx* ctx, int sa, int sb, int sc, int sd, double* out);
+
+/* Packed 8-fold screened tensor of length np(np+1)/2, np = n(n+1)/2, indexed
+ * by eri_index (src/scf.rs:221-235).  Replaces compute_eri_tensor
+ * (src/scf.rs:275-278) for callers that keep HFResult.eri. */
+int hfg_eri_packed(hfg_ctx* ctx, double* out, size_t len);
+
+/* Direct J and K for a symmetric total density P (n x n), over the unique
+ * shell quartets with Q_ab*Q_cd > 1e-12.  Replaces contract_jk
+ * (src/scf.rs:337-364) with the packed tensor never formed. */
+int hfg_contract_jk(hfg_ctx* ctx, const double* p, double* j, double* k);
+/* F = H + J - K/2.  Replaces build_fock_matrix (src/scf.rs:366-369). */
+int hfg_build_fock(hfg_ctx* ctx, const double* h, const double* p, double* f);
+/* Same as hfg_contract_jk on device pointers (HBM-resident P, J, K), on the
+ * context's stream; returns after the results are complete. */
+int hfg_contract_jk_device(hfg_ctx* ctx, const double* d_p, double* d_j, double* d_k);
+
+/* Resident-buffer form used by throughput measurements: the density is
+ * uploaded once, J and K stay in HBM until fetched.  (Same arithmetic as
+ * hfg_contract_jk; no reference counterpart: hf-rs has no device memory.) */
+int hfg_set_density(hfg_ctx* ctx, const double* p);
+int hfg_jk_resident(hfg_ctx* ctx);
+int hfg_get_jk(hfg_ctx* ctx, double* j, double* k);
+
+/* Instrumentation (no reference counterpart; hf-rs has no profiler hooks,
+ * src/scf.rs:486-604 only prints an iteration table).  hfg_profile(1)
+ * brackets every shell-quartet class launch with HIP events on the context's
+ * stream and accumulates their durations; hfg_class_stats reports, per class
+ * index 0..20, the angular momenta (la lb lc ld, kernel orientation), the
+ * algorithmic FLOP count of one launch (McMurchie-Davidson model: sum over the
+ * class's surviving quartets of Ka*Kb*Kc*Kd*flops(la,lb,lc,ld)), quartets per
+ * launch, total measured milliseconds and number of launches. */
+int hfg_profile(hfg_ctx* ctx, int enable);
+int hfg_class_stats(hfg_ctx* ctx, int cls, int32_t* l4, double* flops_per_launch, int64_t* quartets,
+                    double* ms_total, int64_t* launches);
+/* Algorithmic FLOPs of one Fock build (sum over classes) and per primitive
+ * quartet of class (la lb|lc ld): 20 + 8 + 16(L+1) + 2(L+1) + 3(C(L+4,4)-(L+1))
+ * + sum over Cartesian component quartets of 2 sum_ax (lab+1)(lcd+1)
+ * + 3 prod_ax (lab+lcd+1) + 3, the loop counts of src/shell_integrals.rs:101-133
+ * and src/integrals.rs:70-101, 259-346, 905-971. */
+int hfg_fock_flops(hfg_ctx* ctx, double* flops, double* prim_quartets);
+double hfg_md_flops(int la, int lb, int lc, int ld);
+
+/* RHF options and result (ScfOptions, src/api.rs:323-410; HFResult,
+ * src/scf.rs:14-33). */
+typedef struct {
+  int32_t max_iter;
+  double energy_tol, residual_tol, density_tol;
+  int32_t conv_algo; /* 0 ADIIS->CDIIS (default), 1 CDIIS, 2 ADIIS, 3 ADIIS->CDIIS at 0.08 */
+  int32_t spherical; /* BasisKind::Spherical for the orthogonaliser */
+  int32_t verbose;
+} hfg_scf_options;
+
+typedef struct {
+  double total_energy, electronic_energy, nuclear_repulsion;
+  int32_t iterations, converged, n_occ, fock_builds;
+  double seconds_total, seconds_fock;
+} hfg_scf_result;
+
+/* Restricted Hartree-Fock with device-resident Fock builds.  zxyz: per atom
+ * (nuclear charge, x, y, z) in bohr.  Optional outputs P, F (n x n) and
+ * orbital energies may be NULL.  Replaces run_scf_in_space_with_convergence
+ * (src/scf.rs:410-627). */
+int hfg_scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge,
+                const hfg_scf_options* opt, hfg_scf_result* res, double* p_out, double* f_out,
+                double* eps_out);
+
+/* Host-side basis construction from Gaussian-94 text, mirroring build_basis
+ * (src/basis.rs:296-496).  The arrays are owned by the returned handle. */
+typedef struct hfg_basis hfg_basis;
+int hfg_basis_from_g94(const char* g94, const char* symbols, const double* xyz, int natom,
+                       hfg_basis** out);
+void hfg_basis_free(hfg_basis* b);
+int hfg_basis_get_view(const hfg_basis* b, hfg_basis_view* view);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

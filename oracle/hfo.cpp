@@ -2105,7 +2105,8 @@ double hfo_md_flops(int la, int lb, int lc, int ld) {
   return f;
 }
 
-int hfo_fock_flops(const hfo_basis* b, double* flops, double* quartets, double* prim_quartets) {
+int hfo_fock_flops_sample(const hfo_basis* b, int stride, int offset, double* flops, double* quartets,
+                          double* prim_quartets) {
   PairSet ps;
   build_pairs(*b, ps);
   long np = (long)ps.pairs.size();
@@ -2118,6 +2119,7 @@ int hfo_fock_flops(const hfo_basis* b, double* flops, double* quartets, double* 
           if (la + lb + lc + ld <= 8) memo[{la, lb, lc, ld}] = hfo_md_flops(la, lb, lc, ld);
 #pragma omp parallel for reduction(+ : fl, nq, npq) schedule(dynamic)
   for (long ij = 0; ij < np; ++ij) {
+    if (stride > 1 && ij % stride != offset) continue;
     const ShellPair& ab = ps.pairs[ij];
     for (long kl = 0; kl <= ij; ++kl) {
       if (!(ps.bound[ij] * ps.bound[kl] > 1e-12)) continue;
@@ -2134,6 +2136,10 @@ int hfo_fock_flops(const hfo_basis* b, double* flops, double* quartets, double* 
   if (quartets) *quartets = nq;
   if (prim_quartets) *prim_quartets = npq;
   return 0;
+}
+
+int hfo_fock_flops(const hfo_basis* b, double* flops, double* quartets, double* prim_quartets) {
+  return hfo_fock_flops_sample(b, 1, 0, flops, quartets, prim_quartets);
 }
 
 }  // extern "C"

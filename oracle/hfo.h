@@ -116,6 +116,9 @@ double hfo_md_flops(int la, int lb, int lc, int ld);
 /* Algorithmic flops of one direct Fock build: sum over surviving unique shell
  * quartets of Ka*Kb*Kc*Kd*flops(la,lb,lc,ld); also returns quartet counts. */
 int hfo_fock_flops(const hfo_basis* b, double* flops, double* quartets, double* prim_quartets);
+/* Same, restricted to bra pairs ij % stride == offset (the hfo_direct_jk sample). */
+int hfo_fock_flops_sample(const hfo_basis* b, int stride, int offset, double* flops, double* quartets,
+                          double* prim_quartets);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,8 @@ def lib():
         L.hfo_md_flops.restype = C.c_double
         L.hfo_fock_flops.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
+        L.hfo_fock_flops_sample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -184,9 +186,9 @@ class Basis:
         _check(lib().hfo_spherical_transform(self.h, t.ctypes.data_as(C.c_void_p), C.byref(nc)))
         return t
 
-    def fock_flops(self):
+    def fock_flops(self, stride=1, offset=0):
         f, q, pq = C.c_double(), C.c_double(), C.c_double()
-        _check(lib().hfo_fock_flops(self.h, C.byref(f), C.byref(q), C.byref(pq)))
+        _check(lib().hfo_fock_flops_sample(self.h, stride, offset, C.byref(f), C.byref(q), C.byref(pq)))
         return f.value, q.value, pq.value
 
 

@@ -1,0 +1,116 @@
+"""CPU tests of the engine's host side: basis construction, the C ABI's
+exported symbols, and the generated recurrences (compiled for the CPU)
+against the oracle and the PySCF fixture.  No GPU call is made here."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import molecules as M
+from conftest import GOLDEN, ROOT, PKG, load_package
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "hfg.h")).read()
+    return sorted(set(re.findall(r"\b(hfg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    hf = load_package()
+    lib = hf.lib()
+    syms = header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(hf.EXPORTED) <= set(syms)
+
+
+@pytest.mark.parametrize("basis", ["sto-3g", "6-31g*", "cc-pvdz", "def2-svp", "cc-pvtz"])
+def test_basis_builder_matches_oracle(basis):
+    hf = load_package()
+    mol = M.water_cluster((2, 1, 1), seed=7)
+    a = hf.Basis.from_g94(M.basis_text(basis), mol.symbols, mol.xyz).a
+    b = O.Basis.build(M.basis_text(basis), mol.symbols, mol.xyz).arrays()
+    for k in ("atom_index", "lxyz", "prim_offset", "exponent"):
+        assert np.array_equal(np.asarray(a[k]).ravel(), np.asarray(b[k]).ravel()), k
+    assert np.allclose(a["center"].ravel(), b["center"].ravel(), rtol=0, atol=0)
+    assert np.abs(a["coefficient"] - b["coefficient"]).max() <= 1e-13 * np.abs(b["coefficient"]).max()
+
+
+def test_basis_builder_rejects_missing_element():
+    hf = load_package()
+    with pytest.raises(hf.HfError):
+        hf.Basis.from_g94(M.basis_text("sto-3g"), ["Xx"], [[0, 0, 0]])
+
+
+class View(C.Structure):
+    _fields_ = [("n", C.c_int32)] + [(k, C.c_void_p) for k in ("a", "c", "l", "po", "e", "co")]
+
+
+def cpucheck():
+    path = os.path.join(PKG, "libhfg_cpucheck.so")
+    if not os.path.exists(path):
+        pytest.skip("CPU check library not built")
+    L = C.CDLL(path)
+    L.hfgc_boys.argtypes = [C.c_int, C.c_double, C.c_void_p]
+    L.hfgc_shell_quartet.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.hfgc_last_error.restype = C.c_char_p
+    return L
+
+
+def test_engine_boys_against_mpmath_fixture():
+    L = cpucheck()
+    worst = 0.0
+    for row in json.load(open(os.path.join(GOLDEN, "boys.json")))["rows"]:
+        out = np.zeros(25)
+        assert L.hfgc_boys(24, row["t"], out.ctypes.data) == 0
+        for n in range(25):
+            ref = float(row["f"][n])
+            worst = max(worst, abs(out[n] - ref) / ref)
+    assert worst < 1e-14, worst
+
+
+def test_generated_recurrences_match_pyscf_spd_blocks():
+    L = cpucheck()
+    meta = json.load(open(os.path.join(GOLDEN, "spdf_meta.json")))
+    arr = O.Basis.build(meta["basis_gaussian94"], meta["symbols"], meta["geometry_bohr"]).arrays()
+    keep = [np.ascontiguousarray(arr[k]) for k in ("atom_index", "center", "lxyz", "prim_offset", "exponent",
+                                                   "coefficient")]
+    v = View(len(keep[0]), *[k.ctypes.data for k in keep])
+    ref = np.load(os.path.join(GOLDEN, "spdf_eri.npz"))["eri"].reshape(20, 20, 20, 20)
+    ranges = [(0, 1), (1, 4), (4, 10)]
+    worst = 0.0
+    for blk in range(81):
+        ang = [blk // 27, blk // 9 % 3, blk // 3 % 3, blk % 3]
+        dims = [ranges[x][1] - ranges[x][0] for x in ang]
+        out = np.zeros(int(np.prod(dims)))
+        st = L.hfgc_shell_quartet(C.byref(v), *[4 * a + ang[a] for a in range(4)], out.ctypes.data)
+        assert st == 0, L.hfgc_last_error()
+        exp = ref[tuple(slice(*ranges[ang[a]]) for a in range(4))]
+        worst = max(worst, np.abs(out.reshape(dims) - exp).max())
+    assert worst < 1e-13, worst
+
+
+def test_generated_recurrences_match_oracle_on_contracted_shells():
+    L = cpucheck()
+    mol = M.water_cluster((2, 1, 1), seed=3)
+    ob = O.Basis.build(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz)
+    arr = ob.arrays()
+    keep = [np.ascontiguousarray(arr[k]) for k in ("atom_index", "center", "lxyz", "prim_offset", "exponent",
+                                                   "coefficient")]
+    v = View(len(keep[0]), *[k.ctypes.data for k in keep])
+    ss = ob.shells()
+    ns = len(ss) - 1
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for _ in range(150):
+        q = [int(x) for x in rng.integers(0, ns, 4)]
+        exp = ob.shell_quartet(*q)
+        out = np.zeros(exp.size)
+        assert L.hfgc_shell_quartet(C.byref(v), *q, out.ctypes.data) == 0
+        worst = max(worst, np.abs(out.reshape(exp.shape) - exp).max())
+    assert worst < 1e-12, worst
