@@ -21,7 +21,15 @@ struct PairDev {
 // One wave of work: bra pair, first ket pair and number of ket pairs (<= 64);
 // all ket pairs share one primitive count.
 struct JKTask {
-  int bra, ket0, nket, pad;
+  int bra, ket0, nket, grp;
+};
+
+// A group of ket pairs of one class and primitive count, contiguous in the
+// J/K pair order; its primitive data is stored transposed,
+// X[base + prim * size + (pair - start)], so that the lanes of a wave read
+// consecutive addresses.
+struct KetGroup {
+  int start, size, base, npp;
 };
 
 struct BlockTask {
@@ -68,7 +76,7 @@ struct Engine {
   int64_t n_quartets = 0;
   double screen = 1e-12;        // Q_ab*Q_cd > screen (src/shell_integrals.rs:184,226)
 
-  DevBuf d_pp, d_pairs, d_jkpairs, d_boys, d_scale, d_tasks, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
+  DevBuf d_pp, d_kt, d_groups, d_pairs, d_jkpairs, d_boys, d_scale, d_tasks, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
   std::vector<size_t> task_begin;  // per class, into d_tasks (HFG_NCLASS+1 entries)
   std::vector<double> class_flops;      // algorithmic FLOPs per launch
   std::vector<int64_t> class_quartets;  // quartets per launch
@@ -81,6 +89,8 @@ struct Engine {
   explicit Engine(const hfg_basis_view& b);
   ~Engine();
   PrimPairs pp_view() const;
+  PrimPairs kt_view() const;
+  size_t kt_size = 0;
 
   void compute_schwarz();
   void build_tasks();
@@ -93,5 +103,6 @@ struct Engine {
 };
 
 void class_angular(int c, int* l4);
+int persistent_blocks();
 
 }  // namespace hfg

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -36,7 +37,9 @@ struct JKArgs {
   const JKTask* tasks;
   int ntask, n;
   const PairDev* pairs;
-  PrimPairs pp;
+  const KetGroup* groups;
+  PrimPairs pp;  // bra side, contiguous per pair
+  PrimPairs kt;  // ket side, transposed per group
   const double* scale;
   const double* boys;
   const double* D;
@@ -44,23 +47,49 @@ struct JKArgs {
   double* Kacc;
 };
 
+// Boys slice of the class's order M into LDS (see eri_device.h).
+template <int M>
+__device__ __forceinline__ void load_boys_slice(const double* __restrict__ g, double* __restrict__ lds) {
+  for (int i = threadIdx.x; i < BOYS_ROWS * BOYS_SLICE; i += blockDim.x) {
+    const int r = i / BOYS_SLICE, k = i - r * BOYS_SLICE;
+    lds[i] = k == 0 ? g[r * BOYS_ROW] : g[r * BOYS_ROW + M + k];
+  }
+  __syncthreads();
+}
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restrict__ boys, int task, int lane);
+
+// Persistent waves: every wave walks the class's task list with a stride of
+// the total wave count (tasks are ordered heaviest first), so the LDS copy of
+// the Boys slice is made once per workgroup.
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
+  __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
+  load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
+  const int lane = threadIdx.x & 63;
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  for (int task = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); task < a.ntask; task += waves)
+    jk_task<LA, LB, LC, LD>(a, boys, task, lane);
+}
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restrict__ boys, int task, int lane) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND;
-  const int task = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int lane = threadIdx.x & 63;
-  if (task >= a.ntask) return;  // uniform per wave
   const JKTask t = a.tasks[task];
   const int bra = __builtin_amdgcn_readfirstlane(t.bra);
   const int ket0 = __builtin_amdgcn_readfirstlane(t.ket0);
   const int nket = __builtin_amdgcn_readfirstlane(t.nket);
+  const int grp = __builtin_amdgcn_readfirstlane(t.grp);
   const bool act = lane < nket;
   const int ket = ket0 + (act ? lane : 0);
   const PairDev& pb = a.pairs[bra];
   const PairDev& pk = a.pairs[ket];
+  const KetGroup g = a.groups[grp];
   double I[NAB * NCD];
-  eri_block<LA, LB, LC, LD>(a.pp, pb.p0, pb.p1, a.pp, pk.p0, act ? pk.p1 : pk.p0, pb.AB, pk.AB, a.boys, I);
+  eri_block<LA, LB, LC, LD>(a.pp, pb.p0, pb.p1, a.kt, g.base + (ket - g.start), act ? g.npp : 0, g.size, pb.AB,
+                            pk.AB, boys, I);
 
   double w = 1.0;
   if (pb.sh_a == pb.sh_b) w *= 0.5;
@@ -209,13 +238,15 @@ struct BlockArgs {
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void k_block(BlockArgs a) {
   constexpr int NT = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+  __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
+  load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.ntask) return;
   const BlockTask bt = a.tasks[t];
   const PairDev& pb = a.pairs[bt.bra];
   const PairDev& pk = a.pairs[bt.ket];
   double I[NT];
-  eri_block<LA, LB, LC, LD>(a.pp, pb.p0, pb.p1, a.pp, pk.p0, pk.p1, pb.AB, pk.AB, a.boys, I);
+  eri_block<LA, LB, LC, LD>(a.pp, pb.p0, pb.p1, a.pp, pk.p0, pk.p1 - pk.p0, 1, pb.AB, pk.AB, boys, I);
   double* o = a.out + bt.off;
 #pragma unroll
   for (int i = 0; i < NT; ++i) o[i] = I[i];
@@ -244,13 +275,15 @@ __global__ void k_symmetrize(const double* Jacc, const double* Kacc, double* J, 
 typedef void (*JKLaunch)(const JKArgs&, int, hipStream_t);
 typedef void (*BlockLaunch)(const BlockArgs&, hipStream_t);
 
+static int g_max_blocks = 1024;  // persistent grid: workgroups per launch (set from the CU count)
+
 template <int LA, int LB, int LC, int LD>
 void launch_jk(const JKArgs& a, int ntask, hipStream_t s) {
   JKArgs b = a;
   b.ntask = ntask;
   const int waves_per_block = 4;
-  dim3 grid((ntask + waves_per_block - 1) / waves_per_block);
-  hipLaunchKernelGGL((k_jk<LA, LB, LC, LD>), grid, dim3(64 * waves_per_block), 0, s, b);
+  const int blocks = std::min(g_max_blocks, (ntask + waves_per_block - 1) / waves_per_block);
+  hipLaunchKernelGGL((k_jk<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), 0, s, b);
 }
 template <int LA, int LB, int LC, int LD>
 void launch_block(const BlockArgs& a, hipStream_t s) {
@@ -285,6 +318,12 @@ Engine::Engine(const hfg_basis_view& b) {
   if (ndev <= 0) throw HfgError{HFG_DEVICE, "no GPU visible"};
   HIP_CHECK(hipGetDevice(&device));
   HIP_CHECK(hipStreamCreate(&stream));  // default priority
+  {
+    int cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    g_max_blocks = std::max(1, cus) * 4;  // 4 workgroups (16 waves) per CU; LDS 4 x 32 KB
+    if (const char* env = std::getenv("HFG_PERSISTENT_BLOCKS")) g_max_blocks = std::max(1, std::atoi(env));
+  }
   build_pairs(shells, pairs, pp);
   ao_scale.assign(n, 1.0);
   for (auto& s : shells)
@@ -292,9 +331,9 @@ Engine::Engine(const hfg_basis_view& b) {
 
   // primitive pairs: 8 SoA arrays in one allocation
   const size_t np = pp.size();
-  d_pp.alloc(8 * np * sizeof(double));
-  const std::vector<double>* cols[8] = {&pp.p, &pp.Px, &pp.Py, &pp.Pz, &pp.PAx, &pp.PAy, &pp.PAz, &pp.K};
-  for (int c = 0; c < 8; ++c)
+  d_pp.alloc(9 * np * sizeof(double));
+  const std::vector<double>* cols[9] = {&pp.p, &pp.Px, &pp.Py, &pp.Pz, &pp.PAx, &pp.PAy, &pp.PAz, &pp.K, &pp.H};
+  for (int c = 0; c < 9; ++c)
     HIP_CHECK(hipMemcpyAsync(d_pp.as<double>() + c * np, cols[c]->data(), np * sizeof(double), hipMemcpyHostToDevice,
                              stream));
   std::vector<PairDev> pd(pairs.size());
@@ -364,10 +403,16 @@ double md_flops(int la, int lb, int lc, int ld) {
 
 void Engine::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
 
+PrimPairs Engine::kt_view() const {
+  const double* b = d_kt.as<double>();
+  const size_t m = kt_size;
+  return PrimPairs{b, b + m, b + 2 * m, b + 3 * m, b + 4 * m, b + 5 * m, b + 6 * m, b + 7 * m, b + 8 * m};
+}
+
 PrimPairs Engine::pp_view() const {
   const double* b = d_pp.as<double>();
   const size_t np = pp.size();
-  return PrimPairs{b, b + np, b + 2 * np, b + 3 * np, b + 4 * np, b + 5 * np, b + 6 * np, b + 7 * np};
+  return PrimPairs{b, b + np, b + 2 * np, b + 3 * np, b + 4 * np, b + 5 * np, b + 6 * np, b + 7 * np, b + 8 * np};
 }
 
 void Engine::blocks(const std::vector<BlockTask>& tasks, const std::vector<int>& cls, std::vector<double>& out) {
@@ -491,7 +536,7 @@ void Engine::build_tasks() {
           t.bra = B.idx[i];
           t.ket0 = (int)k0;  // position in K.idx; fixed up below
           t.nket = (int)std::min<size_t>(64, nk - k0);
-          t.pad = (int)gk;
+          t.grp = (int)gk;
           per_class[cls].push_back(t);
         }
       }
@@ -526,12 +571,32 @@ void Engine::build_tasks() {
     task_begin[c] = all.size();
     for (JKTask t : per_class[c]) {
       t.bra = newid[t.bra];
-      t.ket0 = gstart[t.pad] + t.ket0;
-      t.pad = 0;
+      t.ket0 = gstart[t.grp] + t.ket0;
       all.push_back(t);
     }
   }
   task_begin[HFG_NCLASS] = all.size();
+  // transposed ket primitive data per group
+  std::vector<KetGroup> kg(groups.size());
+  size_t tot = 0;
+  for (size_t g = 0; g < groups.size(); ++g) {
+    kg[g] = KetGroup{gstart[g], (int)groups[g].idx.size(), (int)tot, groups[g].npp};
+    tot += (size_t)groups[g].npp * groups[g].idx.size();
+  }
+  kt_size = tot;
+  const std::vector<double>* cols[9] = {&pp.p, &pp.Px, &pp.Py, &pp.Pz, &pp.PAx, &pp.PAy, &pp.PAz, &pp.K, &pp.H};
+  std::vector<double> kt(9 * std::max<size_t>(1, tot));
+  for (size_t g = 0; g < groups.size(); ++g)
+    for (size_t j = 0; j < groups[g].idx.size(); ++j) {
+      const PairInfo& p = pairs[groups[g].idx[j]];
+      for (int ik = 0; ik < groups[g].npp; ++ik)
+        for (int c = 0; c < 9; ++c)
+          kt[c * tot + kg[g].base + (size_t)ik * kg[g].size + j] = (*cols[c])[p.p0 + ik];
+    }
+  d_kt.alloc(kt.size() * sizeof(double));
+  HIP_CHECK(hipMemcpyAsync(d_kt.p, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  d_groups.alloc(std::max<size_t>(1, kg.size()) * sizeof(KetGroup));
+  HIP_CHECK(hipMemcpyAsync(d_groups.p, kg.data(), kg.size() * sizeof(KetGroup), hipMemcpyHostToDevice, stream));
   d_jkpairs.alloc(pd.size() * sizeof(PairDev));
   HIP_CHECK(hipMemcpyAsync(d_jkpairs.p, pd.data(), pd.size() * sizeof(PairDev), hipMemcpyHostToDevice, stream));
   d_tasks.alloc(std::max<size_t>(1, all.size()) * sizeof(JKTask));
@@ -548,7 +613,9 @@ void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* d
   JKArgs a;
   a.n = n;
   a.pairs = d_jkpairs.as<PairDev>();
+  a.groups = d_groups.as<KetGroup>();
   a.pp = pp_view();
+  a.kt = kt_view();
   a.scale = d_scale.as<double>();
   a.boys = d_boys.as<double>();
   a.D = dP;
@@ -589,6 +656,8 @@ void Engine::set_profile(bool on) {
   }
   profile = on;
 }
+
+int persistent_blocks() { return g_max_blocks; }
 
 void class_angular(int c, int* l4) {
   for (int k = 0; k < 4; ++k) l4[k] = CLASS_L[c][k];

@@ -32,6 +32,22 @@ constexpr double HALF_SQRT_PI = 0.88622692545275801365;
 // F_0 = sqrt(pi/T)/2 (erf(sqrt T) = 1 to 3e-24) and the upward recursion
 // F_m = ((2m-1) F_{m-1} - e^{-T})/(2T) is stable.
 // ---------------------------------------------------------------------------
+// 1/sqrt(x) for x > 0: the hardware estimate refined by two Newton steps
+// y <- y (3/2 - x/2 y^2), each squaring the relative error.
+HFG_DEV double rsqrt_nr(double x) {
+#ifdef HFG_HOST_CHECK
+  double y = 1.0 / std::sqrt(x);
+#elif defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rsq(x);
+#else
+  double y = 1.0 / std::sqrt(x);
+#endif
+  const double h = 0.5 * x;
+  y = y * (1.5 - h * y * y);
+  y = y * (1.5 - h * y * y);
+  return y;
+}
+
 constexpr int BOYS_TAYLOR = 9;
 constexpr int BOYS_MAXM = 24;                       // highest order served
 constexpr int BOYS_ORDERS = BOYS_MAXM + BOYS_TAYLOR;  // F_0 .. F_32 stored
@@ -40,16 +56,26 @@ constexpr double BOYS_INV_STEP = 8.0;
 constexpr double BOYS_TMAX = 50.0;
 constexpr int BOYS_ROWS = 402;
 
+// Per-class slice of the table, as the kernels keep it in LDS: row r holds
+// exp(-T_r) followed by F_M(T_r) .. F_{M+8}(T_r) (BOYS_SLICE doubles).
+constexpr int BOYS_SLICE = BOYS_TAYLOR + 1;
+
 template <int M>
-HFG_DEV void boys_eval(double T, const double* __restrict__ tab, double* __restrict__ F) {
+HFG_DEV void boys_slice_row(const double* __restrict__ tab, int r, double* __restrict__ row) {
+  row[0] = tab[r * BOYS_ROW];
+  for (int k = 1; k < BOYS_SLICE; ++k) row[k] = tab[r * BOYS_ROW + M + k];
+}
+
+template <int M>
+HFG_DEV void boys_eval(double T, const double* __restrict__ slice, double* __restrict__ F) {
   if (T < BOYS_TMAX) {
     const int r = (int)(T * BOYS_INV_STEP + 0.5);
     const double dT = T - r * (1.0 / BOYS_INV_STEP);
-    const double* row = tab + r * BOYS_ROW;
+    const double* row = slice + r * BOYS_SLICE;
     // F_M(T) = sum_k F_{M+k}(T_r) (-dT)^k / k!   (Horner)
-    double acc = row[1 + M + BOYS_TAYLOR - 1];
+    double acc = row[BOYS_TAYLOR];
 #pragma unroll
-    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = row[1 + M + k - 1] + acc * (-dT) * (1.0 / k);
+    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = row[k] + acc * (-dT) * (1.0 / k);
     F[M] = acc;
     if (M > 0) {
       // e^{-dT} by its Taylor series, |dT| <= 1/16
@@ -62,11 +88,11 @@ HFG_DEV void boys_eval(double T, const double* __restrict__ tab, double* __restr
       for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + emt) * (1.0 / (2 * m + 1));
     }
   } else {
-    const double inv_t = 1.0 / T;
-    F[0] = HALF_SQRT_PI * sqrt(inv_t);
+    const double rt = rsqrt_nr(T);
+    F[0] = HALF_SQRT_PI * rt;
     if (M > 0) {
       const double emt = exp(-T);
-      const double h = 0.5 * inv_t;
+      const double h = 0.5 * rt * rt;
 #pragma unroll
       for (int m = 1; m <= M; ++m) F[m] = ((2 * m - 1) * F[m - 1] - emt) * h;
     }
@@ -96,7 +122,7 @@ HFG_DEV void boys_eval_rt(int M, double T, const double* __restrict__ tab, doubl
 
 // Primitive-pair data, structure of arrays.  For a pair of shells (A, B) in
 // kernel orientation (L_A >= L_B): p = alpha+beta, P = (alpha A + beta B)/p,
-// PA = P - A and K = c_A c_B exp(-alpha beta |AB|^2 / p) / p.
+// PA = P - A, K = c_A c_B exp(-alpha beta |AB|^2 / p) / p and H = 1/(2p).
 struct PrimPairs {
   const double* __restrict__ p;
   const double* __restrict__ Px;
@@ -106,11 +132,13 @@ struct PrimPairs {
   const double* __restrict__ PAy;
   const double* __restrict__ PAz;
   const double* __restrict__ K;
+  const double* __restrict__ H;
 };
 
+// Bra primitives b0..b1-1 (contiguous); ket primitives k0 + j*ks, j < nk.
 template <int LA, int LB, int LC, int LD>
-HFG_DEV void eri_block(const PrimPairs& bp, int b0, int b1, const PrimPairs& kp, int k0, int k1, const double AB[3],
-                       const double CD[3], const double* __restrict__ boys_tab, double* __restrict__ I);
+HFG_DEV void eri_block(const PrimPairs& bp, int b0, int b1, const PrimPairs& kp, int k0, int nk, int ks,
+                       const double AB[3], const double CD[3], const double* __restrict__ boys, double* __restrict__ I);
 
 constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 

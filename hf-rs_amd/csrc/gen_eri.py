@@ -156,37 +156,42 @@ def gen_class(la, lb, lc, ld):
     out.append("// (%s%s|%s%s): %d contracted [e0|f0] accumulators" % ("spdfg"[la], "spdfg"[lb], "spdfg"[lc],
                                                                         "spdfg"[ld], len(E) * len(Fk)))
     out.append("template <> HFG_DEV void eri_block<%d,%d,%d,%d>(" % (la, lb, lc, ld))
-    out.append("    const PrimPairs& bp, int b0, int b1, const PrimPairs& kp, int k0, int k1,")
-    out.append("    const double AB[3], const double CD[3], const double* __restrict__ boys_tab, double* __restrict__ I) {")
+    out.append("    const PrimPairs& bp, int b0, int b1, const PrimPairs& kp, int k0, int nk, int ks,")
+    out.append("    const double AB[3], const double CD[3], const double* __restrict__ boys, double* __restrict__ I) {")
     acc = {}
     for e in E:
         for f in Fk:
             acc[(e, f)] = "c%s_%s" % (tag(e), tag(f))
     out.append("  double " + ", ".join("%s = 0.0" % v for v in acc.values()) + ";")
+    # Bra primitives outer (uniform across the wave in the J/K kernel), ket
+    # primitives inner (one ket pair per lane).  The opposite nesting -- a
+    # wave-uniform loop inside the per-lane loop -- produced wrong J/K on gfx950
+    # in the digestion kernel while the same source was exact on the CPU.
     out.append("  for (int ib = b0; ib < b1; ++ib) {")
-    out.append("    const double p = bp.p[ib], Px = bp.Px[ib], Py = bp.Py[ib], Pz = bp.Pz[ib];")
-    out.append("    const double PAx = bp.PAx[ib], PAy = bp.PAy[ib], PAz = bp.PAz[ib], Kp = bp.K[ib];")
-    if la + lb > 0:
-        out.append("    const double oo2p = 0.5 / p;")
-    out.append("    for (int ik = k0; ik < k1; ++ik) {")
+    out.append("    const double p = bp.p[ib], Px = bp.Px[ib], Py = bp.Py[ib], Pz = bp.Pz[ib], Kp = bp.K[ib];")
+    if M > 0 and la + lb > 0:
+        out.append("    const double PAx = bp.PAx[ib], PAy = bp.PAy[ib], PAz = bp.PAz[ib], oo2p = bp.H[ib];")
+    out.append("    for (int jk = 0; jk < nk; ++jk) {")
+    out.append("      const int ik = k0 + jk * ks;")
     out.append("      const double q = kp.p[ik];")
     out.append("      const double PQx = Px - kp.Px[ik], PQy = Py - kp.Py[ik], PQz = Pz - kp.Pz[ik];")
-    out.append("      const double inv_s = 1.0 / (p + q);")
+    out.append("      const double rs = rsqrt_nr(p + q);  // 1/sqrt(p+q)")
+    out.append("      const double inv_s = rs * rs;")
     out.append("      const double rho = p * q * inv_s;")
     out.append("      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
-    out.append("      const double pref = TWO_PI_2_5 * Kp * kp.K[ik] * sqrt(inv_s);")
+    out.append("      const double pref = TWO_PI_2_5 * Kp * kp.K[ik] * rs;")
     out.append("      double Fm[%d];" % (M + 1))
-    out.append("      boys_eval<%d>(T, boys_tab, Fm);" % M)
+    out.append("      boys_eval<%d>(T, boys, Fm);" % M)
     for m in range(M + 1):
         out.append("      const double F%d = pref * Fm[%d];" % (m, m))
     if M > 0:
         if la + lb > 0:
-            out.append("      const double WPx = -q * inv_s * PQx, WPy = -q * inv_s * PQy, WPz = -q * inv_s * PQz;")
             out.append("      const double rp = q * inv_s;")
+            out.append("      const double WPx = -rp * PQx, WPy = -rp * PQy, WPz = -rp * PQz;")
         if lc + ld > 0:
-            out.append("      const double QCx = kp.PAx[ik], QCy = kp.PAy[ik], QCz = kp.PAz[ik];")
-            out.append("      const double WQx = p * inv_s * PQx, WQy = p * inv_s * PQy, WQz = p * inv_s * PQz;")
-            out.append("      const double oo2q = 0.5 / q, rq = p * inv_s, oo2s = 0.5 * inv_s;")
+            out.append("      const double QCx = kp.PAx[ik], QCy = kp.PAy[ik], QCz = kp.PAz[ik], oo2q = kp.H[ik];")
+            out.append("      const double rq = p * inv_s, oo2s = 0.5 * inv_s;")
+            out.append("      const double WQx = rq * PQx, WQy = rq * PQy, WQz = rq * PQz;")
     vrr = VRR(em)
     for e in E:
         for f in Fk:

@@ -38,7 +38,7 @@ extern "C" int hfgc_shell_quartet(const hfg_basis_view* b, int sa, int sb, int s
     append_pair(sh[o.k[2]], sh[o.k[3]], ket);
     auto view = [](const PrimPairHost& h) {
       return PrimPairs{h.p.data(), h.Px.data(), h.Py.data(), h.Pz.data(),
-                       h.PAx.data(), h.PAy.data(), h.PAz.data(), h.K.data()};
+                       h.PAx.data(), h.PAy.data(), h.PAz.data(), h.K.data(), h.H.data()};
     };
     PrimPairs bp = view(bra), kp = view(ket);
     double AB[3], CD[3];
@@ -46,11 +46,12 @@ extern "C" int hfgc_shell_quartet(const hfg_basis_view* b, int sa, int sb, int s
       AB[k] = sh[o.k[0]].c[k] - sh[o.k[1]].c[k];
       CD[k] = sh[o.k[2]].c[k] - sh[o.k[3]].c[k];
     }
-    std::vector<double> blk(1296);
+    std::vector<double> blk(1296), slice(BOYS_ROWS * BOYS_SLICE);
     switch (o.cls) {
 #define X(a, b_, c, d)                                                                              \
   case (a * (a + 1) / 2 + b_) * ((a * (a + 1) / 2 + b_) + 1) / 2 + (c * (c + 1) / 2 + d):          \
-    eri_block<a, b_, c, d>(bp, 0, (int)bra.size(), kp, 0, (int)ket.size(), AB, CD, tab.data(), blk.data()); \
+    for (int r = 0; r < BOYS_ROWS; ++r) boys_slice_row<a + b_ + c + d>(tab.data(), r, &slice[r * BOYS_SLICE]); \
+    eri_block<a, b_, c, d>(bp, 0, (int)bra.size(), kp, 0, (int)ket.size(), 1, AB, CD, slice.data(), blk.data()); \
     break;
       HFG_FOR_EACH_CLASS(X)
 #undef X

@@ -90,6 +90,7 @@ void append_pair(const Shell& A, const Shell& B, PrimPairHost& pp) {
       pp.PAy.push_back(P[1] - A.c[1]);
       pp.PAz.push_back(P[2] - A.c[2]);
       pp.K.push_back(A.r[ka] * B.r[kb] * std::exp(-a * b / p * ab2) / p);
+      pp.H.push_back(0.5 / p);
     }
 }
 

@@ -38,7 +38,7 @@ struct PairInfo {
 };
 
 struct PrimPairHost {
-  std::vector<double> p, Px, Py, Pz, PAx, PAy, PAz, K;
+  std::vector<double> p, Px, Py, Pz, PAx, PAy, PAz, K, H;
   size_t size() const { return p.size(); }
 };
 

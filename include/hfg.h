@@ -104,6 +104,9 @@ int hfg_class_stats(hfg_ctx* ctx, int cls, int32_t* l4, double* flops_per_launch
  * and src/integrals.rs:70-101, 259-346, 905-971. */
 int hfg_fock_flops(hfg_ctx* ctx, double* flops, double* prim_quartets);
 double hfg_md_flops(int la, int lb, int lc, int ld);
+/* Workgroups of one persistent class launch (4 per compute unit by default,
+ * HFG_PERSISTENT_BLOCKS overrides); instrumentation, no reference counterpart. */
+int hfg_persistent_blocks(void);
 
 /* RHF options and result (ScfOptions, src/api.rs:323-410; HFResult,
  * src/scf.rs:14-33). */
