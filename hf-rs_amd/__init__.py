@@ -52,7 +52,19 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_schwarz", "hfg_shell_quartet", "hfg_eri_packed", "hfg_contract_jk", "hfg_build_fock",
             "hfg_contract_jk_device", "hfg_scf_rhf", "hfg_basis_from_g94", "hfg_basis_free",
             "hfg_basis_get_view", "hfg_set_density", "hfg_jk_resident", "hfg_get_jk", "hfg_profile",
-            "hfg_class_stats", "hfg_fock_flops", "hfg_md_flops"]
+            "hfg_class_stats", "hfg_fock_flops", "hfg_md_flops", "hfg_persistent_blocks", "hfg_df_build",
+            "hfg_df_destroy", "hfg_df_info", "hfg_df_jk", "hfg_df_rhf"]
+
+
+class DfhfOptions(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("energy_tol", C.c_double), ("residual_tol", C.c_double),
+                ("density_tol", C.c_double), ("metric_cutoff", C.c_double)]
+
+
+class DfhfResult(C.Structure):
+    _fields_ = [("total_energy", C.c_double), ("iterations", C.c_int32), ("converged", C.c_int32),
+                ("n_aux", C.c_int32), ("fock_builds", C.c_int32), ("seconds_total", C.c_double),
+                ("seconds_fock", C.c_double)]
 
 
 def lib():
@@ -86,6 +98,12 @@ def lib():
         L.hfg_fock_flops.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl)]
         L.hfg_md_flops.argtypes = [i32] * 4
         L.hfg_md_flops.restype = dbl
+        L.hfg_df_build.argtypes = [C.POINTER(BasisView), C.POINTER(BasisView), dbl, C.POINTER(vp)]
+        L.hfg_df_destroy.argtypes = [vp]
+        L.hfg_df_info.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3
+        L.hfg_df_jk.argtypes = [vp, vp, i32, vp, vp]
+        L.hfg_df_rhf.argtypes = [C.POINTER(BasisView), C.POINTER(BasisView), vp, i32, i32, C.POINTER(DfhfOptions),
+                                 C.POINTER(DfhfResult), vp, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -235,6 +253,47 @@ class Context:
     def contract_jk_device(self, d_p, d_j, d_k):
         """J/K on HBM-resident buffers (device pointers as ints, e.g. torch data_ptr())."""
         _check(lib().hfg_contract_jk_device(self.h, C.c_void_p(d_p), C.c_void_p(d_j), C.c_void_p(d_k)))
+
+
+class DfTensor:
+    """Mirror of DfTensor (src/df.rs:80-174), resident in HBM."""
+
+    def __init__(self, basis, aux, metric_cutoff=1e-10):
+        self.basis, self.aux = basis, aux
+        self.h = C.c_void_p()
+        _check(lib().hfg_df_build(C.byref(basis.view), C.byref(aux.view), metric_cutoff, C.byref(self.h)))
+        v = [C.c_int64() for _ in range(3)]
+        _check(lib().hfg_df_info(self.h, *[C.byref(x) for x in v]))
+        self.n, self.n_aux, self.rank = (x.value for x in v)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value and _LIB is not None:
+            _LIB.hfg_df_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def coulomb_exchange_rhf(self, cocc):
+        cocc = np.ascontiguousarray(cocc, dtype=np.float64)
+        j = np.zeros((self.n, self.n))
+        k = np.zeros((self.n, self.n))
+        _check(lib().hfg_df_jk(self.h, _p(cocc), cocc.shape[1], _p(j), _p(k)))
+        return j, k
+
+
+def df_rhf(basis, aux, zxyz, charge=0, max_iter=100, e_tol=1e-12, r_tol=1e-10, d_tol=1e-10, metric_cutoff=1e-10):
+    """DF-RHF (run_df_hf_with_aux, src/hf_variants/ri_df_hf.rs:122-263); defaults of DFHFOptions."""
+    opt = DfhfOptions(max_iter, e_tol, r_tol, d_tol, metric_cutoff)
+    res = DfhfResult()
+    zxyz = np.ascontiguousarray(zxyz, dtype=np.float64)
+    n = basis.n
+    p, f, j, k = (np.zeros((n, n)) for _ in range(4))
+    _check(lib().hfg_df_rhf(C.byref(basis.view), C.byref(aux.view), _p(zxyz), len(zxyz), charge, C.byref(opt),
+                            C.byref(res), _p(p), _p(f), _p(j), _p(k)))
+    out = {k_: getattr(res, k_) for k_, _ in DfhfResult._fields_}
+    out.update(density=p, fock=f, j=j, k=k)
+    return out
 
 
 CONV = {"adiissoscf": 0, "default": 0, "cdiis": 1, "diis": 1, "pulay": 1, "adiis": 2, "adiiscdiis": 3}

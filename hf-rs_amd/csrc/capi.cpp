@@ -20,6 +20,8 @@ int set_err(int code, const std::string& m) {
 }
 }  // namespace
 
+int hfg::set_error(int code, const std::string& m) { return set_err(code, m); }
+
 struct hfg_ctx {
   std::unique_ptr<Engine> e;
 };

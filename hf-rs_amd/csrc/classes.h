@@ -7,3 +7,7 @@
   X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 0, 0) X(2, 1, 1, 0) X(2, 1, 1, 1) X(2, 1, 2, 0)     \
   X(2, 1, 2, 1) X(2, 2, 0, 0) X(2, 2, 1, 0) X(2, 2, 1, 1) X(2, 2, 2, 0) X(2, 2, 2, 1) X(2, 2, 2, 2)
 #define HFG_NCLASS 21
+
+// All 36 classes, including the auxiliary f-s and g-s pair classes used by the
+// density-fitting integrals (block kernels only).
+#include "gen/class_list.inc"

@@ -1,0 +1,594 @@
+// df.hip -- Coulomb-metric density fitting on the GPU: metric (P|Q), the
+// 3-index tensor (P|mu nu), the whitened factor B = X^T (P|mu nu) with
+// X = U_kept lambda^{-1/2} of the metric (canonical orthogonaliser, cutoff
+// 1e-10), DF-J/K, and the DF-RHF driver.
+//
+// Follows src/df.rs:16-174 (metric_orthogonalizer, coulomb_metric,
+// three_center_integrals, DfTensor::build, coulomb_exchange_rhf) and
+// run_factorized_rhf (src/hf_variants/ri_df_hf.rs:160-263).  The integrals are
+// the generated Head-Gordon-Pople blocks with the auxiliary shell paired with
+// the exponent-0 unit function (src/df.rs:30-40); B is kept unpacked,
+// [rank][n][n], so that DF-J is two FP64 GEMVs over B (HBM-bound) and DF-K is
+// one batched GEMM A_Q = B_Q C plus one GEMM K = 2 A' A'^T over the
+// rank*n_occ index (FP64 matrix cores through rocBLAS).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+
+#include "../../include/hfg.h"
+#include "engine.h"
+#include "la_device.h"
+#include "scf_host.h"
+
+namespace hfg {
+
+// Block element -> 3-index tensor T[P][mu][nu] (both mu nu orders), scaled.
+struct Scatter3 {
+  long long off;
+  int aux_bra;        // 1: block is (P 1|a b), else (a b|P 1)
+  int na, nb, np;     // component counts (kernel orientation of the orbital pair)
+  int mu0, nu0, p0;   // AO offsets of first / second orbital shell and aux shell
+};
+
+__global__ void k_scatter3(const Scatter3* __restrict__ d, int ntask, const double* __restrict__ blk,
+                           const double* __restrict__ so, const double* __restrict__ sx, double* __restrict__ T,
+                           int n) {
+  const int t = blockIdx.x;
+  if (t >= ntask) return;
+  const Scatter3 s = d[t];
+  const int tot = s.na * s.nb * s.np;
+  const size_t nn = (size_t)n * n;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int a, b, p;
+    if (s.aux_bra) {
+      p = e / (s.na * s.nb);
+      const int r = e - p * s.na * s.nb;
+      a = r / s.nb;
+      b = r - a * s.nb;
+    } else {
+      a = e / (s.nb * s.np);
+      const int r = e - a * s.nb * s.np;
+      b = r / s.np;
+      p = r - b * s.np;
+    }
+    const int mu = s.mu0 + a, nu = s.nu0 + b, P = s.p0 + p;
+    const double v = blk[s.off + e] * so[mu] * so[nu] * sx[P];
+    T[(size_t)P * nn + (size_t)mu * n + nu] = v;
+    T[(size_t)P * nn + (size_t)nu * n + mu] = v;
+  }
+}
+
+// Metric block (P 1|Q 1) -> V[P][Q] and V[Q][P].
+struct ScatterM {
+  long long off;
+  int np, nq, p0, q0;
+};
+
+__global__ void k_scatter_metric(const ScatterM* __restrict__ d, int ntask, const double* __restrict__ blk,
+                                 const double* __restrict__ sx, double* __restrict__ V, int naux) {
+  const int t = blockIdx.x;
+  if (t >= ntask) return;
+  const ScatterM s = d[t];
+  for (int e = threadIdx.x; e < s.np * s.nq; e += blockDim.x) {
+    const int i = e / s.nq, j = e - i * s.nq;
+    const int P = s.p0 + i, Q = s.q0 + j;
+    const double v = blk[s.off + e] * sx[P] * sx[Q];
+    V[(size_t)P * naux + Q] = v;
+    V[(size_t)Q * naux + P] = v;
+  }
+}
+
+// A[Q][m][i] -> At[m][Q][i] (rank x n x o to n x rank x o).
+__global__ void k_permute_qmi(const double* __restrict__ A, double* __restrict__ At, int rank, int n, int o) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t tot = (size_t)rank * n * o;
+  if (idx >= tot) return;
+  const int i = (int)(idx % o);
+  const size_t r = idx / o;
+  const int m = (int)(r % n);
+  const int q = (int)(r / n);
+  At[((size_t)m * rank + q) * o + i] = A[idx];
+}
+
+__global__ void k_axpby_fock(const double* H, const double* J, const double* K, double* F, size_t nn) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nn) F[i] = H[i] + J[i] - 0.5 * K[i];
+}
+
+struct DfEngine {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Blas> bl;
+  int n = 0, naux = 0, rank = 0;
+  std::vector<Shell> orb, aux;
+  DevBuf d_B;  // rank x n x n
+  DevBuf d_work, d_A, d_At;
+  size_t work_o = 0;
+
+  DfEngine(const hfg_basis_view& basis, const hfg_basis_view& auxv, double cutoff) {
+    if (!(cutoff > 0) || !std::isfinite(cutoff))
+      throw HfgError{HFG_INVALID_INPUT, "DF metric cutoff must be finite and positive"};
+    orb = group_shells(basis);
+    aux = group_shells(auxv);
+    n = basis.n_functions;
+    naux = auxv.n_functions;
+    for (auto& s : orb)
+      if (s.L > 2) throw HfgError{HFG_UNSUPPORTED, "density fitting compiled for orbital shells up to d"};
+    for (auto& s : aux)
+      if (s.L > 4) throw HfgError{HFG_UNSUPPORTED, "density fitting compiled for auxiliary shells up to g"};
+    const double bytes = 8.0 * ((double)naux * n * n + (double)naux * naux * 2);
+    if (bytes > 64e9) throw HfgError{HFG_INVALID_INPUT, "DF 3-index tensor exceeds the 64 GB working set"};
+    HIP_CHECK(hipStreamCreate(&stream));
+    bl.reset(new Blas(stream));
+    build(cutoff);
+  }
+  ~DfEngine() {
+    bl.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void build(double cutoff) {
+    // pair table: orbital pairs (a >= b) then auxiliary pairs (P, unit)
+    std::vector<PairInfo> pairs;
+    PrimPairHost pp;
+    build_pairs(orb, pairs, pp);
+    const int norb_pairs = (int)pairs.size();
+    for (size_t p = 0; p < aux.size(); ++p) {
+      PairInfo pi;
+      std::memset(&pi, 0, sizeof pi);
+      pi.p0 = (int)pp.size();
+      append_aux_pair(aux[p], pp);
+      pi.p1 = (int)pp.size();
+      pi.ao_a = aux[p].ao;
+      pi.ao_b = 0;
+      pi.sh_a = (int)p;
+      pi.sh_b = -1;
+      pi.la = aux[p].L;
+      pi.lb = 0;
+      pairs.push_back(pi);
+    }
+    const size_t np = pp.size();
+    DevBuf d_pp, d_pairs, d_boys, d_so, d_sx;
+    d_pp.alloc(9 * np * sizeof(double));
+    const std::vector<double>* cols[9] = {&pp.p, &pp.Px, &pp.Py, &pp.Pz, &pp.PAx, &pp.PAy, &pp.PAz, &pp.K, &pp.H};
+    for (int c = 0; c < 9; ++c)
+      HIP_CHECK(hipMemcpyAsync(d_pp.as<double>() + c * np, cols[c]->data(), np * 8, hipMemcpyHostToDevice, stream));
+    std::vector<PairDev> pd(pairs.size());
+    for (size_t i = 0; i < pairs.size(); ++i) {
+      std::memset(&pd[i], 0, sizeof(PairDev));
+      pd[i].p0 = pairs[i].p0;
+      pd[i].p1 = pairs[i].p1;
+      for (int k = 0; k < 3; ++k) pd[i].AB[k] = pairs[i].AB[k];
+    }
+    d_pairs.alloc(pd.size() * sizeof(PairDev));
+    HIP_CHECK(hipMemcpyAsync(d_pairs.p, pd.data(), pd.size() * sizeof(PairDev), hipMemcpyHostToDevice, stream));
+    auto tab = boys_table();
+    d_boys.alloc(tab.size() * 8);
+    HIP_CHECK(hipMemcpyAsync(d_boys.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, stream));
+    std::vector<double> so(n), sx(naux);
+    for (auto& s : orb)
+      for (size_t c = 0; c < s.scale.size(); ++c) so[s.ao + c] = s.scale[c];
+    for (auto& s : aux)
+      for (size_t c = 0; c < s.scale.size(); ++c) sx[s.ao + c] = s.scale[c];
+    d_so.alloc(n * 8);
+    d_sx.alloc(naux * 8);
+    HIP_CHECK(hipMemcpyAsync(d_so.p, so.data(), n * 8, hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(d_sx.p, sx.data(), naux * 8, hipMemcpyHostToDevice, stream));
+    const PrimPairs view{d_pp.as<double>(),          d_pp.as<double>() + np,     d_pp.as<double>() + 2 * np,
+                         d_pp.as<double>() + 3 * np, d_pp.as<double>() + 4 * np, d_pp.as<double>() + 5 * np,
+                         d_pp.as<double>() + 6 * np, d_pp.as<double>() + 7 * np, d_pp.as<double>() + 8 * np};
+    auto nc = [](int L) { return (L + 1) * (L + 2) / 2; };
+
+    // Run block tasks grouped by class in bounded batches, then scatter.
+    auto run_blocks = [&](std::vector<BlockTask>& tasks, std::vector<int>& cls, auto&& scatter) {
+      std::vector<size_t> ord(tasks.size());
+      std::iota(ord.begin(), ord.end(), 0);
+      std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return cls[x] < cls[y]; });
+      const long long budget = 1ll << 26;  // doubles per batch (512 MB)
+      size_t i = 0;
+      DevBuf dt, dout;
+      while (i < ord.size()) {
+        std::vector<BlockTask> bt;
+        std::vector<int> bc;
+        std::vector<size_t> which;
+        long long off = 0;
+        while (i < ord.size() && off < budget) {
+          BlockTask t = tasks[ord[i]];
+          const long long sz = t.off;  // size stashed by the caller
+          t.off = off;
+          bt.push_back(t);
+          bc.push_back(cls[ord[i]]);
+          which.push_back(ord[i]);
+          off += sz;
+          ++i;
+        }
+        dt.alloc(bt.size() * sizeof(BlockTask));
+        dout.alloc(std::max<long long>(1, off) * 8);
+        HIP_CHECK(hipMemcpyAsync(dt.p, bt.data(), bt.size() * sizeof(BlockTask), hipMemcpyHostToDevice, stream));
+        size_t a = 0;
+        while (a < bt.size()) {
+          size_t b = a;
+          while (b < bt.size() && bc[b] == bc[a]) ++b;
+          BlockArgs args{dt.as<BlockTask>() + a, (int)(b - a), d_pairs.as<PairDev>(), view, d_boys.as<double>(),
+                         dout.as<double>()};
+          launch_block_class(bc[a], args, stream);
+          a = b;
+        }
+        scatter(bt, which, dout.as<double>());
+        HIP_CHECK(hipStreamSynchronize(stream));
+      }
+    };
+
+    // Metric (P|Q), Q <= P.
+    DevBuf d_V;
+    d_V.alloc((size_t)naux * naux * 8);
+    {
+      std::vector<BlockTask> tasks;
+      std::vector<int> cls;
+      std::vector<ScatterM> sm;
+      for (int p = 0; p < (int)aux.size(); ++p)
+        for (int q = 0; q <= p; ++q) {
+          int b = norb_pairs + p, k = norb_pairs + q;
+          int pb = pair_class(aux[p].L, 0), pk = pair_class(aux[q].L, 0);
+          if (pk > pb) {
+            std::swap(b, k);
+            std::swap(pb, pk);
+          }
+          const int s0 = b - norb_pairs, s1 = k - norb_pairs;
+          const int n0 = nc(aux[s0].L), n1 = nc(aux[s1].L);
+          tasks.push_back({b, k, (long long)n0 * n1});
+          cls.push_back(quartet_class(pb, pk));
+          sm.push_back({0, n0, n1, aux[s0].ao, aux[s1].ao});
+        }
+      DevBuf dsm;
+      run_blocks(tasks, cls, [&](std::vector<BlockTask>& bt, std::vector<size_t>& which, double* blk) {
+        std::vector<ScatterM> h(bt.size());
+        for (size_t i = 0; i < bt.size(); ++i) {
+          h[i] = sm[which[i]];
+          h[i].off = bt[i].off;
+        }
+        dsm.alloc(h.size() * sizeof(ScatterM));
+        HIP_CHECK(hipMemcpyAsync(dsm.p, h.data(), h.size() * sizeof(ScatterM), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_scatter_metric, dim3(h.size()), dim3(64), 0, stream, dsm.as<ScatterM>(), (int)h.size(),
+                           blk, d_sx.as<double>(), d_V.as<double>(), naux);
+        HIP_CHECK(hipGetLastError());
+      });
+    }
+    // X = U_kept lambda^{-1/2} (src/matrix.rs:329-346 with cutoff)
+    Dev dw;
+    dw.alloc(naux);
+    Eigh eig;
+    eig.run(*bl, d_V.as<double>(), dw.p, naux);
+    std::vector<double> w(naux);
+    download(w.data(), dw.p, naux, stream);
+    if (w[0] < -cutoff) throw HfgError{HFG_INTEGRAL, "metric has a negative eigenvalue"};
+    int first = 0;
+    while (first < naux && w[first] < cutoff) ++first;
+    rank = naux - first;
+    if (rank <= 0) throw HfgError{HFG_INTEGRAL, "metric has zero retained rank"};
+    // rows of d_V's row-major view are eigenvectors; X^T rows = kept vectors / sqrt(lambda)
+    std::vector<double> U((size_t)naux * naux);
+    download(U.data(), d_V.as<double>(), (size_t)naux * naux, stream);
+    std::vector<double> Xt((size_t)rank * naux);
+    for (int r = 0; r < rank; ++r)
+      for (int c = 0; c < naux; ++c) Xt[(size_t)r * naux + c] = U[(size_t)(first + r) * naux + c] / std::sqrt(w[first + r]);
+    DevBuf d_Xt;
+    d_Xt.alloc(Xt.size() * 8);
+    HIP_CHECK(hipMemcpyAsync(d_Xt.p, Xt.data(), Xt.size() * 8, hipMemcpyHostToDevice, stream));
+    d_V.free();
+
+    // 3-index tensor T[P][mu][nu].
+    DevBuf d_T;
+    const size_t nn = (size_t)n * n;
+    d_T.alloc((size_t)naux * nn * 8);
+    {
+      std::vector<BlockTask> tasks;
+      std::vector<int> cls;
+      std::vector<Scatter3> s3;
+      for (int i = 0; i < norb_pairs; ++i) {
+        const PairInfo& o = pairs[i];
+        const int pco = pair_class(o.la, o.lb);
+        for (int p = 0; p < (int)aux.size(); ++p) {
+          const int pcx = pair_class(aux[p].L, 0);
+          const bool aux_bra = pcx > pco;
+          const int na = nc(o.la), nb = nc(o.lb), nx = nc(aux[p].L);
+          tasks.push_back({aux_bra ? norb_pairs + p : i, aux_bra ? i : norb_pairs + p, (long long)na * nb * nx});
+          cls.push_back(aux_bra ? quartet_class(pcx, pco) : quartet_class(pco, pcx));
+          s3.push_back({0, aux_bra ? 1 : 0, na, nb, nx, o.ao_a, o.ao_b, aux[p].ao});
+        }
+      }
+      DevBuf ds3;
+      run_blocks(tasks, cls, [&](std::vector<BlockTask>& bt, std::vector<size_t>& which, double* blk) {
+        std::vector<Scatter3> h(bt.size());
+        for (size_t i = 0; i < bt.size(); ++i) {
+          h[i] = s3[which[i]];
+          h[i].off = bt[i].off;
+        }
+        ds3.alloc(h.size() * sizeof(Scatter3));
+        HIP_CHECK(hipMemcpyAsync(ds3.p, h.data(), h.size() * sizeof(Scatter3), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_scatter3, dim3(h.size()), dim3(64), 0, stream, ds3.as<Scatter3>(), (int)h.size(), blk,
+                           d_so.as<double>(), d_sx.as<double>(), d_T.as<double>(), n);
+        HIP_CHECK(hipGetLastError());
+      });
+    }
+    // B = X^T T  (rank x naux) (naux x n^2)
+    d_B.alloc((size_t)rank * nn * 8);
+    bl->gemm(false, false, rank, (int)nn, naux, 1.0, d_Xt.as<double>(), naux, d_T.as<double>(), (int)nn, 0.0,
+             d_B.as<double>(), (int)nn);
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  // J = sum_Q B_Q (B_Q . P), P = 2 C C^T; K = 2 sum_Q (B_Q C)(B_Q C)^T
+  // (src/df.rs:143-173).  dC: n x o device; outputs device n x n.
+  void jk(const double* dC, int o, double* dJ, double* dK) {
+    const size_t nn = (size_t)n * n;
+    if ((size_t)o != work_o) {
+      d_work.alloc(std::max(nn, (size_t)rank) * 8 + 8);
+      d_A.alloc((size_t)rank * n * o * 8);
+      d_At.alloc((size_t)rank * n * o * 8);
+      work_o = o;
+    }
+    double* P = d_work.as<double>();
+    Dev dq;
+    dq.alloc(rank);
+    bl->gemm(false, true, n, n, o, 2.0, dC, o, dC, o, 0.0, P, n);
+    const double one = 1.0, zero = 0.0;
+    // column-major view of B: nn x rank (B^T); d = B P  ->  gemv(T)
+    RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_transpose, (rocblas_int)nn, rank, &one, d_B.as<double>(),
+                           (rocblas_int)nn, P, 1, &zero, dq.p, 1));
+    RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_none, (rocblas_int)nn, rank, &one, d_B.as<double>(),
+                           (rocblas_int)nn, dq.p, 1, &zero, dJ, 1));
+    // A = B C: rows (Q, m), (rank*n) x o
+    bl->gemm(false, false, rank * n, o, n, 1.0, d_B.as<double>(), n, dC, o, 0.0, d_A.as<double>(), o);
+    const size_t tot = (size_t)rank * n * o;
+    hipLaunchKernelGGL(k_permute_qmi, dim3((tot + 255) / 256), dim3(256), 0, stream, d_A.as<double>(),
+                       d_At.as<double>(), rank, n, o);
+    HIP_CHECK(hipGetLastError());
+    bl->gemm(false, true, n, n, rank * o, 2.0, d_At.as<double>(), rank * o, d_At.as<double>(), rank * o, 0.0, dK, n);
+  }
+};
+
+// DF-RHF (src/hf_variants/ri_df_hf.rs:160-263): core-Hamiltonian guess, CDIIS
+// from the second iteration, convergence on |dE|, the projected commutator
+// residual and max|dP|, then one final Fock build with the new orbitals.
+static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const double* zxyz, int natom, int charge,
+                  const hfg_dfhf_options* opt, hfg_dfhf_result* res, double* p_out, double* f_out, double* j_out,
+                  double* k_out) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  std::memset(res, 0, sizeof *res);
+  if (natom <= 0 || opt->max_iter <= 0 || !(opt->energy_tol > 0) || !(opt->residual_tol > 0) ||
+      !(opt->density_tol > 0) || !(opt->metric_cutoff > 0))
+    throw HfgError{HFG_INVALID_INPUT, "DF-HF requires finite geometry and positive convergence controls"};
+  std::vector<std::array<double, 4>> atoms(natom);
+  double el = -charge;
+  for (int a = 0; a < natom; ++a) {
+    for (int k = 0; k < 4; ++k) atoms[a][k] = zxyz[4 * a + k];
+    el += atoms[a][0];
+  }
+  if (!(el > 0) || std::fabs(el - std::round(el)) > 1e-8 || ((long)std::lround(el)) % 2)
+    throw HfgError{HFG_INVALID_INPUT, "DF-RHF requires a positive even electron count"};
+  DfEngine df(*basis, *auxv, opt->metric_cutoff);
+  const int n = df.n, no = (int)std::lround(el) / 2;
+  hipStream_t st = df.stream;
+  Blas& bl = *df.bl;
+  const size_t nn = (size_t)n * n;
+  double enuc = 0;
+  for (int i = 0; i < natom; ++i)
+    for (int j = i + 1; j < natom; ++j) {
+      const double dx = atoms[i][1] - atoms[j][1], dy = atoms[i][2] - atoms[j][2], dz = atoms[i][3] - atoms[j][3];
+      enuc += atoms[i][0] * atoms[j][0] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  std::vector<double> S, T, V;
+  one_electron(df.orb, n, atoms, S, T, V);
+  std::vector<double> H(nn);
+  for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
+  Dev dS, dH, dX, dTmp, dTmp2, dF, dP, dPn, dJ, dK, dC, dCo, dCn, dW, dErr, dFu;
+  for (Dev* d : {&dS, &dH, &dTmp, &dTmp2, &dF, &dP, &dPn, &dJ, &dK, &dErr, &dFu}) d->alloc(nn);
+  upload(dS.p, S.data(), nn, st);
+  upload(dH.p, H.data(), nn, st);
+  // canonical orthogonaliser of S (cutoff 1e-8, Cartesian)
+  Eigh eig;
+  int m;
+  {
+    Dev dw;
+    dw.alloc(n);
+    HIP_CHECK(hipMemcpyAsync(dTmp.p, dS.p, nn * 8, hipMemcpyDeviceToDevice, st));
+    eig.run(bl, dTmp.p, dw.p, n);
+    std::vector<double> w(n), U(nn);
+    download(w.data(), dw.p, n, st);
+    download(U.data(), dTmp.p, nn, st);
+    if (w[0] < -1e-8) throw HfgError{HFG_INTEGRAL, "overlap has negative eigenvalue"};
+    int first = 0;
+    while (first < n && w[first] < 1e-8) ++first;
+    m = n - first;
+    if (no > m) throw HfgError{HFG_INVALID_INPUT, "factorized RHF occupied count exceeds overlap rank"};
+    std::vector<double> X((size_t)n * m);
+    for (int c = 0; c < m; ++c)
+      for (int r = 0; r < n; ++r) X[(size_t)r * m + c] = U[(size_t)(first + c) * n + r] / std::sqrt(w[first + c]);
+    dX.alloc((size_t)n * m);
+    upload(dX.p, X.data(), (size_t)n * m, st);
+  }
+  dC.alloc((size_t)n * m);
+  dCo.alloc((size_t)n * no);
+  dCn.alloc((size_t)n * m);
+  dW.alloc(m);
+  const int tb = 256;
+  auto grid = [&](size_t c) { return dim3((unsigned)((c + tb - 1) / tb)); };
+  // C = X eigvecs(X^T F X); C_occ; P = 2 C_occ C_occ^T
+  auto diag = [&](const double* F, double* C, double* P) {
+    bl.gemm(false, false, n, m, n, 1.0, F, n, dX.p, m, 0.0, dTmp.p, m);
+    bl.gemm(true, false, m, m, n, 1.0, dX.p, m, dTmp.p, m, 0.0, dTmp2.p, m);
+    eig.run(bl, dTmp2.p, dW.p, m);
+    bl.gemm(false, true, n, m, m, 1.0, dX.p, m, dTmp2.p, m, 0.0, C, m);
+    bl.gemm(false, true, n, n, no, 2.0, C, m, C, m, 0.0, P, n);
+  };
+  auto occ_copy = [&](const double* C) {
+    // C_occ (n x no): the first no columns of C (n x m), a strided device copy
+    HIP_CHECK(hipMemcpy2DAsync(dCo.p, (size_t)no * 8, C, (size_t)m * 8, (size_t)no * 8, n, hipMemcpyDeviceToDevice, st));
+  };
+  double t_fock = 0;
+  int builds = 0;
+  auto fock = [&](double* F) {
+    const auto tf = clk::now();
+    df.jk(dCo.p, no, dJ.p, dK.p);
+    hipLaunchKernelGGL(k_axpby_fock, grid(nn), dim3(tb), 0, st, dH.p, dJ.p, dK.p, F, nn);
+    HIP_CHECK(hipStreamSynchronize(st));
+    t_fock += std::chrono::duration<double>(clk::now() - tf).count();
+    ++builds;
+  };
+  auto energy = [&](const double* P, const double* F) { return 0.5 * (bl.dot(nn, P, dH.p) + bl.dot(nn, P, F)); };
+  diag(dH.p, dC.p, dP.p);
+  occ_copy(dC.p);
+  std::vector<std::vector<double>> fh, eh;  // CDIIS on the host (m x m errors, n x n Focks)
+  double last = INFINITY;
+  const size_t mm2 = (size_t)m * m;
+  for (int it = 1; it <= opt->max_iter; ++it) {
+    fock(dF.p);
+    const double ce = energy(dP.p, dF.p) + enuc;
+    // projected commutator X^T (FPS - SPF) X
+    bl.gemm(false, false, n, n, n, 1.0, dF.p, n, dP.p, n, 0.0, dTmp.p, n);
+    bl.gemm(false, false, n, n, n, 1.0, dTmp.p, n, dS.p, n, 0.0, dTmp2.p, n);
+    hipLaunchKernelGGL(k_antisym, grid(nn), dim3(tb), 0, st, dTmp2.p, dErr.p, n);
+    bl.gemm(false, false, n, m, n, 1.0, dErr.p, n, dX.p, m, 0.0, dTmp.p, m);
+    bl.gemm(true, false, m, m, n, 1.0, dX.p, m, dTmp.p, m, 0.0, dTmp2.p, m);
+    const double resid = bl.amax(mm2, dTmp2.p);
+    std::vector<double> err(mm2);
+    download(err.data(), dTmp2.p, mm2, st);
+    diag(dF.p, dCn.p, dPn.p);
+    hipLaunchKernelGGL(k_sub, grid(nn), dim3(tb), 0, st, dPn.p, dP.p, dTmp.p, nn);
+    const double dch = bl.amax(nn, dTmp.p);
+    if (!std::isfinite(ce) || !std::isfinite(resid)) throw HfgError{HFG_INTEGRAL, "DF-HF energy/residual became nonfinite"};
+    if (std::fabs(ce - last) <= opt->energy_tol && resid <= opt->residual_tol && dch <= opt->density_tol) {
+      occ_copy(dCn.p);
+      fock(dF.p);
+      res->total_energy = energy(dPn.p, dF.p) + enuc;
+      res->iterations = it;
+      res->converged = 1;
+      res->n_aux = df.rank;
+      res->fock_builds = builds;
+      res->seconds_fock = t_fock;
+      if (p_out) download(p_out, dPn.p, nn, st);
+      if (f_out) download(f_out, dF.p, nn, st);
+      if (j_out) download(j_out, dJ.p, nn, st);
+      if (k_out) download(k_out, dK.p, nn, st);
+      res->seconds_total = std::chrono::duration<double>(clk::now() - t0).count();
+      return HFG_OK;
+    }
+    last = ce;
+    std::vector<double> F(nn);
+    download(F.data(), dF.p, nn, st);
+    if (fh.size() >= 8) {
+      fh.erase(fh.begin());
+      eh.erase(eh.begin());
+    }
+    fh.push_back(F);
+    eh.push_back(err);
+    const double* fuse = dF.p;
+    if (it >= 2 && eh.size() >= 2) {
+      const int k = (int)eh.size();
+      std::vector<double> B((size_t)(k + 1) * (k + 1), 0.0), rhs(k + 1, 0.0), c;
+      for (int i = 0; i < k; ++i) {
+        for (int j = 0; j < k; ++j) {
+          double d = 0;
+          for (size_t x = 0; x < mm2; ++x) d += eh[i][x] * eh[j][x];
+          B[(size_t)i * (k + 1) + j] = d;
+        }
+        B[(size_t)i * (k + 1) + k] = B[(size_t)k * (k + 1) + i] = -1.0;
+      }
+      rhs[k] = -1.0;
+      if (solve_small(B, rhs, k + 1, c)) {
+        std::vector<double> fu(nn, 0.0);
+        for (int i = 0; i < k; ++i)
+          for (size_t x = 0; x < nn; ++x) fu[x] += c[i] * fh[i][x];
+        upload(dFu.p, fu.data(), nn, st);
+        fuse = dFu.p;
+      }
+    }
+    diag(fuse, dC.p, dP.p);
+    occ_copy(dC.p);
+  }
+  res->iterations = opt->max_iter;
+  res->seconds_total = std::chrono::duration<double>(clk::now() - t0).count();
+  throw HfgError{HFG_NOT_CONVERGED, "factorized RHF did not converge"};
+}
+
+}  // namespace hfg
+
+using namespace hfg;
+
+static int dferr(int code, const std::string& m) { return set_error(code, m); }
+
+struct hfg_df {
+  std::unique_ptr<DfEngine> e;
+};
+
+extern "C" {
+
+int hfg_df_build(const hfg_basis_view* basis, const hfg_basis_view* aux, double metric_cutoff, hfg_df** out) {
+  if (!basis || !aux || !out) return dferr(HFG_INVALID_INPUT, "null argument");
+  try {
+    auto* h = new hfg_df;
+    try {
+      h->e.reset(new DfEngine(*basis, *aux, metric_cutoff));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return HFG_OK;
+  } catch (HfgError& e) {
+    return dferr(e.code, e.msg);
+  } catch (std::exception& e) {
+    return dferr(HFG_DEVICE, e.what());
+  }
+}
+
+void hfg_df_destroy(hfg_df* df) { delete df; }
+
+int hfg_df_info(const hfg_df* df, int64_t* n, int64_t* naux, int64_t* rank) {
+  if (!df) return dferr(HFG_INVALID_INPUT, "null handle");
+  if (n) *n = df->e->n;
+  if (naux) *naux = df->e->naux;
+  if (rank) *rank = df->e->rank;
+  return HFG_OK;
+}
+
+int hfg_df_jk(hfg_df* df, const double* cocc, int nocc, double* j, double* k) {
+  if (!df || !cocc || !j || !k || nocc <= 0) return dferr(HFG_INVALID_INPUT, "invalid occupied coefficients for DF J/K");
+  try {
+    DfEngine& e = *df->e;
+    const size_t nn = (size_t)e.n * e.n;
+    for (size_t x = 0; x < (size_t)e.n * nocc; ++x)
+      if (!std::isfinite(cocc[x])) return dferr(HFG_INVALID_INPUT, "invalid occupied coefficients for DF J/K");
+    Dev dc, dj, dk;
+    dc.alloc((size_t)e.n * nocc);
+    dj.alloc(nn);
+    dk.alloc(nn);
+    upload(dc.p, cocc, (size_t)e.n * nocc, e.stream);
+    e.jk(dc.p, nocc, dj.p, dk.p);
+    download(j, dj.p, nn, e.stream);
+    download(k, dk.p, nn, e.stream);
+    return HFG_OK;
+  } catch (HfgError& ex) {
+    return dferr(ex.code, ex.msg);
+  }
+}
+
+int hfg_df_rhf(const hfg_basis_view* basis, const hfg_basis_view* aux, const double* zxyz, int natom, int charge,
+               const hfg_dfhf_options* opt, hfg_dfhf_result* res, double* p_out, double* f_out, double* j_out,
+               double* k_out) {
+  if (!basis || !aux || !zxyz || !opt || !res) return dferr(HFG_INVALID_INPUT, "null argument");
+  try {
+    return df_rhf(basis, aux, zxyz, natom, charge, opt, res, p_out, f_out, j_out, k_out);
+  } catch (HfgError& e) {
+    return dferr(e.code, e.msg);
+  }
+}
+
+}  // extern "C"

@@ -37,6 +37,18 @@ struct BlockTask {
   long long off;
 };
 
+// Arguments of the block kernels: one thread per task computes the unscaled
+// block of pair `bra` with pair `ket` (kernel orientation) at out + off.
+struct BlockArgs {
+  const BlockTask* tasks;
+  int ntask;
+  const PairDev* pairs;
+  PrimPairs pp;
+  const double* boys;
+  double* out;
+};
+void launch_block_class(int cls, const BlockArgs& a, hipStream_t s);
+
 #define HIP_CHECK(expr)                                                                           \
   do {                                                                                            \
     hipError_t _e = (expr);                                                                       \
@@ -63,6 +75,8 @@ struct DevBuf {
 };
 
 double md_flops(int la, int lb, int lc, int ld);
+// Records the message returned by hfg_last_error(); returns code.
+int set_error(int code, const std::string& msg);
 
 struct Engine {
   int device = 0;
@@ -76,7 +90,7 @@ struct Engine {
   int64_t n_quartets = 0;
   double screen = 1e-12;        // Q_ab*Q_cd > screen (src/shell_integrals.rs:184,226)
 
-  DevBuf d_pp, d_kt, d_groups, d_pairs, d_jkpairs, d_boys, d_scale, d_tasks, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
+  DevBuf d_pp, d_pps, d_kt, d_groups, d_pairs, d_jkpairs, d_boys, d_scale, d_tasks, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
   std::vector<size_t> task_begin;  // per class, into d_tasks (HFG_NCLASS+1 entries)
   std::vector<double> class_flops;      // algorithmic FLOPs per launch
   std::vector<int64_t> class_quartets;  // quartets per launch
@@ -90,12 +104,19 @@ struct Engine {
   ~Engine();
   PrimPairs pp_view() const;
   PrimPairs kt_view() const;
+  PrimPairs pps_view() const;
+  size_t pps_size = 0;
   size_t kt_size = 0;
 
   void compute_schwarz();
   void build_tasks();
   // Blocks of the listed quartets (kernel orientation, unscaled) into host out.
-  void blocks(const std::vector<BlockTask>& tasks, const std::vector<int>& cls, std::vector<double>& out);
+  void blocks(const std::vector<BlockTask>& tasks, const std::vector<int>& cls, std::vector<double>& out,
+              const PairDev* pair_table = nullptr);
+  // Primitive pairs kept for the J/K work (see engine.hip): per pair [s0, s1) into pps.
+  void screen_primitives(std::vector<int>& s0, std::vector<int>& s1, PrimPairHost& pps);
+  double prim_screen = 1e-20;
+  size_t prim_pairs_kept = 0;
   // J/K from the device density d_D into d_J, d_K (and F = H + J - K/2 when H given).
   void jk_device(const double* dP, double* dJ, double* dK, const double* dH = nullptr, double* dF = nullptr);
   void sync();

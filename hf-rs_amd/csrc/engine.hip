@@ -27,6 +27,20 @@
 
 namespace hfg {
 
+// J/K partial sums live in one copy per XCC (accelerator complex die, each
+// with its own L2).  A wave never migrates, so every update to copy x comes
+// from XCC x and a workgroup-scope atomic -- performed in that XCC's L2 --
+// is enough; no update has to travel to another die's cache.  The copies are
+// summed by k_symmetrize after the kernel boundary.  The XCC id is the 4-bit
+// hardware register HIP's __smid also reads; NXCC_SLOTS covers all 16 values.
+constexpr int NXCC_SLOTS = 16;
+__device__ __forceinline__ unsigned xcc_id() {
+  return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (NXCC_SLOTS - 1);
+}
+__device__ __forceinline__ void add_local(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -40,6 +54,7 @@ struct JKArgs {
   const KetGroup* groups;
   PrimPairs pp;  // bra side, contiguous per pair
   PrimPairs kt;  // ket side, transposed per group
+  size_t nn;      // n*n, the stride between per-XCC accumulator copies
   const double* scale;
   const double* boys;
   const double* D;
@@ -50,10 +65,7 @@ struct JKArgs {
 // Boys slice of the class's order M into LDS (see eri_device.h).
 template <int M>
 __device__ __forceinline__ void load_boys_slice(const double* __restrict__ g, double* __restrict__ lds) {
-  for (int i = threadIdx.x; i < BOYS_ROWS * BOYS_SLICE; i += blockDim.x) {
-    const int r = i / BOYS_SLICE, k = i - r * BOYS_SLICE;
-    lds[i] = k == 0 ? g[r * BOYS_ROW] : g[r * BOYS_ROW + M + k];
-  }
+  for (int r = threadIdx.x; r < BOYS_ROWS; r += blockDim.x) boys_slice_row<M>(g, r, lds + r * BOYS_SLICE);
   __syncthreads();
 }
 
@@ -115,6 +127,9 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
         for (int l = 0; l < ND; ++l) I[((i * NB + j) * NC + k) * ND + l] *= sa[i] * sb[j] * sc[k] * sd[l];
 
   const double* __restrict__ Dm = a.D;
+  const size_t slot = (size_t)xcc_id() * a.nn;
+  double* __restrict__ Jx = a.Jacc + slot;
+  double* __restrict__ Kx = a.Kacc + slot;
   // Coulomb, bra side: reduce over the wave (all lanes share the bra pair).
   {
     double Dcd[NCD];
@@ -130,7 +145,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
 #pragma unroll
         for (int kl = 0; kl < NCD; ++kl) s += Dcd[kl] * I[(i * NB + j) * NCD + kl];
         s = wave_sum(s);
-        if (lane == 0) unsafeAtomicAdd(&a.Jacc[(size_t)(A0 + i) * n + B0 + j], 2.0 * s);
+        if (lane == 0) add_local(&Jx[(size_t)(A0 + i) * n + B0 + j], 2.0 * s);
       }
   }
   if (!act) return;
@@ -147,7 +162,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
         double s = 0.0;
 #pragma unroll
         for (int ij = 0; ij < NAB; ++ij) s += Dab[ij] * I[ij * NCD + k * ND + l];
-        unsafeAtomicAdd(&a.Jacc[(size_t)(C0 + k) * n + D0 + l], 2.0 * s);
+        add_local(&Jx[(size_t)(C0 + k) * n + D0 + l], 2.0 * s);
       }
   }
   // Exchange.
@@ -166,7 +181,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
         for (int j = 0; j < NB; ++j)
 #pragma unroll
           for (int l = 0; l < ND; ++l) s += Dx[j * ND + l] * I[((i * NB + j) * NC + k) * ND + l];
-        unsafeAtomicAdd(&a.Kacc[(size_t)(A0 + i) * n + C0 + k], s);
+        add_local(&Kx[(size_t)(A0 + i) * n + C0 + k], s);
       }
   }
   {
@@ -184,7 +199,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
         for (int i = 0; i < NA; ++i)
 #pragma unroll
           for (int l = 0; l < ND; ++l) s += Dx[i * ND + l] * I[((i * NB + j) * NC + k) * ND + l];
-        unsafeAtomicAdd(&a.Kacc[(size_t)(B0 + j) * n + C0 + k], s);
+        add_local(&Kx[(size_t)(B0 + j) * n + C0 + k], s);
       }
   }
   {
@@ -202,7 +217,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
         for (int j = 0; j < NB; ++j)
 #pragma unroll
           for (int k = 0; k < NC; ++k) s += Dx[j * NC + k] * I[((i * NB + j) * NC + k) * ND + l];
-        unsafeAtomicAdd(&a.Kacc[(size_t)(A0 + i) * n + D0 + l], s);
+        add_local(&Kx[(size_t)(A0 + i) * n + D0 + l], s);
       }
   }
   {
@@ -220,21 +235,12 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
         for (int i = 0; i < NA; ++i)
 #pragma unroll
           for (int k = 0; k < NC; ++k) s += Dx[i * NC + k] * I[((i * NB + j) * NC + k) * ND + l];
-        unsafeAtomicAdd(&a.Kacc[(size_t)(B0 + j) * n + D0 + l], s);
+        add_local(&Kx[(size_t)(B0 + j) * n + D0 + l], s);
       }
   }
 }
 
 // One thread per quartet: the unscaled block in kernel orientation.
-struct BlockArgs {
-  const BlockTask* tasks;
-  int ntask;
-  const PairDev* pairs;
-  PrimPairs pp;
-  const double* boys;
-  double* out;
-};
-
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void k_block(BlockArgs a) {
   constexpr int NT = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
@@ -260,10 +266,15 @@ __global__ void k_fill(double* p, size_t n, double v) {
 __global__ void k_symmetrize(const double* Jacc, const double* Kacc, double* J, double* K, int n, const double* H,
                              double* F) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)n * n) return;
+  const size_t nn = (size_t)n * n;
+  if (idx >= nn) return;
   const int r = (int)(idx / n), c = (int)(idx % n);
-  const double j = Jacc[idx] + Jacc[(size_t)c * n + r];
-  const double k = Kacc[idx] + Kacc[(size_t)c * n + r];
+  const size_t tr = (size_t)c * n + r;
+  double j = 0.0, k = 0.0;
+  for (int x = 0; x < NXCC_SLOTS; ++x) {
+    j += Jacc[x * nn + idx] + Jacc[x * nn + tr];
+    k += Kacc[x * nn + idx] + Kacc[x * nn + tr];
+  }
   if (J) J[idx] = j;
   if (K) K[idx] = k;
   if (F) F[idx] = H[idx] + j - 0.5 * k;
@@ -295,9 +306,9 @@ static const JKLaunch JK_LAUNCH[HFG_NCLASS] = {
     HFG_FOR_EACH_CLASS(X)
 #undef X
 };
-static const BlockLaunch BLOCK_LAUNCH[HFG_NCLASS] = {
+static const BlockLaunch BLOCK_LAUNCH[HFG_NCLASS_ALL] = {
 #define X(a, b, c, d) &launch_block<a, b, c, d>,
-    HFG_FOR_EACH_CLASS(X)
+    HFG_FOR_EACH_CLASS_ALL(X)
 #undef X
 };
 static const int CLASS_L[HFG_NCLASS][4] = {
@@ -307,6 +318,11 @@ static const int CLASS_L[HFG_NCLASS][4] = {
 };
 
 static int nc(int L) { return (L + 1) * (L + 2) / 2; }
+
+void launch_block_class(int cls, const BlockArgs& a, hipStream_t s) {
+  BLOCK_LAUNCH[cls](a, s);
+  HIP_CHECK(hipGetLastError());
+}
 
 Engine::Engine(const hfg_basis_view& b) {
   shells = group_shells(b);
@@ -358,8 +374,8 @@ Engine::Engine(const hfg_basis_view& b) {
   HIP_CHECK(hipMemcpyAsync(d_scale.p, ao_scale.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
   const size_t nn = (size_t)n * n * sizeof(double);
   d_D.alloc(nn);
-  d_Jacc.alloc(nn);
-  d_Kacc.alloc(nn);
+  d_Jacc.alloc(nn * NXCC_SLOTS);
+  d_Kacc.alloc(nn * NXCC_SLOTS);
   d_J.alloc(nn);
   d_K.alloc(nn);
   d_H.alloc(nn);
@@ -403,6 +419,86 @@ double md_flops(int la, int lb, int lc, int ld) {
 
 void Engine::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
 
+PrimPairs Engine::pps_view() const {
+  const double* b = d_pps.as<double>();
+  const size_t m = pps_size;
+  return PrimPairs{b, b + m, b + 2 * m, b + 3 * m, b + 4 * m, b + 5 * m, b + 6 * m, b + 7 * m, b + 8 * m};
+}
+
+// Primitive-pair screening for the J/K work.  B_q = sqrt(max_ij |(ij|ij)|) of
+// the single primitive pair q (component factors applied) bounds every
+// primitive integral it enters: |(q|r)| <= B_q B_r.  A primitive pair is left
+// out of the J/K kernels when B_q * max_r B_r < prim_screen (1e-20), so an
+// integral changes by at most K_ab K_cd * 1e-20 (<= 7e-17 for 81 x 81
+// primitive pairs).  The reference's shell engine evaluates every primitive
+// (src/shell_integrals.rs:101-133); shell blocks, Schwarz bounds and the
+// packed tensor here still do.
+void Engine::screen_primitives(std::vector<int>& s0, std::vector<int>& s1, PrimPairHost& pps) {
+  const size_t np = pp.size();
+  std::vector<PairDev> tmp(np);
+  std::vector<BlockTask> tasks;
+  std::vector<int> cls;
+  std::vector<long long> off(np);
+  std::vector<int> owner(np);
+  std::vector<size_t> order(pairs.size());
+  std::iota(order.begin(), order.end(), 0);
+  auto pcl = [&](size_t i) { return pair_class(pairs[i].la, pairs[i].lb); };
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return pcl(x) < pcl(y); });
+  long long o = 0;
+  for (size_t i : order) {
+    const PairInfo& p = pairs[i];
+    const int nab = nc(p.la) * nc(p.lb);
+    for (int q = p.p0; q < p.p1; ++q) {
+      PairDev& d = tmp[q];
+      std::memset(&d, 0, sizeof d);
+      d.p0 = q;
+      d.p1 = q + 1;
+      d.ao_a = p.ao_a;
+      d.ao_b = p.ao_b;
+      d.sh_a = p.sh_a;
+      d.sh_b = p.sh_b;
+      for (int k = 0; k < 3; ++k) d.AB[k] = p.AB[k];
+      tasks.push_back({q, q, o});
+      cls.push_back(quartet_class(pcl(i), pcl(i)));
+      off[q] = o;
+      owner[q] = (int)i;
+      o += (long long)nab * nab;
+    }
+  }
+  DevBuf dtmp;
+  dtmp.alloc(std::max<size_t>(1, np) * sizeof(PairDev));
+  HIP_CHECK(hipMemcpyAsync(dtmp.p, tmp.data(), np * sizeof(PairDev), hipMemcpyHostToDevice, stream));
+  std::vector<double> out(o);
+  blocks(tasks, cls, out, dtmp.as<PairDev>());
+  std::vector<double> B(np, 0.0);
+  double bmax = 0.0;
+  for (size_t q = 0; q < np; ++q) {
+    const PairInfo& p = pairs[owner[q]];
+    const int na = nc(p.la), nb = nc(p.lb), nab = na * nb;
+    double m = 0.0;
+    for (int x = 0; x < na; ++x)
+      for (int y = 0; y < nb; ++y) {
+        const int ij = x * nb + y;
+        const double f = ao_scale[p.ao_a + x] * ao_scale[p.ao_b + y];
+        m = std::max(m, std::fabs(out[off[q] + (size_t)ij * nab + ij]) * f * f);
+      }
+    B[q] = std::sqrt(m);
+    bmax = std::max(bmax, B[q]);
+  }
+  const std::vector<double>* src[9] = {&pp.p, &pp.Px, &pp.Py, &pp.Pz, &pp.PAx, &pp.PAy, &pp.PAz, &pp.K, &pp.H};
+  std::vector<double>* dst[9] = {&pps.p, &pps.Px, &pps.Py, &pps.Pz, &pps.PAx, &pps.PAy, &pps.PAz, &pps.K, &pps.H};
+  s0.assign(pairs.size(), 0);
+  s1.assign(pairs.size(), 0);
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    s0[i] = (int)pps.size();
+    for (int q = pairs[i].p0; q < pairs[i].p1; ++q)
+      if (B[q] * bmax >= prim_screen)
+        for (int c = 0; c < 9; ++c) dst[c]->push_back((*src[c])[q]);
+    s1[i] = (int)pps.size();
+  }
+  prim_pairs_kept = pps.size();
+}
+
 PrimPairs Engine::kt_view() const {
   const double* b = d_kt.as<double>();
   const size_t m = kt_size;
@@ -415,7 +511,8 @@ PrimPairs Engine::pp_view() const {
   return PrimPairs{b, b + np, b + 2 * np, b + 3 * np, b + 4 * np, b + 5 * np, b + 6 * np, b + 7 * np, b + 8 * np};
 }
 
-void Engine::blocks(const std::vector<BlockTask>& tasks, const std::vector<int>& cls, std::vector<double>& out) {
+void Engine::blocks(const std::vector<BlockTask>& tasks, const std::vector<int>& cls, std::vector<double>& out,
+                    const PairDev* pair_table) {
   // tasks are grouped by class: cls[i] is the class of tasks[i]; offsets are
   // into `out`, whose size the caller has set.
   DevBuf dt, dout;
@@ -426,8 +523,8 @@ void Engine::blocks(const std::vector<BlockTask>& tasks, const std::vector<int>&
   while (i < tasks.size()) {
     size_t j = i;
     while (j < tasks.size() && cls[j] == cls[i]) ++j;
-    BlockArgs a{dt.as<BlockTask>() + i, (int)(j - i), d_pairs.as<PairDev>(), pp_view(), d_boys.as<double>(),
-                dout.as<double>()};
+    BlockArgs a{dt.as<BlockTask>() + i, (int)(j - i), pair_table ? pair_table : d_pairs.as<PairDev>(), pp_view(),
+                d_boys.as<double>(), dout.as<double>()};
     BLOCK_LAUNCH[cls[i]](a, stream);
     HIP_CHECK(hipGetLastError());
     i = j;
@@ -477,13 +574,18 @@ void Engine::build_tasks() {
     int pc, npp;
     std::vector<int> idx;
     std::vector<double> q;
+    std::vector<double> full;
   };
+  std::vector<int> s0, s1;
+  PrimPairHost pps;
+  screen_primitives(s0, s1, pps);
   std::vector<Group> groups;
   for (size_t i = 0; i < pairs.size(); ++i) {
-    const int pc = pair_class(pairs[i].la, pairs[i].lb), npp = pairs[i].p1 - pairs[i].p0;
+    const int pc = pair_class(pairs[i].la, pairs[i].lb), npp = s1[i] - s0[i];
+    if (npp == 0) continue;  // every primitive pair negligible: no J/K work
     auto it = std::find_if(groups.begin(), groups.end(), [&](const Group& g) { return g.pc == pc && g.npp == npp; });
     if (it == groups.end()) {
-      groups.push_back({pc, npp, {}, {}});
+      groups.push_back({pc, npp, {}, {}, {}});
       it = groups.end() - 1;
     }
     it->idx.push_back((int)i);
@@ -491,6 +593,9 @@ void Engine::build_tasks() {
   for (auto& g : groups) {
     std::stable_sort(g.idx.begin(), g.idx.end(), [&](int x, int y) { return Q[x] > Q[y]; });
     for (int i : g.idx) g.q.push_back(Q[i]);
+    // unscreened primitive counts, prefix-summed, for the algorithmic FLOP count
+    g.full.assign(1, 0.0);
+    for (int i : g.idx) g.full.push_back(g.full.back() + (pairs[i].p1 - pairs[i].p0));
   }
   // Heavy groups first inside each class.
   std::sort(groups.begin(), groups.end(), [](const Group& x, const Group& y) {
@@ -528,7 +633,7 @@ void Engine::build_tasks() {
         if (tri) nk = std::min(nk, i + 1);
         n_quartets += (int64_t)nk;
         class_quartets[cls] += (int64_t)nk;
-        const double pq = (double)nk * B.npp * K.npp;
+        const double pq = (double)(B.full[i + 1] - B.full[i]) * K.full[nk];
         prim_quartets += pq;
         class_flops[cls] += pq * md_flops(CLASS_L[cls][0], CLASS_L[cls][1], CLASS_L[cls][2], CLASS_L[cls][3]);
         for (size_t k0 = 0; k0 < nk; k0 += 64) {
@@ -543,7 +648,7 @@ void Engine::build_tasks() {
     }
   // Ket pairs of one task must be consecutive pair ids: renumber pairs so that
   // every group is a contiguous, Q-sorted range.
-  std::vector<int> newid(pairs.size()), oldid;
+  std::vector<int> newid(pairs.size(), -1), oldid;
   std::vector<int> gstart(groups.size());
   for (size_t g = 0; g < groups.size(); ++g) {
     gstart[g] = (int)oldid.size();
@@ -552,13 +657,13 @@ void Engine::build_tasks() {
       oldid.push_back(i);
     }
   }
-  std::vector<PairDev> pd(pairs.size());
+  std::vector<PairDev> pd(std::max<size_t>(1, oldid.size()));
   for (size_t k = 0; k < oldid.size(); ++k) {
     const PairInfo& p = pairs[oldid[k]];
     PairDev& d = pd[k];
     std::memset(&d, 0, sizeof d);
-    d.p0 = p.p0;
-    d.p1 = p.p1;
+    d.p0 = s0[oldid[k]];
+    d.p1 = s1[oldid[k]];
     d.ao_a = p.ao_a;
     d.ao_b = p.ao_b;
     d.sh_a = p.sh_a;
@@ -584,15 +689,21 @@ void Engine::build_tasks() {
     tot += (size_t)groups[g].npp * groups[g].idx.size();
   }
   kt_size = tot;
-  const std::vector<double>* cols[9] = {&pp.p, &pp.Px, &pp.Py, &pp.Pz, &pp.PAx, &pp.PAy, &pp.PAz, &pp.K, &pp.H};
+  const std::vector<double>* cols[9] = {&pps.p, &pps.Px, &pps.Py, &pps.Pz, &pps.PAx, &pps.PAy, &pps.PAz, &pps.K, &pps.H};
   std::vector<double> kt(9 * std::max<size_t>(1, tot));
   for (size_t g = 0; g < groups.size(); ++g)
     for (size_t j = 0; j < groups[g].idx.size(); ++j) {
-      const PairInfo& p = pairs[groups[g].idx[j]];
+      const int first = s0[groups[g].idx[j]];
       for (int ik = 0; ik < groups[g].npp; ++ik)
         for (int c = 0; c < 9; ++c)
-          kt[c * tot + kg[g].base + (size_t)ik * kg[g].size + j] = (*cols[c])[p.p0 + ik];
+          kt[c * tot + kg[g].base + (size_t)ik * kg[g].size + j] = (*cols[c])[first + ik];
     }
+  pps_size = pps.size();
+  d_pps.alloc(9 * std::max<size_t>(1, pps_size) * sizeof(double));
+  for (int c = 0; c < 9; ++c)
+    if (pps_size)
+      HIP_CHECK(hipMemcpyAsync(d_pps.as<double>() + c * pps_size, cols[c]->data(), pps_size * sizeof(double),
+                               hipMemcpyHostToDevice, stream));
   d_kt.alloc(kt.size() * sizeof(double));
   HIP_CHECK(hipMemcpyAsync(d_kt.p, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice, stream));
   d_groups.alloc(std::max<size_t>(1, kg.size()) * sizeof(KetGroup));
@@ -608,13 +719,15 @@ void Engine::build_tasks() {
 void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* dH, double* dF) {
   const size_t nn = (size_t)n * n;
   const int tb = 256;
-  hipLaunchKernelGGL(k_fill, dim3((nn + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(), nn, 0.0);
-  hipLaunchKernelGGL(k_fill, dim3((nn + tb - 1) / tb), dim3(tb), 0, stream, d_Kacc.as<double>(), nn, 0.0);
+  const size_t nacc = nn * NXCC_SLOTS;
+  hipLaunchKernelGGL(k_fill, dim3((nacc + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(), nacc, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3((nacc + tb - 1) / tb), dim3(tb), 0, stream, d_Kacc.as<double>(), nacc, 0.0);
   JKArgs a;
   a.n = n;
+  a.nn = nn;
   a.pairs = d_jkpairs.as<PairDev>();
   a.groups = d_groups.as<KetGroup>();
-  a.pp = pp_view();
+  a.pp = pps_view();
   a.kt = kt_view();
   a.scale = d_scale.as<double>();
   a.boys = d_boys.as<double>();

@@ -57,41 +57,51 @@ constexpr double BOYS_TMAX = 50.0;
 constexpr int BOYS_ROWS = 402;
 
 // Per-class slice of the table, as the kernels keep it in LDS: row r holds
-// exp(-T_r) followed by F_M(T_r) .. F_{M+8}(T_r) (BOYS_SLICE doubles).
+// exp(-T_r) followed by the Taylor coefficients F_{M+k}(T_r)/k!, k = 0..8
+// (BOYS_SLICE doubles), so that each Horner step is one FMA.
 constexpr int BOYS_SLICE = BOYS_TAYLOR + 1;
 
 template <int M>
 HFG_DEV void boys_slice_row(const double* __restrict__ tab, int r, double* __restrict__ row) {
   row[0] = tab[r * BOYS_ROW];
-  for (int k = 1; k < BOYS_SLICE; ++k) row[k] = tab[r * BOYS_ROW + M + k];
+  double inv_fact = 1.0;
+  for (int k = 0; k < BOYS_TAYLOR; ++k) {
+    if (k > 0) inv_fact /= k;
+    row[1 + k] = tab[r * BOYS_ROW + 1 + M + k] * inv_fact;
+  }
 }
 
 template <int M>
 HFG_DEV void boys_eval(double T, const double* __restrict__ slice, double* __restrict__ F) {
   if (T < BOYS_TMAX) {
     const int r = (int)(T * BOYS_INV_STEP + 0.5);
-    const double dT = T - r * (1.0 / BOYS_INV_STEP);
+    const double x = r * (1.0 / BOYS_INV_STEP) - T;  // -dT, |dT| <= 1/16
     const double* row = slice + r * BOYS_SLICE;
-    // F_M(T) = sum_k F_{M+k}(T_r) (-dT)^k / k!   (Horner)
+    // F_M(T) = sum_k [F_{M+k}(T_r)/k!] (-dT)^k
     double acc = row[BOYS_TAYLOR];
 #pragma unroll
-    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = row[k] + acc * (-dT) * (1.0 / k);
+    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = fma(acc, x, row[k]);
     F[M] = acc;
     if (M > 0) {
-      // e^{-dT} by its Taylor series, |dT| <= 1/16
-      double e = 1.0;
+      // e^{-dT} = sum_k x^k/k!, 10 terms
+      constexpr double c[10] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040,
+                                1.0 / 40320, 1.0 / 362880};
+      double e = c[9];
 #pragma unroll
-      for (int k = 9; k >= 1; --k) e = 1.0 + e * (-dT) * (1.0 / k);
+      for (int k = 8; k >= 0; --k) e = fma(e, x, c[k]);
       const double emt = row[0] * e;
       const double t2 = 2.0 * T;
 #pragma unroll
       for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + emt) * (1.0 / (2 * m + 1));
     }
   } else {
+    // F_0 = sqrt(pi/T)/2 (erf(sqrt T) = 1 to 3e-24), upward recursion
+    // F_m = ((2m-1) F_{m-1} - e^{-T})/(2T).  For M <= 4 the e^{-T} terms
+    // (< 2e-22/T) change F_m by less than 2e-17 relative and are left out.
     const double rt = rsqrt_nr(T);
     F[0] = HALF_SQRT_PI * rt;
     if (M > 0) {
-      const double emt = exp(-T);
+      const double emt = M > 4 ? exp(-T) : 0.0;
       const double h = 0.5 * rt * rt;
 #pragma unroll
       for (int m = 1; m <= M; ++m) F[m] = ((2 * m - 1) * F[m - 1] - emt) * h;

@@ -249,11 +249,16 @@ def gen_class(la, lb, lc, ld):
     return "\n".join(out) + "\n", em.nflops
 
 
+# Pair classes: every (la, lb) with lb <= la <= 2 (orbital shell pairs up to
+# d), then (3,0) and (4,0): an auxiliary f or g shell paired with the
+# exponent-0 unit function of the density-fitting integrals (src/df.rs:30-40).
+PAIR_CLASSES = [(a, b) for a in range(LMAX + 1) for b in range(a + 1)] + [(3, 0), (4, 0)]
+
+
 def classes():
-    pcs = [(a, b) for a in range(LMAX + 1) for b in range(a + 1)]
     out = []
-    for i, (la, lb) in enumerate(pcs):
-        for (lc, ld) in pcs[:i + 1]:
+    for i, (la, lb) in enumerate(PAIR_CLASSES):
+        for (lc, ld) in PAIR_CLASSES[:i + 1]:
             out.append((la, lb, lc, ld))
     return out
 
@@ -270,6 +275,11 @@ def main(outdir):
         f.write("\n".join(parts))
     with open(os.path.join(outdir, "eri_classes.txt"), "w") as f:
         f.write("\n".join(report) + "\n")
+    # class list for the C++ side, in quartet_class() order
+    with open(os.path.join(outdir, "class_list.inc"), "w") as f:
+        f.write("// Generated by gen_eri.py: X(la,lb,lc,ld) for every class, quartet_class() order.\n")
+        f.write("#define HFG_FOR_EACH_CLASS_ALL(X) " + " ".join("X(%d, %d, %d, %d)" % c for c in classes()) + "\n")
+        f.write("#define HFG_NCLASS_ALL %d\n" % len(classes()))
 
 
 if __name__ == "__main__":

@@ -94,6 +94,21 @@ void append_pair(const Shell& A, const Shell& B, PrimPairHost& pp) {
     }
 }
 
+void append_aux_pair(const Shell& A, PrimPairHost& pp) {
+  for (int k = 0; k < A.K; ++k) {
+    const double p = A.e[k];
+    pp.p.push_back(p);
+    pp.Px.push_back(A.c[0]);
+    pp.Py.push_back(A.c[1]);
+    pp.Pz.push_back(A.c[2]);
+    pp.PAx.push_back(0.0);
+    pp.PAy.push_back(0.0);
+    pp.PAz.push_back(0.0);
+    pp.K.push_back(A.r[k] / p);
+    pp.H.push_back(0.5 / p);
+  }
+}
+
 void build_pairs(const std::vector<Shell>& sh, std::vector<PairInfo>& pairs, PrimPairHost& pp) {
   pairs.clear();
   const int ns = (int)sh.size();
@@ -144,8 +159,10 @@ std::vector<double> boys_table() {
 }
 
 int pair_class(int la, int lb) {
-  if (la > 2 || lb > la) return -1;
-  return la * (la + 1) / 2 + lb;  // ss0 ps1 pp2 ds3 dp4 dd5
+  if (lb > la) return -1;
+  if (la <= 2) return la * (la + 1) / 2 + lb;  // ss0 ps1 pp2 ds3 dp4 dd5
+  if (lb == 0 && la <= 4) return 3 + la;     // fs6 gs7 (auxiliary shell with the unit function)
+  return -1;
 }
 int quartet_class(int pb, int pk) { return pb * (pb + 1) / 2 + pk; }
 

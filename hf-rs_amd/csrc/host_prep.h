@@ -44,6 +44,9 @@ struct PrimPairHost {
 
 void build_pairs(const std::vector<Shell>& sh, std::vector<PairInfo>& pairs, PrimPairHost& pp);
 void append_pair(const Shell& A, const Shell& B, PrimPairHost& pp);  // kernel orientation A, B
+// Auxiliary shell P paired with the exponent-0 unit function at its centre
+// (src/df.rs:30-40): p = alpha, P = C, PA = 0, K = c / alpha.
+void append_aux_pair(const Shell& P, PrimPairHost& pp);
 
 // Boys table (layout documented in eri_device.h), computed in long double.
 std::vector<double> boys_table();

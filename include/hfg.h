@@ -132,6 +132,36 @@ int hfg_scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int 
                 const hfg_scf_options* opt, hfg_scf_result* res, double* p_out, double* f_out,
                 double* eps_out);
 
+/* Density fitting with a Coulomb metric.  hfg_df_build computes the metric
+ * (P|Q), its canonical orthogonaliser X (eigenvalues below metric_cutoff
+ * dropped), the 3-index integrals (P|mu nu) with the unit function of
+ * src/df.rs:30-40 and B = X^T (P|mu nu), kept in HBM.  Replaces
+ * DfTensor::build (src/df.rs:89-127; Cartesian auxiliary space). */
+typedef struct hfg_df hfg_df;
+int hfg_df_build(const hfg_basis_view* basis, const hfg_basis_view* aux, double metric_cutoff,
+                 hfg_df** out);
+void hfg_df_destroy(hfg_df* df);
+int hfg_df_info(const hfg_df* df, int64_t* n, int64_t* n_aux, int64_t* rank);
+/* J and K of the RHF density P = 2 C_occ C_occ^T from occupied coefficients
+ * (n x nocc).  Replaces DfTensor::coulomb_exchange_rhf (src/df.rs:133-174). */
+int hfg_df_jk(hfg_df* df, const double* cocc, int nocc, double* j, double* k);
+
+/* DFHFOptions (src/hf_variants/ri_df_hf.rs:21-45) and DFHFResult (:47-64). */
+typedef struct {
+  int32_t max_iter;
+  double energy_tol, residual_tol, density_tol, metric_cutoff;
+} hfg_dfhf_options;
+typedef struct {
+  double total_energy;
+  int32_t iterations, converged, n_aux, fock_builds;
+  double seconds_total, seconds_fock;
+} hfg_dfhf_result;
+/* DF-RHF.  Replaces run_df_hf_with_aux (src/hf_variants/ri_df_hf.rs:122-155)
+ * and run_factorized_rhf (:160-263).  Optional outputs may be NULL. */
+int hfg_df_rhf(const hfg_basis_view* basis, const hfg_basis_view* aux, const double* zxyz, int natom,
+               int charge, const hfg_dfhf_options* opt, hfg_dfhf_result* res, double* p_out,
+               double* f_out, double* j_out, double* k_out);
+
 /* Host-side basis construction from Gaussian-94 text, mirroring build_basis
  * (src/basis.rs:296-496).  The arrays are owned by the returned handle. */
 typedef struct hfg_basis hfg_basis;

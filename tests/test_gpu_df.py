@@ -1,0 +1,93 @@
+"""Density fitting on the GPU vs the oracle and the PySCF fixtures
+(tests/df_hf_test.rs:29-163 style): J/K for given occupied coefficients,
+DF-RHF energy and every J, K, F, P element at 1e-9."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import molecules as M
+from conftest import GOLDEN, load_package
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hf():
+    mod = load_package()
+    if mod.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP engine")
+    return mod
+
+
+def water_fixture():
+    ref = json.load(open(os.path.join(GOLDEN, "df_hf_water.json")))["water"]
+    syms = [a[0] for a in ref["atoms"]]
+    xyz = np.array([a[1] for a in ref["atoms"]])
+    return ref, syms, xyz
+
+
+@pytest.mark.parametrize("obas,abas,grid", [("sto-3g", "def2-svp-rifit", None),
+                                            ("cc-pvdz", "def2-universal-jkfit", (2, 1, 1))])
+def test_df_jk_matches_oracle(hf, obas, abas, grid):
+    if grid is None:
+        _, syms, xyz = water_fixture()
+    else:
+        mol = M.water_cluster(grid, seed=2)
+        syms, xyz = mol.symbols, mol.xyz
+    b = hf.Basis.from_g94(M.basis_text(obas), syms, xyz)
+    a = hf.Basis.from_g94(M.basis_text(abas), syms, xyz)
+    df = hf.DfTensor(b, a)
+    ob = O.Basis.build(M.basis_text(obas), syms, xyz)
+    oa = O.Basis.build(M.basis_text(abas), syms, xyz)
+    bt = O.df_build(ob, oa)
+    assert df.rank == bt.shape[0]
+    rng = np.random.default_rng(4)
+    c = rng.standard_normal((b.n, 5)) * 0.3
+    j, k = df.coulomb_exchange_rhf(c)
+    jr, kr = O.df_jk(bt, b.n, c)
+    print("%s/%s n=%d rank=%d  J %.2e K %.2e" % (obas, abas, b.n, df.rank, np.abs(j - jr).max(), np.abs(k - kr).max()))
+    assert np.abs(j - jr).max() < 1e-10 and np.abs(k - kr).max() < 1e-10
+
+
+def test_df_rhf_water_matches_pyscf_every_element(hf):
+    ref, syms, xyz = water_fixture()
+    b = hf.Basis.from_g94(M.basis_text("sto-3g"), syms, xyz)
+    a = hf.Basis.from_g94(M.basis_text("def2-svp-rifit"), syms, xyz)
+    zxyz = np.column_stack([[M.Z[s] for s in syms], xyz])
+    r = hf.df_rhf(b, a, zxyz)
+    assert r["converged"] and r["n_aux"] == int(ref["n_aux"])
+    assert abs(r["total_energy"] - float(ref["e_scf"])) < 1e-9
+    for name, key in (("j", "j"), ("k", "k"), ("fock", "fock"), ("density", "density")):
+        assert np.abs(r[key] - np.array(ref[name])).max() < 1e-9, name
+
+
+def test_df_rhf_water_dimer_matches_oracle(hf):
+    mol = M.water_cluster((2, 1, 1), seed=2)
+    ob, oa = M.basis_text("cc-pvdz"), M.basis_text("def2-universal-jkfit")
+    r = hf.df_rhf(hf.Basis.from_g94(ob, mol.symbols, mol.xyz), hf.Basis.from_g94(oa, mol.symbols, mol.xyz), mol.zxyz)
+    o, _, _, _, _ = O.df_rhf(O.Basis.build(ob, mol.symbols, mol.xyz), O.Basis.build(oa, mol.symbols, mol.xyz),
+                             mol.zxyz)
+    print("DF dimer: E %.12f vs %.12f, iters %d vs %d" % (r["total_energy"], o.total_energy, r["iterations"],
+                                                         o.iterations))
+    assert r["converged"] and o.converged
+    assert abs(r["total_energy"] - o.total_energy) < 1e-8
+    assert abs(r["iterations"] - o.iterations) <= 1
+
+
+def test_df_rejects_bad_inputs(hf):
+    _, syms, xyz = water_fixture()
+    b = hf.Basis.from_g94(M.basis_text("sto-3g"), syms, xyz)
+    a = hf.Basis.from_g94(M.basis_text("def2-svp-rifit"), syms, xyz)
+    with pytest.raises(hf.HfError):
+        hf.DfTensor(b, a, metric_cutoff=-1.0)
+    df = hf.DfTensor(b, a)
+    with pytest.raises(hf.HfError):
+        df.coulomb_exchange_rhf(np.full((b.n, 2), np.nan))
+    zxyz = np.column_stack([[M.Z[s] for s in syms], xyz])
+    with pytest.raises(hf.HfError):
+        hf.df_rhf(b, a, zxyz, charge=1)  # odd electron count
+    with pytest.raises(hf.HfError):
+        hf.df_rhf(b, a, zxyz, max_iter=1)  # cannot converge in one iteration

@@ -114,3 +114,18 @@ def test_generated_recurrences_match_oracle_on_contracted_shells():
         assert L.hfgc_shell_quartet(C.byref(v), *q, out.ctypes.data) == 0
         worst = max(worst, np.abs(out.reshape(exp.shape) - exp).max())
     assert worst < 1e-12, worst
+
+
+def test_kernel_boys_path_against_mpmath_fixture():
+    """The kernels' per-class evaluation (LDS slice, T >= 50 branch) for M = 0..10."""
+    L = cpucheck()
+    L.hfgc_boys_class.argtypes = [C.c_int, C.c_double, C.c_void_p]
+    worst = 0.0
+    for row in json.load(open(os.path.join(GOLDEN, "boys.json")))["rows"]:
+        for M in range(11):
+            out = np.zeros(M + 1)
+            assert L.hfgc_boys_class(M, row["t"], out.ctypes.data) == 0
+            for m in range(M + 1):
+                ref = float(row["f"][m])
+                worst = max(worst, abs(out[m] - ref) / ref)
+    assert worst < 5e-15, worst
