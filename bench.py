@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="run only the counter-calibration copy kernel over BYTES bytes and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine's workloads are single-GPU (see DESIGN.md, sharding); "
@@ -66,6 +68,11 @@ def main():
         sys.exit(2)
     hf = load_pkg()
     sys.path.insert(0, PKG)
+    if args.calibrate_copy:
+        ms = hf.calibration_copy(args.calibrate_copy)
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "ms": ms,
+                          "GB_per_s": 2 * args.calibrate_copy / ms / 1e6}))
+        return
     import molecules as M
     if hf.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -124,7 +131,14 @@ def main():
         r = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
         line["scf"] = {"solves_per_s": 1.0 / r["seconds_total"], "seconds": r["seconds_total"],
                        "iterations": r["iterations"], "converged": r["converged"], "fock_builds": r["fock_builds"],
-                       "seconds_fock": r["seconds_fock"], "energy": r["total_energy"]}
+                       "energy": r["total_energy"],
+                       "phases_s": {k: r["seconds_" + k] for k in ("setup", "one_electron", "guess", "fock",
+                                                                   "linalg")}}
+        ref = os.path.join(ROOT, "tests", "golden", "h2o16_ccpvdz_scf.json")
+        if os.path.exists(ref):
+            o = json.load(open(ref))
+            line["scf"]["dE_vs_oracle"] = r["total_energy"] - o["total_energy"]
+            line["scf"]["oracle_iterations"] = o["iterations"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mol, text, args.cpu_stride, args.cpu_threads, dens)
     print(json.dumps(line), flush=True)

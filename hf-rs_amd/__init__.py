@@ -45,7 +45,8 @@ class ScfResult(C.Structure):
     _fields_ = [("total_energy", C.c_double), ("electronic_energy", C.c_double),
                 ("nuclear_repulsion", C.c_double), ("iterations", C.c_int32), ("converged", C.c_int32),
                 ("n_occ", C.c_int32), ("fock_builds", C.c_int32), ("seconds_total", C.c_double),
-                ("seconds_fock", C.c_double)]
+                ("seconds_fock", C.c_double), ("seconds_setup", C.c_double), ("seconds_one_electron", C.c_double),
+                ("seconds_guess", C.c_double), ("seconds_linalg", C.c_double)]
 
 
 EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_destroy", "hfg_ctx_info",
@@ -53,7 +54,7 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_contract_jk_device", "hfg_scf_rhf", "hfg_basis_from_g94", "hfg_basis_free",
             "hfg_basis_get_view", "hfg_set_density", "hfg_jk_resident", "hfg_get_jk", "hfg_profile",
             "hfg_class_stats", "hfg_fock_flops", "hfg_md_flops", "hfg_persistent_blocks", "hfg_df_build",
-            "hfg_df_destroy", "hfg_df_info", "hfg_df_jk", "hfg_df_rhf"]
+            "hfg_df_destroy", "hfg_df_info", "hfg_df_jk", "hfg_df_rhf", "hfg_calibration_copy"]
 
 
 class DfhfOptions(C.Structure):
@@ -98,6 +99,8 @@ def lib():
         L.hfg_fock_flops.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl)]
         L.hfg_md_flops.argtypes = [i32] * 4
         L.hfg_md_flops.restype = dbl
+        L.hfg_calibration_copy.argtypes = [C.c_size_t, C.POINTER(dbl)]
+        L.hfg_persistent_blocks.restype = i32
         L.hfg_df_build.argtypes = [C.POINTER(BasisView), C.POINTER(BasisView), dbl, C.POINTER(vp)]
         L.hfg_df_destroy.argtypes = [vp]
         L.hfg_df_info.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3
@@ -119,6 +122,13 @@ def _p(a):
 
 def md_flops(la, lb, lc, ld):
     return lib().hfg_md_flops(la, lb, lc, ld)
+
+
+def calibration_copy(nbytes):
+    """Time (ms) of one plain device-to-device copy of nbytes (counter calibration)."""
+    ms = C.c_double()
+    _check(lib().hfg_calibration_copy(int(nbytes), C.byref(ms)))
+    return ms.value
 
 
 def device_count():

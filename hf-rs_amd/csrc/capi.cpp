@@ -299,6 +299,14 @@ double hfg_md_flops(int la, int lb, int lc, int ld) { return md_flops(la, lb, lc
 
 int hfg_persistent_blocks(void) { return persistent_blocks(); }
 
+int hfg_calibration_copy(size_t bytes, double* ms) {
+  if (!ms || bytes < 64) return set_err(HFG_INVALID_INPUT, "invalid calibration size");
+  GUARD({
+    *ms = calibration_copy(bytes);
+    return HFG_OK;
+  })
+}
+
 int hfg_scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
                 hfg_scf_result* res, double* p_out, double* f_out, double* eps_out) {
   if (!basis || !zxyz || !opt || !res) return set_err(HFG_INVALID_INPUT, "null argument");

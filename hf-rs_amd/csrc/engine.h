@@ -125,5 +125,6 @@ struct Engine {
 
 void class_angular(int c, int* l4);
 int persistent_blocks();
+double calibration_copy(size_t bytes);
 
 }  // namespace hfg

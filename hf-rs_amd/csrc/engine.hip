@@ -258,6 +258,36 @@ __global__ __launch_bounds__(64) void k_block(BlockArgs a) {
   for (int i = 0; i < NT; ++i) o[i] = I[i];
 }
 
+__global__ void k_fill(double* p, size_t n, double v);
+
+__global__ void k_copy_calibration(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+double calibration_copy(size_t bytes) {
+  const size_t n = bytes / sizeof(double4);
+  DevBuf a, b;
+  a.alloc(n * sizeof(double4));
+  b.alloc(n * sizeof(double4));
+  hipStream_t s;
+  HIP_CHECK(hipStreamCreate(&s));
+  hipLaunchKernelGGL(k_fill, dim3((4 * n + 255) / 256), dim3(256), 0, s, a.as<double>(), 4 * n, 1.0);
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  HIP_CHECK(hipEventRecord(e0, s));
+  hipLaunchKernelGGL(k_copy_calibration, dim3(4096), dim3(256), 0, s, a.as<double4>(), b.as<double4>(), n);
+  HIP_CHECK(hipEventRecord(e1, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  return ms;
+}
+
 __global__ void k_fill(double* p, size_t n, double v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

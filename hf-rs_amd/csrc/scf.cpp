@@ -367,7 +367,15 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   const int n = basis->n_functions;
   if (nocc > n) throw HfgError{HFG_INVALID_INPUT, "more occupied orbitals than basis functions"};
 
+  auto t_phase = clk::now();
+  auto lap = [&]() {
+    const auto now = clk::now();
+    const double d = std::chrono::duration<double>(now - t_phase).count();
+    t_phase = now;
+    return d;
+  };
   Engine eng(*basis);
+  res->seconds_setup = lap();
   hipStream_t st = eng.stream;
   Blas bl(st);
   const size_t nn = (size_t)n * n;
@@ -381,6 +389,7 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   one_electron(eng.shells, n, atoms, S, T, V);
   HMat H(nn);
   for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
+  res->seconds_one_electron = lap();
 
   // Orthogonaliser X = Tr * U_kept s^{-1/2} of S' = Tr^T S Tr (src/ao.rs:58-67).
   HMat Tr;
@@ -473,6 +482,8 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
     upload(dP.p, P0.data(), nn, st);
   else
     diag_density(dH.p, dP.p, false);
+  HIP_CHECK(hipStreamSynchronize(st));
+  res->seconds_guess = lap();
 
   // DIIS histories (device), src/diis.rs:72-137 and 289-359
   const int MAXS = 8;
@@ -627,6 +638,8 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   if (eps_out) std::memcpy(eps_out, eps.data(), m * sizeof(double));
   HIP_CHECK(hipStreamSynchronize(st));
   res->seconds_total = std::chrono::duration<double>(clk::now() - t_start).count();
+  res->seconds_linalg = res->seconds_total - res->seconds_setup - res->seconds_one_electron - res->seconds_guess -
+                        res->seconds_fock;
   return conv ? HFG_OK : HFG_NOT_CONVERGED;
 }
 

@@ -104,6 +104,10 @@ int hfg_class_stats(hfg_ctx* ctx, int cls, int32_t* l4, double* flops_per_launch
  * and src/integrals.rs:70-101, 259-346, 905-971. */
 int hfg_fock_flops(hfg_ctx* ctx, double* flops, double* prim_quartets);
 double hfg_md_flops(int la, int lb, int lc, int ld);
+/* Calibration for hardware-counter traffic: copies `bytes` bytes between two
+ * fresh device buffers with a plain streaming kernel (k_copy_calibration) and
+ * returns its time in milliseconds.  Instrumentation, no reference counterpart. */
+int hfg_calibration_copy(size_t bytes, double* ms);
 /* Workgroups of one persistent class launch (4 per compute unit by default,
  * HFG_PERSISTENT_BLOCKS overrides); instrumentation, no reference counterpart. */
 int hfg_persistent_blocks(void);
@@ -122,6 +126,9 @@ typedef struct {
   double total_energy, electronic_energy, nuclear_repulsion;
   int32_t iterations, converged, n_occ, fock_builds;
   double seconds_total, seconds_fock;
+  /* phase breakdown: engine set-up (pairs, Schwarz bounds, work lists),
+   * one-electron matrices, initial guess, eigensolver + DIIS */
+  double seconds_setup, seconds_one_electron, seconds_guess, seconds_linalg;
 } hfg_scf_result;
 
 /* Restricted Hartree-Fock with device-resident Fock builds.  zxyz: per atom

@@ -174,3 +174,17 @@ def test_rhf_water_dimer_matches_oracle(hf, conv, spherical):
     assert r["converged"] and o.converged
     assert abs(r["total_energy"] - o.total_energy) < 1e-8
     assert abs(r["iterations"] - o.iterations) <= 1
+
+
+def test_rhf_h2o16_ccpvdz_matches_oracle_at_full_size(hf):
+    """The bench workload itself: energy within 1e-8 Eh and the same iteration
+    count (+-1) as the oracle's in-core run (tests/golden/make_h2o16_reference.py)."""
+    ref = json.load(open(os.path.join(GOLDEN, "h2o16_ccpvdz_scf.json")))
+    mol = M.water_cluster((4, 2, 2), seed=2)
+    r = hf.scf_rhf(hf.Basis.from_g94(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz), mol.zxyz, tol=1e-10,
+                   conv="adiissoscf", spherical=True)
+    print("(H2O)16: E %.12f vs oracle %.12f, iterations %d vs %d" % (r["total_energy"], ref["total_energy"],
+                                                                     r["iterations"], ref["iterations"]))
+    assert r["converged"]
+    assert abs(r["total_energy"] - ref["total_energy"]) < 1e-8
+    assert abs(r["iterations"] - ref["iterations"]) <= 1
