@@ -9,33 +9,35 @@
 #include "classes.h"
 #include "eri_device.h"
 #include "host_prep.h"
+#include "jk_plan.h"
 
 namespace hfg {
 
-// Shell pair as the kernels read it (kernel orientation, see PairInfo).
-struct PairDev {
-  int p0, p1, ao_a, ao_b, sh_a, sh_b, pad0, pad1;
-  double AB[3], pad2;
-};
+// Number of per-XCC copies of the J/K accumulators (see jk_kernel.h).
+constexpr int NXCC_SLOTS = 16;
 
-// One wave of work: bra pair, first ket pair and number of ket pairs (<= 64);
-// all ket pairs share one primitive count.
-struct JKTask {
-  int bra, ket0, nket, grp;
+// Arguments of the J/K kernels (jk_kernel.h).
+struct JKArgs {
+  const JKTask* tasks;
+  int ntask, n;
+  const ViewDev* views;
+  const ComboDev* combos;
+  const KetGroup* groups;
+  PrimPairs bp;          // bra side, contiguous per view
+  const double* bcoef;   // bra coefficients [prim][cap] per view
+  PrimPairs kt;          // ket side, transposed per group
+  const double* kcoef;   // ket coefficients, transposed per group
+  size_t nn;             // n*n, the stride between per-XCC accumulator copies
+  double screen;         // Schwarz threshold of a combination pair
+  const double* scale;
+  const double* boys;
+  const double* D;
+  double* Jacc;
+  double* Kacc;
 };
-
-// A group of ket pairs of one class and primitive count, contiguous in the
-// J/K pair order; its primitive data is stored transposed,
-// X[base + prim * size + (pair - start)], so that the lanes of a wave read
-// consecutive addresses.
-struct KetGroup {
-  int start, size, base, npp;
-};
-
-struct BlockTask {
-  int bra, ket;
-  long long off;
-};
+// Launch of class cls's J/K kernel over ntask tasks with at most max_blocks
+// workgroups; the kernels are split over three translation units.
+void launch_jk_class(int cls, const JKArgs& a, int ntask, int max_blocks, hipStream_t s);
 
 // Arguments of the block kernels: one thread per task computes the unscaled
 // block of pair `bra` with pair `ket` (kernel orientation) at out + off.
@@ -90,7 +92,8 @@ struct Engine {
   int64_t n_quartets = 0;
   double screen = 1e-12;        // Q_ab*Q_cd > screen (src/shell_integrals.rs:184,226)
 
-  DevBuf d_pp, d_pps, d_kt, d_groups, d_pairs, d_jkpairs, d_boys, d_scale, d_tasks, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
+  DevBuf d_pp, d_vb, d_bcoef, d_kt, d_kcoef, d_groups, d_pairs, d_views, d_combos, d_boys, d_scale, d_tasks, d_D, d_Jacc,
+      d_Kacc, d_J, d_K, d_H, d_F;
   std::vector<size_t> task_begin;  // per class, into d_tasks (HFG_NCLASS+1 entries)
   std::vector<double> class_flops;      // algorithmic FLOPs per launch
   std::vector<int64_t> class_quartets;  // quartets per launch
@@ -104,19 +107,23 @@ struct Engine {
   ~Engine();
   PrimPairs pp_view() const;
   PrimPairs kt_view() const;
-  PrimPairs pps_view() const;
-  size_t pps_size = 0;
+  PrimPairs vb_view() const;
+  size_t vb_size = 0;
   size_t kt_size = 0;
 
   void compute_schwarz();
+  // The J/K work plan (jk_plan.cpp): generally contracted shells, views,
+  // ket groups and per-class task lists; then the algorithmic work count.
   void build_tasks();
-  // Blocks of the listed quartets (kernel orientation, unscaled) into host out.
+  void count_work();
+  // Blocks of the listed quartets (kernel orientation, unscaled) into host out;
+  // pair_table / prims replace the engine's shell-pair tables when given.
   void blocks(const std::vector<BlockTask>& tasks, const std::vector<int>& cls, std::vector<double>& out,
-              const PairDev* pair_table = nullptr);
-  // Primitive pairs kept for the J/K work (see engine.hip): per pair [s0, s1) into pps.
-  void screen_primitives(std::vector<int>& s0, std::vector<int>& s1, PrimPairHost& pps);
+              const PairDev* pair_table = nullptr, const PrimPairs* prims = nullptr);
   double prim_screen = 1e-20;
-  size_t prim_pairs_kept = 0;
+  size_t prim_pairs_kept = 0;  // primitive pairs over all views
+  int n_gen_shells = 0;
+  int n_views = 0;
   // J/K from the device density d_D into d_J, d_K (and F = H + J - K/2 when H given).
   void jk_device(const double* dP, double* dJ, double* dK, const double* dH = nullptr, double* dF = nullptr);
   void sync();

@@ -152,6 +152,15 @@ HFG_DEV void eri_block(const PrimPairs& bp, int b0, int b1, const PrimPairs& kp,
 
 constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
+#ifndef HFG_HOST_CHECK
+// Boys slice of order M into LDS by the whole workgroup (layout above).
+template <int M>
+__device__ __forceinline__ void load_boys_slice(const double* __restrict__ g, double* __restrict__ lds) {
+  for (int r = threadIdx.x; r < BOYS_ROWS; r += blockDim.x) boys_slice_row<M>(g, r, lds + r * BOYS_SLICE);
+  __syncthreads();
+}
+#endif
+
 }  // namespace hfg
 
 namespace hfg {

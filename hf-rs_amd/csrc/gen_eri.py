@@ -144,6 +144,84 @@ def levels(lo, hi):
     return out
 
 
+def hrr_lines(la, lb, lc, ld, acc, pre, base):
+    """Horizontal recurrences on contracted [e0|f0] values `acc` (names keyed
+    by (e, f)), writing the Cartesian block to I[base ...]; `pre` keeps the
+    temporaries of different contraction combinations apart."""
+    hl = []
+    memo = {}
+
+    def bra(a, b, f):
+        # (a b | f0)
+        key = ("b", a, b, f)
+        if key in memo:
+            return memo[key]
+        if sum(b) == 0:
+            memo[key] = acc[(a, f)]
+            return memo[key]
+        i = choose_axis(b, (0, 0, 0))
+        b1 = sub(b, i)
+        x = bra(add(a, i), b1, f)
+        y = bra(a, b1, f)
+        name = "%shb%s_%s_%s" % (pre, tag(a), tag(b), tag(f))
+        hl.append("  const double %s = %s + AB[%d] * %s;" % (name, x, i, y))
+        memo[key] = name
+        return name
+
+    def ket(a, b, c, d):
+        key = ("k", a, b, c, d)
+        if key in memo:
+            return memo[key]
+        if sum(d) == 0:
+            return bra(a, b, c)
+        i = choose_axis(d, (0, 0, 0))
+        d1 = sub(d, i)
+        x = ket(a, b, add(c, i), d1)
+        y = ket(a, b, c, d1)
+        name = "%shk%s%s_%s_%s" % (pre, tag(a), tag(b), tag(c), tag(d))
+        hl.append("  const double %s = %s + CD[%d] * %s;" % (name, x, i, y))
+        memo[key] = name
+        return name
+
+    idx = base
+    for a in cart(la):
+        for b in cart(lb):
+            for c in cart(lc):
+                for d in cart(ld):
+                    v = ket(a, b, c, d)
+                    hl.append("  I[%d] = %s;" % (idx, v))
+                    idx += 1
+    return hl
+
+
+def ket_prim_lines(la, lb, lc, ld):
+    """Per primitive quartet inside the ket loop: geometry, Boys values scaled
+    by the prefactor (F0..FM) and the recurrence coefficients."""
+    M = la + lb + lc + ld
+    o = []
+    o.append("      const int ik = k0 + jk * ks;")
+    o.append("      const double q = kp.p[ik];")
+    o.append("      const double PQx = Px - kp.Px[ik], PQy = Py - kp.Py[ik], PQz = Pz - kp.Pz[ik];")
+    o.append("      const double rs = rsqrt_nr(p + q);  // 1/sqrt(p+q)")
+    o.append("      const double inv_s = rs * rs;")
+    o.append("      const double rho = p * q * inv_s;")
+    o.append("      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    o.append("      const double pref = TWO_PI_2_5 * Kp * kp.K[ik] * rs;")
+    o.append("      double Fm[%d];" % (M + 1))
+    o.append("      boys_eval<%d>(T, boys, Fm);" % M)
+    for m in range(M + 1):
+        o.append("      const double F%d = pref * Fm[%d];" % (m, m))
+    if M > 0:
+        if la + lb > 0:
+            o.append("      const double rp = q * inv_s;")
+            o.append("      const double WPx = -rp * PQx, WPy = -rp * PQy, WPz = -rp * PQz;")
+        if lc + ld > 0:
+            o.append("      const double QCx = kp.PAx[ik], QCy = kp.PAy[ik], QCz = kp.PAz[ik], oo2q = kp.H[ik];")
+            o.append("      const double rq = p * inv_s, oo2s = 0.5 * inv_s;")
+            o.append("      const double WQx = rq * PQx, WQy = rq * PQy, WQz = rq * PQz;")
+    return o
+
+
 def gen_class(la, lb, lc, ld):
     """Return C++ source of eri_<la><lb><lc><ld>(...)."""
     M = la + lb + lc + ld
@@ -172,26 +250,7 @@ def gen_class(la, lb, lc, ld):
     if M > 0 and la + lb > 0:
         out.append("    const double PAx = bp.PAx[ib], PAy = bp.PAy[ib], PAz = bp.PAz[ib], oo2p = bp.H[ib];")
     out.append("    for (int jk = 0; jk < nk; ++jk) {")
-    out.append("      const int ik = k0 + jk * ks;")
-    out.append("      const double q = kp.p[ik];")
-    out.append("      const double PQx = Px - kp.Px[ik], PQy = Py - kp.Py[ik], PQz = Pz - kp.Pz[ik];")
-    out.append("      const double rs = rsqrt_nr(p + q);  // 1/sqrt(p+q)")
-    out.append("      const double inv_s = rs * rs;")
-    out.append("      const double rho = p * q * inv_s;")
-    out.append("      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
-    out.append("      const double pref = TWO_PI_2_5 * Kp * kp.K[ik] * rs;")
-    out.append("      double Fm[%d];" % (M + 1))
-    out.append("      boys_eval<%d>(T, boys, Fm);" % M)
-    for m in range(M + 1):
-        out.append("      const double F%d = pref * Fm[%d];" % (m, m))
-    if M > 0:
-        if la + lb > 0:
-            out.append("      const double rp = q * inv_s;")
-            out.append("      const double WPx = -rp * PQx, WPy = -rp * PQy, WPz = -rp * PQz;")
-        if lc + ld > 0:
-            out.append("      const double QCx = kp.PAx[ik], QCy = kp.PAy[ik], QCz = kp.PAz[ik], oo2q = kp.H[ik];")
-            out.append("      const double rq = p * inv_s, oo2s = 0.5 * inv_s;")
-            out.append("      const double WQx = rq * PQx, WQy = rq * PQy, WQz = rq * PQz;")
+    out += ket_prim_lines(la, lb, lc, ld)
     vrr = VRR(em)
     for e in E:
         for f in Fk:
@@ -201,52 +260,110 @@ def gen_class(la, lb, lc, ld):
     out.append("    }")
     out.append("  }")
     # Horizontal recurrences on the contracted values.
-    hl = []
-    memo = {}
-
-    def bra(a, b, f):
-        # (a b | f0)
-        key = ("b", a, b, f)
-        if key in memo:
-            return memo[key]
-        if sum(b) == 0:
-            memo[key] = acc[(a, f)]
-            return memo[key]
-        i = choose_axis(b, (0, 0, 0))
-        b1 = sub(b, i)
-        x = bra(add(a, i), b1, f)
-        y = bra(a, b1, f)
-        name = "hb%s_%s_%s" % (tag(a), tag(b), tag(f))
-        hl.append("  const double %s = %s + AB[%d] * %s;" % (name, x, i, y))
-        memo[key] = name
-        return name
-
-    def ket(a, b, c, d):
-        key = ("k", a, b, c, d)
-        if key in memo:
-            return memo[key]
-        if sum(d) == 0:
-            return bra(a, b, c)
-        i = choose_axis(d, (0, 0, 0))
-        d1 = sub(d, i)
-        x = ket(a, b, add(c, i), d1)
-        y = ket(a, b, c, d1)
-        name = "hk%s%s_%s_%s" % (tag(a), tag(b), tag(c), tag(d))
-        hl.append("  const double %s = %s + CD[%d] * %s;" % (name, x, i, y))
-        memo[key] = name
-        return name
-
-    idx = 0
-    for a in cart(la):
-        for b in cart(lb):
-            for c in cart(lc):
-                for d in cart(ld):
-                    v = ket(a, b, c, d)
-                    hl.append("  I[%d] = %s;" % (idx, v))
-                    idx += 1
-    out += hl
+    out += hrr_lines(la, lb, lc, ld, acc, "", 0)
     out.append("}")
     return "\n".join(out) + "\n", em.nflops
+
+
+# Contraction members evaluated together by the J/K kernel, per quartet
+# class: (RA, RB | RC, RD).  A pair of generally contracted shells (several
+# contractions over one exponent set) is split into views of at most RA
+# members of its first shell and RB of its second; the CA = RA * RB bra and
+# CC = RC * RD ket combinations of a view share every primitive integral.  The
+# kernel holds CA x CC contracted blocks in registers, so the caps keep
+# CA * CC * (accumulators per block) within about 96 doubles.  Classes not
+# listed evaluate one combination.
+GC_CAPS = {
+    (0, 0, 0, 0): (3, 3, 3, 3),
+    (1, 0, 0, 0): (1, 3, 3, 3),
+    (1, 0, 1, 0): (1, 3, 1, 3),
+    (1, 1, 0, 0): (1, 1, 3, 3),
+    (1, 1, 1, 0): (1, 1, 1, 3),
+    (2, 0, 0, 0): (1, 1, 3, 3),
+    (2, 0, 1, 0): (1, 1, 1, 3),
+    (2, 1, 0, 0): (1, 1, 1, 3),
+}
+
+
+def gc_caps(c):
+    return GC_CAPS.get(tuple(c), (1, 1, 1, 1))
+
+
+def gen_class_gc(la, lb, lc, ld):
+    """eri_gc<la,lb,lc,ld>: blocks of CA x CC contraction combinations of one
+    bra pair and one ket pair that share their primitive pairs.
+
+    Primitive data carry no contraction coefficients when a side has more than
+    one combination (cap > 1); the coefficients come from bc[(ib - b0) * CA + g] (bra,
+    wave-uniform) and kc[(jk * CC + g) * ks] (ket, one pair per lane).  With
+    cap 1 the single combination's coefficient is folded into K as in
+    eri_block.  Contraction runs ket-first: per bra primitive the ket sums
+    t[gcd] = sum_k c_k[gcd] [e0|f0]_k are formed, then a[gab][gcd] +=
+    c_b[gab] t[gcd].  Output: I[(gab * CC + gcd) * NAB * NCD + ...]."""
+    RA, RB, RC, RD = gc_caps((la, lb, lc, ld))
+    CA, CC = RA * RB, RC * RD
+    M = la + lb + lc + ld
+    em = Emitter()
+    E = levels(la, la + lb)
+    Fk = levels(lc, lc + ld)
+    nab, ncd = ncart(la) * ncart(lb), ncart(lc) * ncart(ld)
+    out = []
+    out.append("// (%s%s|%s%s) x %d x %d combinations" % ("spdfg"[la], "spdfg"[lb], "spdfg"[lc], "spdfg"[ld], CA, CC))
+    out.append("template <> HFG_DEV void eri_gc<%d,%d,%d,%d>(" % (la, lb, lc, ld))
+    out.append("    const PrimPairs& bp, const double* __restrict__ bc, int b0, int b1, const PrimPairs& kp,")
+    out.append("    const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],")
+    out.append("    const double* __restrict__ boys, double* __restrict__ I) {")
+    acc = {}
+    for ga in range(CA):
+        for gc in range(CC):
+            for e in E:
+                for f in Fk:
+                    acc[(ga, gc, e, f)] = "a%d_%d_%s_%s" % (ga, gc, tag(e), tag(f))
+    out.append("  double " + ", ".join("%s = 0.0" % v for v in acc.values()) + ";")
+    out.append("  for (int ib = b0; ib < b1; ++ib) {")
+    out.append("    const double p = bp.p[ib], Px = bp.Px[ib], Py = bp.Py[ib], Pz = bp.Pz[ib], Kp = bp.K[ib];")
+    if M > 0 and la + lb > 0:
+        out.append("    const double PAx = bp.PAx[ib], PAy = bp.PAy[ib], PAz = bp.PAz[ib], oo2p = bp.H[ib];")
+    if CA > 1:
+        tmp = {}
+        for gc in range(CC):
+            for e in E:
+                for f in Fk:
+                    tmp[(gc, e, f)] = "t%d_%s_%s" % (gc, tag(e), tag(f))
+        out.append("    double " + ", ".join("%s = 0.0" % v for v in tmp.values()) + ";")
+        tgt = lambda gc, e, f: tmp[(gc, e, f)]
+    else:
+        tgt = lambda gc, e, f: acc[(0, gc, e, f)]
+    out.append("    for (int jk = 0; jk < nk; ++jk) {")
+    out += ket_prim_lines(la, lb, lc, ld)
+    if CC > 1:
+        out.append("      const double " + ", ".join("ck%d = kc[(jk * %d + %d) * ks]" % (g, CC, g) for g in range(CC)) + ";")
+    vrr = VRR(em)
+    for e in E:
+        for f in Fk:
+            n = vrr.node(e, f, 0)
+            for gc in range(CC):
+                if CC > 1:
+                    em.emit("      %s += ck%d * %s;" % (tgt(gc, e, f), gc, n))
+                else:
+                    em.emit("      %s += %s;" % (tgt(gc, e, f), n))
+    out += em.lines
+    out.append("    }")
+    if CA > 1:
+        out.append("    const double " + ", ".join("cb%d = bc[(ib - b0) * %d + %d]" % (g, CA, g) for g in range(CA)) + ";")
+        for ga in range(CA):
+            for gc in range(CC):
+                for e in E:
+                    for f in Fk:
+                        out.append("    %s += cb%d * %s;" % (acc[(ga, gc, e, f)], ga, tmp[(gc, e, f)]))
+    out.append("  }")
+    for ga in range(CA):
+        for gc in range(CC):
+            sub_acc = {(e, f): acc[(ga, gc, e, f)] for e in E for f in Fk}
+            out += hrr_lines(la, lb, lc, ld, sub_acc, "g%d_%d_" % (ga, gc), (ga * CC + gc) * nab * ncd)
+    out.append("}")
+    return "\n".join(out) + "\n"
+
 
 
 # Pair classes: every (la, lb) with lb <= la <= 2 (orbital shell pairs up to
@@ -273,6 +390,11 @@ def main(outdir):
         report.append("%s %d" % ("".join(map(str, c)), fl))
     with open(os.path.join(outdir, "eri_classes.inc"), "w") as f:
         f.write("\n".join(parts))
+    jk = classes()[:21]  # orbital classes (bra and ket up to dd): the J/K kernel's
+    with open(os.path.join(outdir, "eri_gc.inc"), "w") as f:
+        f.write("// Generated by gen_eri.py -- do not edit.\n")
+        for c in jk:
+            f.write(gen_class_gc(*c))
     with open(os.path.join(outdir, "eri_classes.txt"), "w") as f:
         f.write("\n".join(report) + "\n")
     # class list for the C++ side, in quartet_class() order
@@ -280,6 +402,8 @@ def main(outdir):
         f.write("// Generated by gen_eri.py: X(la,lb,lc,ld) for every class, quartet_class() order.\n")
         f.write("#define HFG_FOR_EACH_CLASS_ALL(X) " + " ".join("X(%d, %d, %d, %d)" % c for c in classes()) + "\n")
         f.write("#define HFG_NCLASS_ALL %d\n" % len(classes()))
+        f.write("// Contraction-member caps (RA, RB, RC, RD) of the J/K classes, quartet_class() order.\n")
+        f.write("#define HFG_GC_CAPS " + ", ".join("{%d, %d, %d, %d}" % gc_caps(c) for c in jk) + "\n")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,319 @@
+// jk_kernel.h -- direct J/K digestion of screened shell quartets on gfx950.
+//
+// Work decomposition: one wavefront per task = one bra view (uniform across
+// the wave, so its primitive data and contraction coefficients are read
+// through scalar loads) and up to 64 ket views (one per lane) of one ket
+// group, drawn from a Schwarz-sorted list so that the surviving kets of a bra
+// are a prefix.  A view is a pair of generally contracted shells restricted to
+// at most RA x RB (bra) / RC x RD (ket) contraction members that share their
+// primitive pairs (see jk_plan.cpp).  Each lane evaluates the CA x CC blocks
+// of its quartet with the generated Head-Gordon-Pople recurrences (eri_gc),
+// then digests every combination whose hf-rs quartet passes the Schwarz test
+// into J and K:
+//   Jacc[ij] += 2w sum_kl D_kl (ij|kl)   (wave-reduced, one atomic per ij)
+//   Jacc[kl] += 2w sum_ij D_ij (ij|kl)
+//   Kacc[ik] += w sum_jl D_jl v, Kacc[jk] += w sum_il D_il v,
+//   Kacc[il] += w sum_jk D_jk v, Kacc[jl] += w sum_ik D_ik v
+// then J = Jacc + Jacc^T and K = Kacc + Kacc^T.  This equals contract_jk
+// (src/scf.rs:337-364) applied to the packed tensor of
+// compute_shell_eri_tensor (src/shell_integrals.rs:156-240) without ever
+// storing it.
+#pragma once
+#include <algorithm>
+
+#include "engine.h"
+#include "eri_device.h"
+
+namespace hfg {
+
+// Blocks of CA x CC contraction combinations (generated, see gen_eri.py).
+template <int LA, int LB, int LC, int LD>
+HFG_DEV void eri_gc(const PrimPairs& bp, const double* __restrict__ bc, int b0, int b1, const PrimPairs& kp,
+                    const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],
+                    const double* __restrict__ boys, double* __restrict__ I);
+#include "gen/eri_gc.inc"
+
+template <int LA, int LB, int LC, int LD>
+constexpr int class_index() {
+  int c = 0, found = -1;
+#define X(a, b, cc, d) \
+  if (a == LA && b == LB && cc == LC && d == LD) found = c; \
+  ++c;
+  HFG_FOR_EACH_CLASS(X)
+#undef X
+  return found;
+}
+
+// J/K partial sums live in one copy per XCC (accelerator complex die, each
+// with its own L2).  A wave never migrates, so every update to copy x comes
+// from XCC x and a workgroup-scope atomic -- performed in that XCC's L2 --
+// is enough; no update has to travel to another die's cache.  The copies are
+// summed by k_symmetrize after the kernel boundary.  The XCC id is the 4-bit
+// hardware register HIP's __smid also reads; NXCC_SLOTS covers all 16 values.
+__device__ __forceinline__ unsigned xcc_id() {
+  return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (NXCC_SLOTS - 1);
+}
+__device__ __forceinline__ void add_local(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One task: the bra view (wave-uniform) against up to 64 ket views.  The
+// CA x CC blocks of a lane are scaled (weight, component factors, and 0 for a
+// combination whose hf-rs quartet fails the Schwarz test or whose member is
+// missing), then digested with every sum over the members that do not address
+// the output taken in registers first: K[a_i, c_k] sums over the b and d
+// members, K[b_j, c_k] over a and d, K[a_i, d_l] over b and c, K[b_j, d_l]
+// over a and c, J[c_k, d_l] over a and b, and J[a_i, b_j] over c, d and the
+// 64 lanes (one wave reduction).  One atomic per output element and member
+// tuple remains.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restrict__ boys, int task, int lane) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND, NBLK = NAB * NCD;
+  constexpr int CLS = class_index<LA, LB, LC, LD>();
+  constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1], RC = GC_CAPS[CLS][2], RD = GC_CAPS[CLS][3];
+  constexpr int CA = RA * RB, CC = RC * RD;
+  const JKTask t = a.tasks[task];
+  const int bra = __builtin_amdgcn_readfirstlane(t.bra);
+  const int ket0 = __builtin_amdgcn_readfirstlane(t.ket0);
+  const int nket = __builtin_amdgcn_readfirstlane(t.nket);
+  const int grp = __builtin_amdgcn_readfirstlane(t.grp);
+  const bool act = lane < nket;
+  const int ket = ket0 + (act ? lane : 0);
+  const ViewDev& vb = a.views[bra];
+  const ViewDev& vk = a.views[ket];
+  const KetGroup g = a.groups[grp];
+  const int j0 = ket - g.start;
+  double I[CA * CC * NBLK];
+  eri_gc<LA, LB, LC, LD>(a.bp, a.bcoef + vb.coef0, vb.p0, vb.p1, a.kt, a.kcoef + g.cbase + j0, g.base + j0,
+                         act ? g.npp : 0, g.size, vb.AB, vk.AB, boys, I);
+
+  double w = 1.0;
+  if (vb.same) w *= 0.5;
+  if (vk.same) w *= 0.5;
+  if (bra == ket) w *= 0.5;
+  const int na = __builtin_amdgcn_readfirstlane(vb.na), nb = __builtin_amdgcn_readfirstlane(vb.nb);
+  const int nc = vk.na, nd = vk.nb;
+  const ComboDev* __restrict__ cbra = a.combos + vb.c0;
+  const ComboDev* __restrict__ cket = a.combos + vk.c0;
+  int Ao[RA], Bo[RB], Co[RC], Do[RD];
+#pragma unroll
+  for (int x = 0; x < RA; ++x) Ao[x] = __builtin_amdgcn_readfirstlane(cbra[x * RB].ao_a);
+#pragma unroll
+  for (int y = 0; y < RB; ++y) Bo[y] = __builtin_amdgcn_readfirstlane(cbra[y].ao_b);
+#pragma unroll
+  for (int z = 0; z < RC; ++z) Co[z] = cket[z * RD].ao_a;
+#pragma unroll
+  for (int u = 0; u < RD; ++u) Do[u] = cket[u].ao_b;
+  const double* __restrict__ sc_ = a.scale;
+  // scaling
+#pragma unroll
+  for (int xy = 0; xy < CA; ++xy) {
+    const double qb = cbra[xy].q;
+    const int x = xy / RB, y = xy % RB;
+#pragma unroll
+    for (int zu = 0; zu < CC; ++zu) {
+      const int z = zu / RD, u = zu % RD;
+      const double f = (act && qb * cket[zu].q > a.screen) ? w : 0.0;
+      double* blk = I + (xy * CC + zu) * NBLK;
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) {
+          const double fab = f * sc_[Ao[x] + i] * sc_[Bo[y] + jj];
+#pragma unroll
+          for (int k = 0; k < NC; ++k)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) blk[((i * NB + jj) * NC + k) * ND + l] *= fab * sc_[Co[z] + k] * sc_[Do[u] + l];
+        }
+    }
+  }
+
+  const double* __restrict__ Dm = a.D;
+  const int n = a.n;
+  const size_t slot = (size_t)xcc_id() * a.nn;
+  double* __restrict__ Jx = a.Jacc + slot;
+  double* __restrict__ Kx = a.Kacc + slot;
+  // Each term is its own pass over the blocks, one output element at a time,
+  // so that only the blocks, one density block and one sum are live.
+  // Coulomb, bra side: J[a_i, b_j] over c, d and the wave (all lanes share the bra).
+#pragma unroll
+  for (int x = 0; x < RA; ++x)
+#pragma unroll
+    for (int y = 0; y < RB; ++y) {
+      if (x >= na || y >= nb) continue;  // wave-uniform
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) {
+          double s = 0.0;
+#pragma unroll
+          for (int z = 0; z < RC; ++z)
+#pragma unroll
+            for (int u = 0; u < RD; ++u) {
+              const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK + (i * NB + jj) * NCD;
+#pragma unroll
+              for (int k = 0; k < NC; ++k)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) s += Dm[(size_t)(Co[z] + k) * n + Do[u] + l] * blk[k * ND + l];
+            }
+          s = wave_sum(s);
+          if (lane == 0) add_local(&Jx[(size_t)(Ao[x] + i) * n + Bo[y] + jj], 2.0 * s);
+        }
+    }
+  if (!act) return;
+  // Coulomb, ket side: J[c_k, d_l] over a and b.
+#pragma unroll
+  for (int z = 0; z < RC; ++z)
+#pragma unroll
+    for (int u = 0; u < RD; ++u) {
+      if (z >= nc || u >= nd) continue;
+#pragma unroll
+      for (int k = 0; k < NC; ++k)
+#pragma unroll
+        for (int l = 0; l < ND; ++l) {
+          double s = 0.0;
+#pragma unroll
+          for (int x = 0; x < RA; ++x)
+#pragma unroll
+            for (int y = 0; y < RB; ++y) {
+              const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK + k * ND + l;
+#pragma unroll
+              for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj) s += Dm[(size_t)(Ao[x] + i) * n + Bo[y] + jj] * blk[(i * NB + jj) * NCD];
+            }
+          add_local(&Jx[(size_t)(Co[z] + k) * n + Do[u] + l], 2.0 * s);
+        }
+    }
+  // Exchange: K[a_i, c_k] over b, d.
+#pragma unroll
+  for (int x = 0; x < RA; ++x)
+#pragma unroll
+    for (int z = 0; z < RC; ++z) {
+      if (x >= na || z >= nc) continue;
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+          double s = 0.0;
+#pragma unroll
+          for (int y = 0; y < RB; ++y)
+#pragma unroll
+            for (int u = 0; u < RD; ++u) {
+              const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK;
+#pragma unroll
+              for (int jj = 0; jj < NB; ++jj)
+#pragma unroll
+                for (int l = 0; l < ND; ++l)
+                  s += Dm[(size_t)(Bo[y] + jj) * n + Do[u] + l] * blk[((i * NB + jj) * NC + k) * ND + l];
+            }
+          add_local(&Kx[(size_t)(Ao[x] + i) * n + Co[z] + k], s);
+        }
+    }
+  // K[b_j, c_k] over a, d.
+#pragma unroll
+  for (int y = 0; y < RB; ++y)
+#pragma unroll
+    for (int z = 0; z < RC; ++z) {
+      if (y >= nb || z >= nc) continue;
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj)
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+          double s = 0.0;
+#pragma unroll
+          for (int x = 0; x < RA; ++x)
+#pragma unroll
+            for (int u = 0; u < RD; ++u) {
+              const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK;
+#pragma unroll
+              for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int l = 0; l < ND; ++l)
+                  s += Dm[(size_t)(Ao[x] + i) * n + Do[u] + l] * blk[((i * NB + jj) * NC + k) * ND + l];
+            }
+          add_local(&Kx[(size_t)(Bo[y] + jj) * n + Co[z] + k], s);
+        }
+    }
+  // K[a_i, d_l] over b, c.
+#pragma unroll
+  for (int x = 0; x < RA; ++x)
+#pragma unroll
+    for (int u = 0; u < RD; ++u) {
+      if (x >= na || u >= nd) continue;
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int l = 0; l < ND; ++l) {
+          double s = 0.0;
+#pragma unroll
+          for (int y = 0; y < RB; ++y)
+#pragma unroll
+            for (int z = 0; z < RC; ++z) {
+              const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK;
+#pragma unroll
+              for (int jj = 0; jj < NB; ++jj)
+#pragma unroll
+                for (int k = 0; k < NC; ++k)
+                  s += Dm[(size_t)(Bo[y] + jj) * n + Co[z] + k] * blk[((i * NB + jj) * NC + k) * ND + l];
+            }
+          add_local(&Kx[(size_t)(Ao[x] + i) * n + Do[u] + l], s);
+        }
+    }
+  // K[b_j, d_l] over a, c.
+#pragma unroll
+  for (int y = 0; y < RB; ++y)
+#pragma unroll
+    for (int u = 0; u < RD; ++u) {
+      if (y >= nb || u >= nd) continue;
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj)
+#pragma unroll
+        for (int l = 0; l < ND; ++l) {
+          double s = 0.0;
+#pragma unroll
+          for (int x = 0; x < RA; ++x)
+#pragma unroll
+            for (int z = 0; z < RC; ++z) {
+              const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK;
+#pragma unroll
+              for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int k = 0; k < NC; ++k)
+                  s += Dm[(size_t)(Ao[x] + i) * n + Co[z] + k] * blk[((i * NB + jj) * NC + k) * ND + l];
+            }
+          add_local(&Kx[(size_t)(Bo[y] + jj) * n + Do[u] + l], s);
+        }
+    }
+}
+
+// Persistent waves: every wave walks the class's task list with a stride of
+// the total wave count (tasks are ordered heaviest first), so the LDS copy of
+// the Boys slice is made once per workgroup.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void k_jk(JKArgs a) {
+  __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
+  load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
+  const int lane = threadIdx.x & 63;
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  for (int task = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); task < a.ntask; task += waves)
+    jk_task<LA, LB, LC, LD>(a, boys, task, lane);
+}
+
+template <int LA, int LB, int LC, int LD>
+void launch_jk(const JKArgs& a, int ntask, int max_blocks, hipStream_t s) {
+  JKArgs b = a;
+  b.ntask = ntask;
+  const int waves_per_block = 4;
+  const int blocks = std::min(max_blocks, (ntask + waves_per_block - 1) / waves_per_block);
+  hipLaunchKernelGGL((k_jk<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), 0, s, b);
+}
+
+}  // namespace hfg

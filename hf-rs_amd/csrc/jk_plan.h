@@ -1,0 +1,95 @@
+// jk_plan.h -- tables of the direct J/K kernel and their construction
+// (jk_plan.cpp).  Plain C++: the CPU check library builds the same plan.
+#pragma once
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <functional>
+#include <vector>
+
+#include "classes.h"
+#include "host_prep.h"
+
+namespace hfg {
+
+// Shell pair as the kernels read it (kernel orientation, see PairInfo).
+struct PairDev {
+  int p0, p1, ao_a, ao_b, sh_a, sh_b, pad0, pad1;
+  double AB[3], pad2;
+};
+
+// A view of a pair of generally contracted shells as the J/K kernel reads it:
+// na (<= RA) members of the first shell times nb (<= RB) members of the
+// second, kernel orientation (higher L first), with the primitive pairs that
+// matter for them.  Primitive data [p0, p1) in the view-contiguous bra
+// tables; contraction coefficients at coef0, [prim][RA * RB] (ga * RB + gb,
+// zero for missing members; only when RA * RB > 1, otherwise the single
+// combination's coefficient is folded into K); RA * RB combinations at c0 in
+// the combination table (missing members: Schwarz bound 0); same = 1 when
+// both sides are one shell.
+struct ViewDev {
+  int p0, p1, na, nb, c0, coef0, same, pad0;
+  double AB[3], pad2;
+};
+
+// One contraction combination = one hf-rs shell pair: AO offsets of its two
+// shells (kernel orientation) and its Schwarz bound.
+struct ComboDev {
+  int ao_a, ao_b;
+  double q;
+};
+
+// One wave of work: bra view, first ket view and number of ket views (<= 64);
+// all ket views share one primitive count.
+struct JKTask {
+  int bra, ket0, nket, grp;
+};
+
+// A group of ket views of one pair class, cap and primitive count, contiguous
+// in the view order; its primitive data is stored transposed,
+// X[base + prim * size + (view - start)], and its coefficients as
+// C[cbase + (prim * cap + g) * size + (view - start)], so that the lanes of a
+// wave read consecutive addresses.
+struct KetGroup {
+  int start, size, base, npp, cbase, pad0, pad1, pad2;
+};
+
+struct BlockTask {
+  int bra, ket;
+  long long off;
+};
+
+// Contraction-member caps (RA, RB, RC, RD) of the J/K classes.
+constexpr int GC_CAPS[HFG_NCLASS][4] = {HFG_GC_CAPS};
+
+// Everything the J/K kernels read besides the density, built on the host.
+struct JKPlan {
+  std::vector<ViewDev> views;
+  std::vector<ComboDev> combos;
+  std::vector<KetGroup> groups;
+  std::vector<double> vb;     // bra primitive pairs, 9 columns of vb_size
+  std::vector<double> bcoef;  // bra coefficients
+  std::vector<double> kt;     // ket primitive pairs, 9 columns of kt_size
+  std::vector<double> kcoef;  // ket coefficients
+  std::vector<JKTask> tasks;
+  std::vector<size_t> task_begin;  // HFG_NCLASS + 1 entries
+  size_t vb_size = 0, kt_size = 0, prim_pairs_kept = 0;
+  int n_gen_shells = 0;
+};
+
+// Unscaled blocks of the listed quartets (one task per quartet, grouped by
+// class, offsets into out) of primitive pair table `prims` through pair table
+// `tab`: supplied by the caller (GPU block kernels, or the CPU check).
+using BlockFn = std::function<void(const PrimPairHost& prims, const std::vector<PairDev>& tab,
+                                   const std::vector<BlockTask>& tasks, const std::vector<int>& cls,
+                                   std::vector<double>& out)>;
+
+// Q: Schwarz bound per hf-rs shell pair (index a(a+1)/2 + b, a >= b);
+// ao_scale: component factor per AO.
+// caps: contraction-member caps per class (GC_CAPS when null; other
+// values only for work-count studies on the CPU, the kernels are compiled for
+// GC_CAPS).
+JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>& Q, const std::vector<double>& ao_scale,
+                    double screen, double prim_screen, const BlockFn& blocks, const int (*caps)[4] = nullptr);
+
+}  // namespace hfg
