@@ -18,8 +18,8 @@ constexpr int NXCC_SLOTS = 16;
 
 // Arguments of the J/K kernels (jk_kernel.h).
 struct JKArgs {
-  const JKTask* tasks;
-  int ntask, n;
+  const JKTask* tasks;   // all classes' tasks; this class's start at task0
+  int task0, ntask, n;
   const ViewDev* views;
   const ComboDev* combos;
   const KetGroup* groups;
@@ -34,6 +34,11 @@ struct JKArgs {
   const double* D;
   double* Jacc;
   double* Kacc;
+  // LDS-row path (k_jk_rows): the class's jobs (absolute task offsets, njob + 1
+  // entries) and its job counter, zero at launch; njob = 0 selects k_jk.
+  const int* jobs;
+  int njob;
+  int* job_counter;
 };
 // Launch of class cls's J/K kernel over ntask tasks with at most max_blocks
 // workgroups; the kernels are split over three translation units.
@@ -92,9 +97,11 @@ struct Engine {
   int64_t n_quartets = 0;
   double screen = 1e-12;        // Q_ab*Q_cd > screen (src/shell_integrals.rs:184,226)
 
-  DevBuf d_pp, d_vb, d_bcoef, d_kt, d_kcoef, d_groups, d_pairs, d_views, d_combos, d_boys, d_scale, d_tasks, d_D, d_Jacc,
-      d_Kacc, d_J, d_K, d_H, d_F;
+  DevBuf d_pp, d_vb, d_bcoef, d_kt, d_kcoef, d_groups, d_pairs, d_views, d_combos, d_boys, d_scale, d_tasks, d_jobs,
+      d_counters, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
   std::vector<size_t> task_begin;  // per class, into d_tasks (HFG_NCLASS+1 entries)
+  std::vector<size_t> job_begin;   // per class, into d_jobs (HFG_NCLASS+1 entries)
+  bool use_rows = true;            // LDS-row J/K kernels where they fit (HFG_JK_ROWS=0: global only)
   std::vector<double> class_flops;      // algorithmic FLOPs per launch
   std::vector<int64_t> class_quartets;  // quartets per launch
   double prim_quartets = 0;

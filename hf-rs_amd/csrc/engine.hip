@@ -63,6 +63,10 @@ double calibration_copy(size_t bytes) {
   return ms;
 }
 
+__global__ void k_zero_int(int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 __global__ void k_fill(double* p, size_t n, double v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -319,6 +323,10 @@ void Engine::build_tasks() {
   up_load(d_views, plan.views.data(), plan.views.size() * sizeof(ViewDev));
   up_load(d_combos, plan.combos.data(), plan.combos.size() * sizeof(ComboDev));
   up_load(d_tasks, plan.tasks.data(), plan.tasks.size() * sizeof(JKTask));
+  up_load(d_jobs, plan.jobs.data(), plan.jobs.size() * sizeof(int));
+  d_counters.alloc(HFG_NCLASS * sizeof(int));
+  job_begin = plan.job_begin;
+  if (const char* env = std::getenv("HFG_JK_ROWS")) use_rows = std::atoi(env) != 0;
   vb_size = plan.vb_size;
   kt_size = plan.kt_size;
   task_begin = plan.task_begin;
@@ -398,10 +406,15 @@ void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* d
   a.D = dP;
   a.Jacc = d_Jacc.as<double>();
   a.Kacc = d_Kacc.as<double>();
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, d_counters.as<int>(), HFG_NCLASS);
+  a.tasks = d_tasks.as<JKTask>();
   for (int c = 0; c < HFG_NCLASS; ++c) {
     const size_t nt = task_begin[c + 1] - task_begin[c];
     if (!nt) continue;
-    a.tasks = d_tasks.as<JKTask>() + task_begin[c];
+    a.task0 = (int)task_begin[c];
+    a.jobs = d_jobs.as<int>() + job_begin[c];
+    a.njob = use_rows ? (int)(job_begin[c + 1] - job_begin[c]) : 0;
+    a.job_counter = d_counters.as<int>() + c;
     if (profile) HIP_CHECK(hipEventRecord(events[2 * c], stream));
     launch_jk_class(c, a, (int)nt, g_max_blocks, stream);
     HIP_CHECK(hipGetLastError());

@@ -280,7 +280,11 @@ extern "C" int hfgc_jk_plan_work(const hfg_basis_view* b, const int* caps, doubl
                 (double)vb.na * vk.nb * NA * ND + (double)vb.nb * vk.nb * NB * ND;
         }
       }
-      out[4 * c] = nt;
+      std::vector<int> bras;
+      for (size_t t = plan.task_begin[c]; t < plan.task_begin[c + 1]; ++t) bras.push_back(plan.tasks[t].bra);
+      std::sort(bras.begin(), bras.end());
+      const double njob = (double)(std::unique(bras.begin(), bras.end()) - bras.begin());
+      out[4 * c] = nt + 1e-6 * njob;  // tasks, and distinct bra views in the fraction
       out[4 * c + 1] = pq;
       out[4 * c + 2] = nb;
       out[4 * c + 3] = at;

@@ -57,6 +57,10 @@ __device__ __forceinline__ void add_local(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+__device__ __forceinline__ void lds_add(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -72,13 +76,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 // over a and c, J[c_k, d_l] over a and b, and J[a_i, b_j] over c, d and the
 // 64 lanes (one wave reduction).  One atomic per output element and member
 // tuple remains.
+// rows: the workgroup's LDS accumulator of the bra's K rows and bra Coulomb
+// block (see k_jk_rows), or null to update J and K in global memory only.
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restrict__ boys, int task, int lane) {
+__device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restrict__ boys, int task, int lane,
+                                        double* __restrict__ rows) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NBLK = NAB * NCD;
   constexpr int CLS = class_index<LA, LB, LC, LD>();
   constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1], RC = GC_CAPS[CLS][2], RD = GC_CAPS[CLS][3];
   constexpr int CA = RA * RB, CC = RC * RD;
+  constexpr int NR = RA * NA + RB * NB;  // K rows of the bra
   const JKTask t = a.tasks[task];
   const int bra = __builtin_amdgcn_readfirstlane(t.bra);
   const int ket0 = __builtin_amdgcn_readfirstlane(t.ket0);
@@ -164,7 +172,12 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                 for (int l = 0; l < ND; ++l) s += Dm[(size_t)(Co[z] + k) * n + Do[u] + l] * blk[k * ND + l];
             }
           s = wave_sum(s);
-          if (lane == 0) add_local(&Jx[(size_t)(Ao[x] + i) * n + Bo[y] + jj], 2.0 * s);
+          if (lane == 0) {
+            if (rows)
+              lds_add(&rows[NR * n + (x * RB + y) * NAB + i * NB + jj], 2.0 * s);
+            else
+              add_local(&Jx[(size_t)(Ao[x] + i) * n + Bo[y] + jj], 2.0 * s);
+          }
         }
     }
   if (!act) return;
@@ -214,7 +227,10 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                 for (int l = 0; l < ND; ++l)
                   s += Dm[(size_t)(Bo[y] + jj) * n + Do[u] + l] * blk[((i * NB + jj) * NC + k) * ND + l];
             }
-          add_local(&Kx[(size_t)(Ao[x] + i) * n + Co[z] + k], s);
+          if (rows)
+            lds_add(&rows[(size_t)(x * NA + i) * n + Co[z] + k], s);
+          else
+            add_local(&Kx[(size_t)(Ao[x] + i) * n + Co[z] + k], s);
         }
     }
   // K[b_j, c_k] over a, d.
@@ -239,7 +255,10 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                 for (int l = 0; l < ND; ++l)
                   s += Dm[(size_t)(Ao[x] + i) * n + Do[u] + l] * blk[((i * NB + jj) * NC + k) * ND + l];
             }
-          add_local(&Kx[(size_t)(Bo[y] + jj) * n + Co[z] + k], s);
+          if (rows)
+            lds_add(&rows[(size_t)(RA * NA + y * NB + jj) * n + Co[z] + k], s);
+          else
+            add_local(&Kx[(size_t)(Bo[y] + jj) * n + Co[z] + k], s);
         }
     }
   // K[a_i, d_l] over b, c.
@@ -264,7 +283,10 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                 for (int k = 0; k < NC; ++k)
                   s += Dm[(size_t)(Bo[y] + jj) * n + Co[z] + k] * blk[((i * NB + jj) * NC + k) * ND + l];
             }
-          add_local(&Kx[(size_t)(Ao[x] + i) * n + Do[u] + l], s);
+          if (rows)
+            lds_add(&rows[(size_t)(x * NA + i) * n + Do[u] + l], s);
+          else
+            add_local(&Kx[(size_t)(Ao[x] + i) * n + Do[u] + l], s);
         }
     }
   // K[b_j, d_l] over a, c.
@@ -289,14 +311,18 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                 for (int k = 0; k < NC; ++k)
                   s += Dm[(size_t)(Ao[x] + i) * n + Co[z] + k] * blk[((i * NB + jj) * NC + k) * ND + l];
             }
-          add_local(&Kx[(size_t)(Bo[y] + jj) * n + Do[u] + l], s);
+          if (rows)
+            lds_add(&rows[(size_t)(RA * NA + y * NB + jj) * n + Do[u] + l], s);
+          else
+            add_local(&Kx[(size_t)(Bo[y] + jj) * n + Do[u] + l], s);
         }
     }
 }
 
 // Persistent waves: every wave walks the class's task list with a stride of
-// the total wave count (tasks are ordered heaviest first), so the LDS copy of
-// the Boys slice is made once per workgroup.
+// the total wave count (tasks are ordered by job, heaviest first), so the LDS
+// copy of the Boys slice is made once per workgroup.  J and K are updated in
+// global memory (per-XCC copies).
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
   __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
@@ -304,7 +330,67 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
   const int lane = threadIdx.x & 63;
   const int waves = (int)(gridDim.x * (blockDim.x >> 6));
   for (int task = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); task < a.ntask; task += waves)
-    jk_task<LA, LB, LC, LD>(a, boys, task, lane);
+    jk_task<LA, LB, LC, LD>(a, boys, task, lane, nullptr);
+}
+
+// Persistent workgroups over jobs (tasks of one bra view), taken from a
+// counter in order (heaviest first).  The four waves share the job's tasks
+// and accumulate every K update -- all of them have a bra index as their row
+// -- and the bra Coulomb block in LDS, NR x n rows plus CA x NAB; after the
+// job the workgroup adds the nonzero entries to the global copies and clears
+// them.  Only the ket Coulomb updates go to global memory directly.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void k_jk_rows(JKArgs a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB;
+  constexpr int CLS = class_index<LA, LB, LC, LD>();
+  constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1], CA = RA * RB;
+  constexpr int NR = RA * NA + RB * NB;
+  __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
+  __shared__ int job_sh;
+  extern __shared__ double rows[];
+  const int n = a.n;
+  const int nrow = NR * n + CA * NAB;
+  for (int e = threadIdx.x; e < nrow; e += blockDim.x) rows[e] = 0.0;
+  load_boys_slice<LA + LB + LC + LD>(a.boys, boys);  // ends with a barrier
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const size_t slot = (size_t)xcc_id() * a.nn;
+  double* __restrict__ Jx = a.Jacc + slot;
+  double* __restrict__ Kx = a.Kacc + slot;
+  for (;;) {
+    if (threadIdx.x == 0) job_sh = atomicAdd(a.job_counter, 1);
+    __syncthreads();
+    const int job = job_sh;
+    __syncthreads();
+    if (job >= a.njob) break;
+    const int t0 = a.jobs[job], t1 = a.jobs[job + 1];
+    for (int task = t0 + wave; task < t1; task += nwave) jk_task<LA, LB, LC, LD>(a, boys, task, lane, rows);
+    __syncthreads();
+    const ViewDev& vb = a.views[a.tasks[t0].bra];
+    const ComboDev* __restrict__ cbra = a.combos + vb.c0;
+    for (int e = threadIdx.x; e < nrow; e += blockDim.x) {
+      const double v = rows[e];
+      if (v == 0.0) continue;
+      rows[e] = 0.0;
+      if (e < NR * n) {
+        const int r = e / n, col = e - r * n;
+        const int ao = r < RA * NA ? cbra[(r / NA) * RB].ao_a + r % NA : cbra[(r - RA * NA) / NB].ao_b + (r - RA * NA) % NB;
+        add_local(&Kx[(size_t)ao * n + col], v);
+      } else {
+        const int f = e - NR * n, xy = f / NAB, ij = f % NAB;
+        const int ao_a = cbra[(xy / RB) * RB].ao_a + ij / NB, ao_b = cbra[xy % RB].ao_b + ij % NB;
+        add_local(&Jx[(size_t)ao_a * n + ao_b], v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// LDS bytes of k_jk_rows for n basis functions, beyond the static Boys slice.
+template <int LA, int LB, int LC, int LD>
+constexpr size_t jk_rows_bytes(int n) {
+  constexpr int CLS = class_index<LA, LB, LC, LD>();
+  constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1];
+  return sizeof(double) * ((size_t)(RA * ncart(LA) + RB * ncart(LB)) * n + RA * RB * ncart(LA) * ncart(LB));
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -312,6 +398,23 @@ void launch_jk(const JKArgs& a, int ntask, int max_blocks, hipStream_t s) {
   JKArgs b = a;
   b.ntask = ntask;
   const int waves_per_block = 4;
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  static int lds_limit = -1;  // bytes of LDS one workgroup may use
+  if (lds_limit < 0) HIP_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t lds = jk_rows_bytes<LA, LB, LC, LD>(a.n);
+  if (a.njob > 0 && lds + sizeof(double) * BOYS_ROWS * BOYS_SLICE + 64 <= (size_t)lds_limit) {
+    int per_cu = 0, cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_CHECK(hipFuncSetAttribute((const void*)k_jk_rows<LA, LB, LC, LD>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jk_rows<LA, LB, LC, LD>,
+                                                           64 * waves_per_block, lds));
+    const int blocks = std::max(1, std::min(a.njob, std::max(1, per_cu) * cus));
+    hipLaunchKernelGGL((k_jk_rows<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), lds, s, b);
+    return;
+  }
+  b.tasks = a.tasks + a.task0;
   const int blocks = std::min(max_blocks, (ntask + waves_per_block - 1) / waves_per_block);
   hipLaunchKernelGGL((k_jk<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), 0, s, b);
 }

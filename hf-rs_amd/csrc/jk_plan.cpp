@@ -371,22 +371,31 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
 
   // ---- tasks per class: bra views of the class's bra set, ket groups of its
   // ket set; kets with qmax_bra * qmax_ket > screen form a prefix.  When bra
-  // and ket sets coincide each unordered pair of views is listed once.
+  // and ket sets coincide each unordered pair of views is listed once.  The
+  // tasks of one job share their bra view and are contiguous; jobs are ordered
+  // by estimated cost, heaviest first.
   std::vector<JKTask> all;
   plan.task_begin.assign(HFG_NCLASS + 1, 0);
+  plan.job_begin.assign(HFG_NCLASS + 1, 0);
+  plan.jobs.clear();
   for (int c = 0; c < HFG_NCLASS; ++c) {
     plan.task_begin[c] = all.size();
+    plan.job_begin[c] = plan.jobs.size();
     const int sb = class_sets[c][0], sk = class_sets[c][1];
     const bool same_set = sb == sk;
-    for (size_t ib = 0; ib < set_groups[sb].size(); ++ib)
-      for (size_t ik = 0; ik < set_groups[sk].size(); ++ik) {
-        if (same_set && ik > ib) continue;
-        const int gb = set_groups[sb][ib], gk = set_groups[sk][ik];
-        const Group& B = groups[gb];
-        const Group& K = groups[gk];
-        const bool tri = gb == gk;
-        for (size_t i = 0; i < B.v.size(); ++i) {
-          const double qb = sets[sb].views[B.v[i]].qmax;
+    const double digest = 40.0 * caps[c][0] * caps[c][1] * caps[c][2] * caps[c][3];
+    std::vector<std::pair<double, std::vector<JKTask>>> per_bra;
+    for (size_t ib = 0; ib < set_groups[sb].size(); ++ib) {
+      const int gb = set_groups[sb][ib];
+      const Group& B = groups[gb];
+      for (size_t i = 0; i < B.v.size(); ++i) {
+        const double qb = sets[sb].views[B.v[i]].qmax;
+        std::vector<JKTask> tl;
+        double cost = 0.0;
+        for (size_t ik = 0; ik < set_groups[sk].size(); ++ik) {
+          if (same_set && ik > ib) continue;
+          const int gk = set_groups[sk][ik];
+          const Group& K = groups[gk];
           size_t lo = 0, hi = K.v.size();
           while (lo < hi) {
             const size_t mid = (lo + hi) / 2;
@@ -396,13 +405,42 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
               hi = mid;
           }
           size_t nk = lo;
-          if (tri) nk = std::min(nk, i + 1);
-          for (size_t k0 = 0; k0 < nk; k0 += 64)
-            all.push_back(JKTask{kg[gb].start + (int)i, kg[gk].start + (int)k0, (int)std::min<size_t>(64, nk - k0),
-                                 gk});
+          if (gb == gk) nk = std::min(nk, i + 1);
+          for (size_t k0 = 0; k0 < nk; k0 += 64) {
+            const int m = (int)std::min<size_t>(64, nk - k0);
+            tl.push_back(JKTask{kg[gb].start + (int)i, kg[gk].start + (int)k0, m, gk});
+            cost += (double)m * ((double)B.npp * K.npp + digest);
+          }
         }
+        if (!tl.empty()) per_bra.emplace_back(cost, std::move(tl));
       }
+    }
+    // A bra with many tasks is cut into jobs of at most jt tasks, jt chosen so
+    // that the class has about 1024 jobs (four per CU and more) when it has the
+    // tasks for it: enough workgroups for the whole chip, and as many tasks per
+    // job as that allows (each job ends with one flush of its LDS rows).
+    size_t ntask_c = 0;
+    for (auto& pb : per_bra) ntask_c += pb.second.size();
+    const size_t jt = std::max<size_t>(4, ntask_c / 1024);
+    std::vector<std::pair<double, std::vector<JKTask>>> jobs_c;
+    for (auto& pb : per_bra) {
+      const size_t nt = pb.second.size(), nj = (nt + jt - 1) / jt;
+      for (size_t j = 0; j < nj; ++j) {
+        const size_t a0 = j * nt / nj, a1 = (j + 1) * nt / nj;
+        jobs_c.emplace_back(pb.first * (double)(a1 - a0) / nt,
+                            std::vector<JKTask>(pb.second.begin() + a0, pb.second.begin() + a1));
+      }
+    }
+    std::stable_sort(jobs_c.begin(), jobs_c.end(),
+                     [](const std::pair<double, std::vector<JKTask>>& x,
+                        const std::pair<double, std::vector<JKTask>>& y) { return x.first > y.first; });
+    for (auto& jb : jobs_c) {
+      plan.jobs.push_back((int)all.size());
+      all.insert(all.end(), jb.second.begin(), jb.second.end());
+    }
   }
+  plan.job_begin[HFG_NCLASS] = plan.jobs.size();
+  plan.jobs.push_back((int)all.size());
   plan.task_begin[HFG_NCLASS] = all.size();
 
   plan.vb_size = vb.size();

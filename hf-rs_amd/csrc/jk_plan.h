@@ -73,6 +73,11 @@ struct JKPlan {
   std::vector<double> kcoef;  // ket coefficients
   std::vector<JKTask> tasks;
   std::vector<size_t> task_begin;  // HFG_NCLASS + 1 entries
+  // Jobs: tasks of one bra view, [jobs[j], jobs[j + 1]) (absolute task
+  // indices, one sentinel at the end); class c owns jobs job_begin[c] ..
+  // job_begin[c + 1] - 1.
+  std::vector<int> jobs;
+  std::vector<size_t> job_begin;  // HFG_NCLASS + 1 entries
   size_t vb_size = 0, kt_size = 0, prim_pairs_kept = 0;
   int n_gen_shells = 0;
 };
