@@ -17,6 +17,15 @@
 
 namespace hfg {
 
+// Read-only data whose address is uniform across the wave: loads through the
+// constant address space become scalar loads (the compiler cannot otherwise
+// prove that the kernel's atomics do not alias them).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const double __attribute__((address_space(4)))* cdptr;
+#else
+typedef const double* cdptr;
+#endif
+
 constexpr double TWO_PI_2_5 = 34.986836655249726164477;  // 2 pi^(5/2)
 constexpr double HALF_SQRT_PI = 0.88622692545275801365;
 

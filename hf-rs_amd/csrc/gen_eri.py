@@ -297,9 +297,9 @@ def gen_class_gc(la, lb, lc, ld):
     one combination (cap > 1); the coefficients come from bc[(ib - b0) * CA + g] (bra,
     wave-uniform) and kc[(jk * CC + g) * ks] (ket, one pair per lane).  With
     cap 1 the single combination's coefficient is folded into K as in
-    eri_block.  Contraction runs ket-first: per bra primitive the ket sums
-    t[gcd] = sum_k c_k[gcd] [e0|f0]_k are formed, then a[gab][gcd] +=
-    c_b[gab] t[gcd].  Output: I[(gab * CC + gcd) * NAB * NCD + ...]."""
+    eri_block.  Contraction runs bra-first: per ket primitive the bra sums
+    t[gab] = sum_b c_b[gab] [e0|f0]_b are formed, then a[gab][gcd] +=
+    c_k[gcd] t[gab].  Output: I[(gab * CC + gcd) * NAB * NCD + ...]."""
     RA, RB, RC, RD = gc_caps((la, lb, lc, ld))
     CA, CC = RA * RB, RC * RD
     M = la + lb + lc + ld
@@ -320,42 +320,78 @@ def gen_class_gc(la, lb, lc, ld):
                 for f in Fk:
                     acc[(ga, gc, e, f)] = "a%d_%d_%s_%s" % (ga, gc, tag(e), tag(f))
     out.append("  double " + ", ".join("%s = 0.0" % v for v in acc.values()) + ";")
-    out.append("  for (int ib = b0; ib < b1; ++ib) {")
-    out.append("    const double p = bp.p[ib], Px = bp.Px[ib], Py = bp.Py[ib], Pz = bp.Pz[ib], Kp = bp.K[ib];")
+    # Ket primitives outer (one ket pair per lane; every lane runs the same
+    # count, idle lanes on a valid pair, masked at digestion), bra primitives
+    # inner (wave-uniform: scalar loads), so each ket primitive and its
+    # coefficients are loaded once per quartet.
+    # Bra data of the next primitive are loaded before the current one is
+    # evaluated (software prefetch: one wave per SIMD hides no latency).
+    bra_fields = [("p", "p"), ("Kp", "K"), ("Bx", "Px"), ("By", "Py"), ("Bz", "Pz")]
     if M > 0 and la + lb > 0:
-        out.append("    const double PAx = bp.PAx[ib], PAy = bp.PAy[ib], PAz = bp.PAz[ib], oo2p = bp.H[ib];")
-    if CA > 1:
+        bra_fields += [("PAx", "PAx"), ("PAy", "PAy"), ("PAz", "PAz"), ("oo2p", "H")]
+    def bra_loads(idx, pre):
+        o = ["%s%s = s%s[%s]" % (pre, v, f, idx) for v, f in bra_fields]
+        if CA > 1:
+            o += ["%scb%d = sbc[(%s - b0) * %d + %d]" % (pre, g, idx, CA, g) for g in range(CA)]
+        return o
+    out.append("  const cdptr " + ", ".join("s%s = (cdptr)bp.%s" % (f, f) for v, f in bra_fields) +
+               (", sbc = (cdptr)bc" if CA > 1 else "") + ";")
+    out.append("  for (int jk = 0; jk < nk; ++jk) {")
+    out.append("    const int ik = k0 + jk * ks;")
+    out.append("    const double q = kp.p[ik], Qx = kp.Px[ik], Qy = kp.Py[ik], Qz = kp.Pz[ik], Kq = kp.K[ik];")
+    if M > 0 and lc + ld > 0:
+        out.append("    const double QCx = kp.PAx[ik], QCy = kp.PAy[ik], QCz = kp.PAz[ik], oo2q = kp.H[ik];")
+    if CC > 1:
+        out.append("    const double " + ", ".join("ck%d = kc[(jk * %d + %d) * ks]" % (g, CC, g) for g in range(CC)) + ";")
         tmp = {}
-        for gc in range(CC):
+        for ga in range(CA):
             for e in E:
                 for f in Fk:
-                    tmp[(gc, e, f)] = "t%d_%s_%s" % (gc, tag(e), tag(f))
+                    tmp[(ga, e, f)] = "t%d_%s_%s" % (ga, tag(e), tag(f))
         out.append("    double " + ", ".join("%s = 0.0" % v for v in tmp.values()) + ";")
-        tgt = lambda gc, e, f: tmp[(gc, e, f)]
+        tgt = lambda ga, e, f: tmp[(ga, e, f)]
     else:
-        tgt = lambda gc, e, f: acc[(0, gc, e, f)]
-    out.append("    for (int jk = 0; jk < nk; ++jk) {")
-    out += ket_prim_lines(la, lb, lc, ld)
-    if CC > 1:
-        out.append("      const double " + ", ".join("ck%d = kc[(jk * %d + %d) * ks]" % (g, CC, g) for g in range(CC)) + ";")
+        tgt = lambda ga, e, f: acc[(ga, 0, e, f)]
+    out.append("    double " + ", ".join(bra_loads("b0", "n_")) + ";")
+    out.append("    for (int ib = b0; ib < b1; ++ib) {")
+    names = [v for v, f in bra_fields] + (["cb%d" % g for g in range(CA)] if CA > 1 else [])
+    out.append("      const double " + ", ".join("%s = n_%s" % (v, v) for v in names) + ";")
+    out.append("      const int in = ib + 1 < b1 ? ib + 1 : ib;")
+    out.append("      " + "; ".join(bra_loads("in", "n_")) + ";")
+    out.append("      const double PQx = Bx - Qx, PQy = By - Qy, PQz = Bz - Qz;")
+    out.append("      const double rs = rsqrt_nr(p + q);  // 1/sqrt(p+q)")
+    out.append("      const double inv_s = rs * rs;")
+    out.append("      const double rho = p * q * inv_s;")
+    out.append("      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    out.append("      const double pref = TWO_PI_2_5 * Kp * Kq * rs;")
+    out.append("      double Fm[%d];" % (M + 1))
+    out.append("      boys_eval<%d>(T, boys, Fm);" % M)
+    for m in range(M + 1):
+        out.append("      const double F%d = pref * Fm[%d];" % (m, m))
+    if M > 0:
+        if la + lb > 0:
+            out.append("      const double rp = q * inv_s;")
+            out.append("      const double WPx = -rp * PQx, WPy = -rp * PQy, WPz = -rp * PQz;")
+        if lc + ld > 0:
+            out.append("      const double rq = p * inv_s, oo2s = 0.5 * inv_s;")
+            out.append("      const double WQx = rq * PQx, WQy = rq * PQy, WQz = rq * PQz;")
     vrr = VRR(em)
     for e in E:
         for f in Fk:
             n = vrr.node(e, f, 0)
-            for gc in range(CC):
-                if CC > 1:
-                    em.emit("      %s += ck%d * %s;" % (tgt(gc, e, f), gc, n))
+            for ga in range(CA):
+                if CA > 1:
+                    em.emit("      %s += cb%d * %s;" % (tgt(ga, e, f), ga, n))
                 else:
-                    em.emit("      %s += %s;" % (tgt(gc, e, f), n))
+                    em.emit("      %s += %s;" % (tgt(ga, e, f), n))
     out += em.lines
     out.append("    }")
-    if CA > 1:
-        out.append("    const double " + ", ".join("cb%d = bc[(ib - b0) * %d + %d]" % (g, CA, g) for g in range(CA)) + ";")
+    if CC > 1:
         for ga in range(CA):
             for gc in range(CC):
                 for e in E:
                     for f in Fk:
-                        out.append("    %s += cb%d * %s;" % (acc[(ga, gc, e, f)], ga, tmp[(gc, e, f)]))
+                        out.append("    %s += ck%d * %s;" % (acc[(ga, gc, e, f)], gc, tmp[(ga, e, f)]))
     out.append("  }")
     for ga in range(CA):
         for gc in range(CC):

@@ -100,7 +100,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
   const int j0 = ket - g.start;
   double I[CA * CC * NBLK];
   eri_gc<LA, LB, LC, LD>(a.bp, a.bcoef + vb.coef0, vb.p0, vb.p1, a.kt, a.kcoef + g.cbase + j0, g.base + j0,
-                         act ? g.npp : 0, g.size, vb.AB, vk.AB, boys, I);
+                         g.npp, g.size, vb.AB, vk.AB, boys, I);  // idle lanes run a valid pair, masked below
 
   double w = 1.0;
   if (vb.same) w *= 0.5;
