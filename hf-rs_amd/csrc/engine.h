@@ -23,8 +23,7 @@ struct JKArgs {
   const ViewDev* views;
   const ComboDev* combos;
   const KetGroup* groups;
-  PrimPairs bp;          // bra side, contiguous per view
-  const double* bcoef;   // bra coefficients [prim][cap] per view
+  const double* brec;    // bra primitive records, contiguous per view
   PrimPairs kt;          // ket side, transposed per group
   const double* kcoef;   // ket coefficients, transposed per group
   size_t nn;             // n*n, the stride between per-XCC accumulator copies
@@ -97,7 +96,7 @@ struct Engine {
   int64_t n_quartets = 0;
   double screen = 1e-12;        // Q_ab*Q_cd > screen (src/shell_integrals.rs:184,226)
 
-  DevBuf d_pp, d_vb, d_bcoef, d_kt, d_kcoef, d_groups, d_pairs, d_views, d_combos, d_boys, d_scale, d_tasks, d_jobs,
+  DevBuf d_pp, d_vb, d_kt, d_kcoef, d_groups, d_pairs, d_views, d_combos, d_boys, d_scale, d_tasks, d_jobs,
       d_counters, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
   std::vector<size_t> task_begin;  // per class, into d_tasks (HFG_NCLASS+1 entries)
   std::vector<size_t> job_begin;   // per class, into d_jobs (HFG_NCLASS+1 entries)
@@ -114,7 +113,6 @@ struct Engine {
   ~Engine();
   PrimPairs pp_view() const;
   PrimPairs kt_view() const;
-  PrimPairs vb_view() const;
   size_t vb_size = 0;
   size_t kt_size = 0;
 

@@ -277,12 +277,6 @@ void Engine::compute_schwarz() {
   }
 }
 
-PrimPairs Engine::vb_view() const {
-  const double* b = d_vb.as<double>();
-  const size_t m = vb_size;
-  return PrimPairs{b, b + m, b + 2 * m, b + 3 * m, b + 4 * m, b + 5 * m, b + 6 * m, b + 7 * m, b + 8 * m};
-}
-
 PrimPairs Engine::kt_view() const {
   const double* b = d_kt.as<double>();
   const size_t m = kt_size;
@@ -316,7 +310,6 @@ void Engine::build_tasks() {
     if (bytes) HIP_CHECK(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream));
   };
   up_load(d_vb, plan.vb.data(), plan.vb.size() * sizeof(double));
-  up_load(d_bcoef, plan.bcoef.data(), plan.bcoef.size() * sizeof(double));
   up_load(d_kt, plan.kt.data(), plan.kt.size() * sizeof(double));
   up_load(d_kcoef, plan.kcoef.data(), plan.kcoef.size() * sizeof(double));
   up_load(d_groups, plan.groups.data(), plan.groups.size() * sizeof(KetGroup));
@@ -396,8 +389,7 @@ void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* d
   a.views = d_views.as<ViewDev>();
   a.combos = d_combos.as<ComboDev>();
   a.groups = d_groups.as<KetGroup>();
-  a.bp = vb_view();
-  a.bcoef = d_bcoef.as<double>();
+  a.brec = d_vb.as<double>();
   a.kt = kt_view();
   a.kcoef = d_kcoef.as<double>();
   a.screen = screen;

@@ -289,19 +289,32 @@ def gc_caps(c):
     return GC_CAPS.get(tuple(c), (1, 1, 1, 1))
 
 
+# Bra primitive record of the J/K kernel: 16 doubles, read with wide scalar
+# loads from one pointer: p, K, Px, Py, Pz, then for pair classes beyond ss
+# PAx, PAy, PAz, 1/(2p); the contraction coefficients of the view's
+# combinations follow (from BRA_COEF_SS or BRA_COEF).
+BRA_REC = 16
+BRA_COEF_SS = 5
+BRA_COEF = 9
+
+
 def gen_class_gc(la, lb, lc, ld):
     """eri_gc<la,lb,lc,ld>: blocks of CA x CC contraction combinations of one
     bra pair and one ket pair that share their primitive pairs.
 
-    Primitive data carry no contraction coefficients when a side has more than
-    one combination (cap > 1); the coefficients come from bc[(ib - b0) * CA + g] (bra,
-    wave-uniform) and kc[(jk * CC + g) * ks] (ket, one pair per lane).  With
-    cap 1 the single combination's coefficient is folded into K as in
-    eri_block.  Contraction runs bra-first: per ket primitive the bra sums
-    t[gab] = sum_b c_b[gab] [e0|f0]_b are formed, then a[gab][gcd] +=
-    c_k[gcd] t[gab].  Output: I[(gab * CC + gcd) * NAB * NCD + ...]."""
+    Bra primitives are records of BRA_REC doubles at br (wave-uniform, read
+    through scalar loads), ket primitives the transposed arrays kp (one pair
+    per lane).  Primitive data carry no contraction coefficients when a side
+    has more than one combination (cap > 1): the bra coefficients are in the
+    record, the ket ones at kc[(jk * CC + g) * ks].  With cap 1 the single
+    combination's coefficient is folded into K as in eri_block.  Ket
+    primitives are the outer loop, bra primitives the inner one: per ket
+    primitive the bra sums t[gab] = sum_b c_b[gab] [e0|f0]_b are formed, then
+    a[gab][gcd] += c_k[gcd] t[gab].  Output: I[(gab * CC + gcd) * NAB * NCD + ...]."""
     RA, RB, RC, RD = gc_caps((la, lb, lc, ld))
     CA, CC = RA * RB, RC * RD
+    coff = BRA_COEF_SS if la + lb == 0 else BRA_COEF
+    assert CA == 1 or coff + CA <= BRA_REC
     M = la + lb + lc + ld
     em = Emitter()
     E = levels(la, la + lb)
@@ -310,7 +323,7 @@ def gen_class_gc(la, lb, lc, ld):
     out = []
     out.append("// (%s%s|%s%s) x %d x %d combinations" % ("spdfg"[la], "spdfg"[lb], "spdfg"[lc], "spdfg"[ld], CA, CC))
     out.append("template <> HFG_DEV void eri_gc<%d,%d,%d,%d>(" % (la, lb, lc, ld))
-    out.append("    const PrimPairs& bp, const double* __restrict__ bc, int b0, int b1, const PrimPairs& kp,")
+    out.append("    const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,")
     out.append("    const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],")
     out.append("    const double* __restrict__ boys, double* __restrict__ I) {")
     acc = {}
@@ -323,19 +336,17 @@ def gen_class_gc(la, lb, lc, ld):
     # Ket primitives outer (one ket pair per lane; every lane runs the same
     # count, idle lanes on a valid pair, masked at digestion), bra primitives
     # inner (wave-uniform: scalar loads), so each ket primitive and its
-    # coefficients are loaded once per quartet.
-    # Bra data of the next primitive are loaded before the current one is
-    # evaluated (software prefetch: one wave per SIMD hides no latency).
-    bra_fields = [("p", "p"), ("Kp", "K"), ("Bx", "Px"), ("By", "Py"), ("Bz", "Pz")]
+    # coefficients are loaded once per quartet.  The next bra record is loaded
+    # after the Boys table reads of the current one (software prefetch: one
+    # wave per SIMD hides no latency, and scalar and LDS loads share a counter).
+    bra_fields = [("p", 0), ("Kp", 1), ("Bx", 2), ("By", 3), ("Bz", 4)]
     if M > 0 and la + lb > 0:
-        bra_fields += [("PAx", "PAx"), ("PAy", "PAy"), ("PAz", "PAz"), ("oo2p", "H")]
-    def bra_loads(idx, pre):
-        o = ["%s%s = s%s[%s]" % (pre, v, f, idx) for v, f in bra_fields]
-        if CA > 1:
-            o += ["%scb%d = sbc[(%s - b0) * %d + %d]" % (pre, g, idx, CA, g) for g in range(CA)]
-        return o
-    out.append("  const cdptr " + ", ".join("s%s = (cdptr)bp.%s" % (f, f) for v, f in bra_fields) +
-               (", sbc = (cdptr)bc" if CA > 1 else "") + ";")
+        bra_fields += [("PAx", 5), ("PAy", 6), ("PAz", 7), ("oo2p", 8)]
+    if CA > 1:
+        bra_fields += [("cb%d" % g, coff + g) for g in range(CA)]
+    def bra_loads(rec, pre):
+        return ["%s%s = %s[%d]" % (pre, v, rec, k) for v, k in bra_fields]
+    out.append("  const cdptr rec0 = (cdptr)br;")
     out.append("  for (int jk = 0; jk < nk; ++jk) {")
     out.append("    const int ik = k0 + jk * ks;")
     out.append("    const double q = kp.p[ik], Qx = kp.Px[ik], Qy = kp.Py[ik], Qz = kp.Pz[ik], Kq = kp.K[ik];")
@@ -352,12 +363,12 @@ def gen_class_gc(la, lb, lc, ld):
         tgt = lambda ga, e, f: tmp[(ga, e, f)]
     else:
         tgt = lambda ga, e, f: acc[(ga, 0, e, f)]
-    out.append("    double " + ", ".join(bra_loads("b0", "n_")) + ";")
+    out.append("    const cdptr rb = rec0 + (size_t)b0 * %d;" % BRA_REC)
+    out.append("    double " + ", ".join(bra_loads("rb", "n_")) + ";")
     out.append("    for (int ib = b0; ib < b1; ++ib) {")
-    names = [v for v, f in bra_fields] + (["cb%d" % g for g in range(CA)] if CA > 1 else [])
+    names = [v for v, k in bra_fields]
     out.append("      const double " + ", ".join("%s = n_%s" % (v, v) for v in names) + ";")
-    out.append("      const int in = ib + 1 < b1 ? ib + 1 : ib;")
-    out.append("      " + "; ".join(bra_loads("in", "n_")) + ";")
+    out.append("      const cdptr rn = rec0 + (size_t)(ib + 1 < b1 ? ib + 1 : ib) * %d;" % BRA_REC)
     out.append("      const double PQx = Bx - Qx, PQy = By - Qy, PQz = Bz - Qz;")
     out.append("      const double rs = rsqrt_nr(p + q);  // 1/sqrt(p+q)")
     out.append("      const double inv_s = rs * rs;")
@@ -368,6 +379,7 @@ def gen_class_gc(la, lb, lc, ld):
     out.append("      boys_eval<%d>(T, boys, Fm);" % M)
     for m in range(M + 1):
         out.append("      const double F%d = pref * Fm[%d];" % (m, m))
+    out.append("      " + "; ".join(bra_loads("rn", "n_")) + ";")
     if M > 0:
         if la + lb > 0:
             out.append("      const double rp = q * inv_s;")
@@ -399,7 +411,6 @@ def gen_class_gc(la, lb, lc, ld):
             out += hrr_lines(la, lb, lc, ld, sub_acc, "g%d_%d_" % (ga, gc), (ga * CC + gc) * nab * ncd)
     out.append("}")
     return "\n".join(out) + "\n"
-
 
 
 # Pair classes: every (la, lb) with lb <= la <= 2 (orbital shell pairs up to
@@ -440,6 +451,8 @@ def main(outdir):
         f.write("#define HFG_NCLASS_ALL %d\n" % len(classes()))
         f.write("// Contraction-member caps (RA, RB, RC, RD) of the J/K classes, quartet_class() order.\n")
         f.write("#define HFG_GC_CAPS " + ", ".join("{%d, %d, %d, %d}" % gc_caps(c) for c in jk) + "\n")
+        f.write("// Bra primitive record of the J/K kernels: doubles per record, first coefficient (ss / other pairs).\n")
+        f.write("#define HFG_BRA_REC %d\n#define HFG_BRA_COEF_SS %d\n#define HFG_BRA_COEF %d\n" % (BRA_REC, BRA_COEF_SS, BRA_COEF))
 
 
 if __name__ == "__main__":

@@ -17,7 +17,7 @@
 
 namespace hfg {
 template <int LA, int LB, int LC, int LD>
-HFG_DEV void eri_gc(const PrimPairs& bp, const double* __restrict__ bc, int b0, int b1, const PrimPairs& kp,
+HFG_DEV void eri_gc(const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,
                     const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],
                     const double* __restrict__ boys, double* __restrict__ I);
 #include "gen/eri_gc.inc"
@@ -82,7 +82,7 @@ void replay_class(Replay& r) {
   constexpr int NCD = NC * ND, NBLK = NA * NB * NCD;
   constexpr int CA = GC_CAPS[CLS][0] * GC_CAPS[CLS][1], CC = GC_CAPS[CLS][2] * GC_CAPS[CLS][3];
   const JKPlan& P = r.P;
-  const PrimPairs bp = view_cols(P.vb, P.vb_size), kp = view_cols(P.kt, P.kt_size);
+  const PrimPairs kp = view_cols(P.kt, P.kt_size);
   const int n = r.n;
   std::vector<double> I(CA * CC * NBLK);
   for (size_t t = P.task_begin[CLS]; t < P.task_begin[CLS + 1]; ++t) {
@@ -93,7 +93,7 @@ void replay_class(Replay& r) {
       const int ket = task.ket0 + lane;
       const ViewDev& vk = P.views[ket];
       const int j = ket - g.start;
-      eri_gc<LA, LB, LC, LD>(bp, P.bcoef.data() + vb.coef0, vb.p0, vb.p1, kp, P.kcoef.data() + g.cbase + j,
+      eri_gc<LA, LB, LC, LD>(P.vb.data(), vb.p0, vb.p1, kp, P.kcoef.data() + g.cbase + j,
                              g.base + j, g.npp, g.size, vb.AB, vk.AB, slice<LA + LB + LC + LD>(), I.data());
       double w = 1.0;
       if (vb.same) w *= 0.5;

@@ -28,7 +28,7 @@ namespace hfg {
 
 // Blocks of CA x CC contraction combinations (generated, see gen_eri.py).
 template <int LA, int LB, int LC, int LD>
-HFG_DEV void eri_gc(const PrimPairs& bp, const double* __restrict__ bc, int b0, int b1, const PrimPairs& kp,
+HFG_DEV void eri_gc(const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,
                     const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],
                     const double* __restrict__ boys, double* __restrict__ I);
 #include "gen/eri_gc.inc"
@@ -99,7 +99,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
   const KetGroup g = a.groups[grp];
   const int j0 = ket - g.start;
   double I[CA * CC * NBLK];
-  eri_gc<LA, LB, LC, LD>(a.bp, a.bcoef + vb.coef0, vb.p0, vb.p1, a.kt, a.kcoef + g.cbase + j0, g.base + j0,
+  eri_gc<LA, LB, LC, LD>(a.brec, vb.p0, vb.p1, a.kt, a.kcoef + g.cbase + j0, g.base + j0,
                          g.npp, g.size, vb.AB, vk.AB, boys, I);  // idle lanes run a valid pair, masked below
 
   double w = 1.0;

@@ -307,8 +307,8 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
   // ---- device tables
   std::vector<ViewDev> vd;
   std::vector<ComboDev> cd;
-  PrimPairHost vb;
-  std::vector<double> bcoef;
+  std::vector<double> brec;  // bra records, HFG_BRA_REC doubles per primitive pair
+  size_t nbrec = 0;
   std::vector<KetGroup> kg(groups.size());
   std::vector<double> kt_cols[9];
   std::vector<double> kcoef;
@@ -318,7 +318,6 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
     ctot += (size_t)g.npp * sets[g.set].ra * sets[g.set].rb * g.v.size();
   }
   const std::vector<double>* ucol[9] = {&up.p, &up.Px, &up.Py, &up.Pz, &up.PAx, &up.PAy, &up.PAz, &up.K, &up.H};
-  std::vector<double>* vcol[9] = {&vb.p, &vb.Px, &vb.Py, &vb.Pz, &vb.PAx, &vb.PAy, &vb.PAz, &vb.K, &vb.H};
   for (auto& c : kt_cols) c.assign(std::max<size_t>(1, ktot), 0.0);
   kcoef.assign(std::max<size_t>(1, ctot), 0.0);
   size_t kb = 0, cb = 0;
@@ -333,12 +332,12 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
       const GenPair& P = gp[v.pair];
       ViewDev d;
       std::memset(&d, 0, sizeof d);
-      d.p0 = (int)vb.size();
+      d.p0 = (int)nbrec;
       d.p1 = d.p0 + g.npp;
       d.na = v.na;
       d.nb = v.nb;
       d.c0 = (int)cd.size();
-      d.coef0 = (int)bcoef.size();
+      d.coef0 = 0;
       d.same = P.same ? 1 : 0;
       for (int k = 0; k < 3; ++k) d.AB[k] = P.AB[k];
       vd.push_back(d);
@@ -348,20 +347,37 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
         const int sa = gs[P.A].mem[mm[0]], sb = gs[P.B].mem[mm[1]];
         cd.push_back(ComboDev{shells[sa].ao, shells[sb].ao, ci >= 0 ? Q[seg_pair(sa, sb)] : 0.0});
       }
+      const int coff = S.pc == 0 ? HFG_BRA_COEF_SS : HFG_BRA_COEF;
+      if (cap > 1 && coff + cap > HFG_BRA_REC) throw HfgError{HFG_INVALID_INPUT, "contraction cap beyond the bra record"};
       for (int ip = 0; ip < g.npp; ++ip) {
         const int q = v.prims[ip];
+        double x[9];
         for (int c = 0; c < 9; ++c) {
-          double x = (*ucol[c])[q];
-          if (c == 7 && cap == 1) x *= P.coef[v.combos[0]][q - P.q0];
-          vcol[c]->push_back(x);
-          kt_cols[c][kb + (size_t)ip * size + j] = x;
+          x[c] = (*ucol[c])[q];
+          if (c == 7 && cap == 1) x[c] *= P.coef[v.combos[0]][q - P.q0];
+          kt_cols[c][kb + (size_t)ip * size + j] = x[c];
+        }
+        // record: p, K, Px, Py, Pz, [PAx, PAy, PAz, 1/(2p) beyond ss], coefficients
+        brec.resize(brec.size() + HFG_BRA_REC, 0.0);
+        double* r = &brec[brec.size() - HFG_BRA_REC];
+        r[0] = x[0];
+        r[1] = x[7];
+        r[2] = x[1];
+        r[3] = x[2];
+        r[4] = x[3];
+        if (S.pc != 0) {
+          r[5] = x[4];
+          r[6] = x[5];
+          r[7] = x[6];
+          r[8] = x[8];
         }
         if (cap > 1)
           for (int gg = 0; gg < cap; ++gg) {
             const double cf = v.combos[gg] >= 0 ? P.coef[v.combos[gg]][q - P.q0] : 0.0;
-            bcoef.push_back(cf);
+            r[coff + gg] = cf;
             kcoef[cb + ((size_t)ip * cap + gg) * size + j] = cf;
           }
+        ++nbrec;
       }
       plan.prim_pairs_kept += g.npp;
     }
@@ -443,16 +459,14 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
   plan.jobs.push_back((int)all.size());
   plan.task_begin[HFG_NCLASS] = all.size();
 
-  plan.vb_size = vb.size();
+  plan.vb_size = nbrec;
   plan.kt_size = ktot;
-  plan.vb.assign(9 * std::max<size_t>(1, plan.vb_size), 0.0);
-  for (int c = 0; c < 9; ++c) std::copy(vcol[c]->begin(), vcol[c]->end(), plan.vb.begin() + c * plan.vb_size);
+  if (brec.empty()) brec.assign(HFG_BRA_REC, 0.0);
+  plan.vb = std::move(brec);
   plan.kt.assign(9 * std::max<size_t>(1, ktot), 0.0);
   for (int c = 0; c < 9; ++c) std::copy(kt_cols[c].begin(), kt_cols[c].begin() + ktot, plan.kt.begin() + c * ktot);
-  if (bcoef.empty()) bcoef.push_back(0.0);
   if (vd.empty()) vd.emplace_back();
   if (cd.empty()) cd.push_back(ComboDev{0, 0, 0.0});
-  plan.bcoef = std::move(bcoef);
   plan.kcoef = std::move(kcoef);
   plan.views = std::move(vd);
   plan.combos = std::move(cd);

@@ -21,10 +21,11 @@ struct PairDev {
 // A view of a pair of generally contracted shells as the J/K kernel reads it:
 // na (<= RA) members of the first shell times nb (<= RB) members of the
 // second, kernel orientation (higher L first), with the primitive pairs that
-// matter for them.  Primitive data [p0, p1) in the view-contiguous bra
-// tables; contraction coefficients at coef0, [prim][RA * RB] (ga * RB + gb,
-// zero for missing members; only when RA * RB > 1, otherwise the single
-// combination's coefficient is folded into K); RA * RB combinations at c0 in
+// matter for them.  Primitive records [p0, p1) in the view-contiguous bra
+// table, each holding the contraction coefficients of the RA * RB
+// combinations (ga * RB + gb, zero for missing members; only when
+// RA * RB > 1, otherwise the single combination's coefficient is folded into
+// K); coef0 is unused; RA * RB combinations at c0 in
 // the combination table (missing members: Schwarz bound 0); same = 1 when
 // both sides are one shell.
 struct ViewDev {
@@ -67,8 +68,7 @@ struct JKPlan {
   std::vector<ViewDev> views;
   std::vector<ComboDev> combos;
   std::vector<KetGroup> groups;
-  std::vector<double> vb;     // bra primitive pairs, 9 columns of vb_size
-  std::vector<double> bcoef;  // bra coefficients
+  std::vector<double> vb;     // bra primitive records, HFG_BRA_REC doubles each (vb_size records)
   std::vector<double> kt;     // ket primitive pairs, 9 columns of kt_size
   std::vector<double> kcoef;  // ket coefficients
   std::vector<JKTask> tasks;
