@@ -87,6 +87,11 @@ int set_error(int code, const std::string& msg);
 struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Second stream (default priority) for the latency-bound high-L classes,
+  // which then overlap the large low-L classes; joined before symmetrisation.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int side_blocks = 256;  // workgroup cap of a side-stream class (HFG_SIDE_BLOCKS; 0: one stream)
   int n = 0;
   std::vector<Shell> shells;
   std::vector<PairInfo> pairs;  // hf-rs pair order

@@ -129,6 +129,10 @@ Engine::Engine(const hfg_basis_view& b) {
   if (ndev <= 0) throw HfgError{HFG_DEVICE, "no GPU visible"};
   HIP_CHECK(hipGetDevice(&device));
   HIP_CHECK(hipStreamCreate(&stream));  // default priority
+  HIP_CHECK(hipStreamCreate(&side));    // default priority
+  HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  if (const char* env = std::getenv("HFG_SIDE_BLOCKS")) side_blocks = std::max(0, std::atoi(env));
   {
     int cus = 0;
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -182,6 +186,9 @@ Engine::Engine(const hfg_basis_view& b) {
 
 Engine::~Engine() {
   for (auto e : events) (void)hipEventDestroy(e);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -400,17 +407,32 @@ void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* d
   a.Kacc = d_Kacc.as<double>();
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, d_counters.as<int>(), HFG_NCLASS);
   a.tasks = d_tasks.as<JKTask>();
-  for (int c = 0; c < HFG_NCLASS; ++c) {
-    const size_t nt = task_begin[c + 1] - task_begin[c];
-    if (!nt) continue;
-    a.task0 = (int)task_begin[c];
-    a.jobs = d_jobs.as<int>() + job_begin[c];
-    a.njob = use_rows ? (int)(job_begin[c + 1] - job_begin[c]) : 0;
-    a.job_counter = d_counters.as<int>() + c;
-    if (profile) HIP_CHECK(hipEventRecord(events[2 * c], stream));
-    launch_jk_class(c, a, (int)nt, g_max_blocks, stream);
-    HIP_CHECK(hipGetLastError());
-    if (profile) HIP_CHECK(hipEventRecord(events[2 * c + 1], stream));
+  // (dp|pp), (dp|ds), (dp|dp) and (dd|pp) .. (dd|dd): few, long quartets with
+  // register spills; on the side stream, with a capped grid, they fill the
+  // room the low-L classes leave.
+  auto on_side = [&](int c) { return side_blocks > 0 && (c == 12 || c == 13 || c == 14 || c >= 17); };
+  const bool two = side_blocks > 0;
+  if (two) {
+    HIP_CHECK(hipEventRecord(ev_fork, stream));
+    HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
+  }
+  for (int pass = 0; pass < 2; ++pass)
+    for (int c = 0; c < HFG_NCLASS; ++c) {
+      const size_t nt = task_begin[c + 1] - task_begin[c];
+      if (!nt || on_side(c) != (pass == 0)) continue;
+      hipStream_t s = on_side(c) ? side : stream;
+      a.task0 = (int)task_begin[c];
+      a.jobs = d_jobs.as<int>() + job_begin[c];
+      a.njob = use_rows ? (int)(job_begin[c + 1] - job_begin[c]) : 0;
+      a.job_counter = d_counters.as<int>() + c;
+      if (profile) HIP_CHECK(hipEventRecord(events[2 * c], s));
+      launch_jk_class(c, a, (int)nt, on_side(c) ? side_blocks : g_max_blocks, s);
+      HIP_CHECK(hipGetLastError());
+      if (profile) HIP_CHECK(hipEventRecord(events[2 * c + 1], s));
+    }
+  if (two) {
+    HIP_CHECK(hipEventRecord(ev_join, side));
+    HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
   }
   hipLaunchKernelGGL(k_symmetrize, dim3((nn + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(),
                      d_Kacc.as<double>(), dJ, dK, n, dH, dF);

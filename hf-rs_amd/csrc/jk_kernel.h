@@ -44,6 +44,16 @@ constexpr int class_index() {
   return found;
 }
 
+// Waves per SIMD the register allocator must leave room for in k_jk_rows:
+// two for (ss|ss), (ps|ss) and (pp|ss), whose few spills cost less than the
+// latency a second wave hides; one (no constraint) elsewhere (measured on
+// (H2O)16/cc-pVDZ).
+template <int LA, int LB, int LC, int LD>
+constexpr int jk_min_waves() {
+  constexpr int c = class_index<LA, LB, LC, LD>();
+  return (c == 0 || c == 1 || c == 3) ? 2 : 1;
+}
+
 // J/K partial sums live in one copy per XCC (accelerator complex die, each
 // with its own L2).  A wave never migrates, so every update to copy x comes
 // from XCC x and a workgroup-scope atomic -- performed in that XCC's L2 --
@@ -340,7 +350,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
 // job the workgroup adds the nonzero entries to the global copies and clears
 // them.  Only the ket Coulomb updates go to global memory directly.
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(256) void k_jk_rows(JKArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves<LA, LB, LC, LD>()))) void k_jk_rows(JKArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB;
   constexpr int CLS = class_index<LA, LB, LC, LD>();
   constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1], CA = RA * RB;
@@ -410,7 +420,7 @@ void launch_jk(const JKArgs& a, int ntask, int max_blocks, hipStream_t s) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jk_rows<LA, LB, LC, LD>,
                                                            64 * waves_per_block, lds));
-    const int blocks = std::max(1, std::min(a.njob, std::max(1, per_cu) * cus));
+    const int blocks = std::max(1, std::min(std::min(a.njob, max_blocks), std::max(1, per_cu) * cus));
     hipLaunchKernelGGL((k_jk_rows<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), lds, s, b);
     return;
   }
