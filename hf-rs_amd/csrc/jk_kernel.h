@@ -408,19 +408,31 @@ void launch_jk(const JKArgs& a, int ntask, int max_blocks, hipStream_t s) {
   JKArgs b = a;
   b.ntask = ntask;
   const int waves_per_block = 4;
+  // Device limits and the resident-workgroup count of this kernel for the
+  // current LDS size, queried once per (device, size): the queries cost more
+  // than a launch.
+  static thread_local int cached_dev = -1, lds_limit = 0, cus = 0, resident = 0;
+  static thread_local size_t cached_lds = 0;
   int dev = 0;
   HIP_CHECK(hipGetDevice(&dev));
-  static int lds_limit = -1;  // bytes of LDS one workgroup may use
-  if (lds_limit < 0) HIP_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   const size_t lds = jk_rows_bytes<LA, LB, LC, LD>(a.n);
-  if (a.njob > 0 && lds + sizeof(double) * BOYS_ROWS * BOYS_SLICE + 64 <= (size_t)lds_limit) {
-    int per_cu = 0, cus = 0;
+  if (dev != cached_dev) {
+    HIP_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_CHECK(hipFuncSetAttribute((const void*)k_jk_rows<LA, LB, LC, LD>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jk_rows<LA, LB, LC, LD>,
-                                                           64 * waves_per_block, lds));
-    const int blocks = std::max(1, std::min(std::min(a.njob, max_blocks), std::max(1, per_cu) * cus));
+    cached_dev = dev;
+    cached_lds = 0;
+  }
+  if (a.njob > 0 && lds + sizeof(double) * BOYS_ROWS * BOYS_SLICE + 64 <= (size_t)lds_limit) {
+    if (lds != cached_lds) {
+      int per_cu = 0;
+      HIP_CHECK(hipFuncSetAttribute((const void*)k_jk_rows<LA, LB, LC, LD>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jk_rows<LA, LB, LC, LD>,
+                                                             64 * waves_per_block, lds));
+      resident = std::max(1, per_cu) * cus;
+      cached_lds = lds;
+    }
+    const int blocks = std::max(1, std::min(std::min(a.njob, max_blocks), resident));
     hipLaunchKernelGGL((k_jk_rows<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), lds, s, b);
     return;
   }

@@ -209,7 +209,11 @@ static AtomicDensity atomic_hf(const std::vector<Shell>& sh_in, double z) {
   hfg_basis_view view{n, atom.data(), cen.data(), lx.data(), po.data(), ex.data(), co.data()};
   std::unique_ptr<Engine> eng;
   const bool hydrogenic = std::fabs(z - 1.0) < 1e-12;
+  const bool trace = std::getenv("HFG_TRACE") != nullptr;
+  const auto t_start = std::chrono::steady_clock::now();
+  auto since = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
   if (!hydrogenic) eng.reset(new Engine(view));
+  if (trace) std::fprintf(stderr, "[hfg] atomic HF Z=%g n=%d: engine %.4f s\n", z, n, since());
   const size_t nn = (size_t)n * n;
   auto fock = [&](const HMat& P) {
     if (hydrogenic) return H;
@@ -256,6 +260,7 @@ static AtomicDensity atomic_hf(const std::vector<Shell>& sh_in, double z) {
     for (size_t i = 0; i < nn; ++i) rms += (P[i] - next[i]) * (P[i] - next[i]);
     rms = std::sqrt(rms / nn);
     if (std::fabs(e - prev) < 1e-11 && rms < 1e-9 && emax < 1e-8) {
+      if (trace) std::fprintf(stderr, "[hfg] atomic HF Z=%g: %d iterations, %.4f s\n", z, it, since());
       HMat Fn = fock(next);
       return {next, energy(next, Fn)};
     }
