@@ -38,7 +38,7 @@ class BasisView(C.Structure):
 class ScfOptions(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("energy_tol", C.c_double), ("residual_tol", C.c_double),
                 ("density_tol", C.c_double), ("conv_algo", C.c_int32), ("spherical", C.c_int32),
-                ("verbose", C.c_int32)]
+                ("verbose", C.c_int32), ("incremental_fock", C.c_int32)]
 
 
 class ScfResult(C.Structure):
@@ -310,14 +310,16 @@ CONV = {"adiissoscf": 0, "default": 0, "cdiis": 1, "diis": 1, "pulay": 1, "adiis
 
 
 def scf_rhf(basis, zxyz, charge=0, max_iter=100, tol=1e-10, residual=None, density=None, conv="adiissoscf",
-            spherical=False, verbose=False, want_matrices=False):
-    """RHF with the derived tolerances of run_scf_in_space (src/scf.rs:384-400)."""
+            spherical=False, verbose=False, want_matrices=False, incremental=False):
+    """RHF with the derived tolerances of run_scf_in_space (src/scf.rs:384-400).
+    incremental: J/K of the density change inside the iterations (full first
+    build and final rebuild)."""
     if residual is None:
         residual = max(math.sqrt(tol) * 1e-2, 1e-11)
     if density is None:
         density = min(max(math.sqrt(tol) * 0.1, 1e-12), 1e-10)
     key = conv.replace("-", "").replace("_", "").lower()
-    opt = ScfOptions(max_iter, tol, residual, density, CONV[key], int(spherical), int(verbose))
+    opt = ScfOptions(max_iter, tol, residual, density, CONV[key], int(spherical), int(verbose), int(incremental))
     res = ScfResult()
     zxyz = np.ascontiguousarray(zxyz, dtype=np.float64)
     n = basis.n

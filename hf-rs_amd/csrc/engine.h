@@ -27,7 +27,9 @@ struct JKArgs {
   PrimPairs kt;          // ket side, transposed per group
   const double* kcoef;   // ket coefficients, transposed per group
   size_t nn;             // n*n, the stride between per-XCC accumulator copies
-  double screen;         // Schwarz threshold of a combination pair
+  double screen;         // Schwarz threshold of a combination pair in this build
+  double screen0;        // the threshold the task lists were built with
+  const double* vq;      // per view: largest Schwarz bound of its combinations
   const double* scale;
   const double* boys;
   const double* D;
@@ -101,7 +103,7 @@ struct Engine {
   int64_t n_quartets = 0;
   double screen = 1e-12;        // Q_ab*Q_cd > screen (src/shell_integrals.rs:184,226)
 
-  DevBuf d_pp, d_vb, d_kt, d_kcoef, d_groups, d_pairs, d_views, d_combos, d_boys, d_scale, d_tasks, d_jobs,
+  DevBuf d_vq, d_pp, d_vb, d_kt, d_kcoef, d_groups, d_pairs, d_views, d_combos, d_boys, d_scale, d_tasks, d_jobs,
       d_counters, d_D, d_Jacc, d_Kacc, d_J, d_K, d_H, d_F;
   std::vector<size_t> task_begin;  // per class, into d_tasks (HFG_NCLASS+1 entries)
   std::vector<size_t> job_begin;   // per class, into d_jobs (HFG_NCLASS+1 entries)
@@ -134,8 +136,11 @@ struct Engine {
   size_t prim_pairs_kept = 0;  // primitive pairs over all views
   int n_gen_shells = 0;
   int n_views = 0;
-  // J/K from the device density d_D into d_J, d_K (and F = H + J - K/2 when H given).
-  void jk_device(const double* dP, double* dJ, double* dK, const double* dH = nullptr, double* dF = nullptr);
+  // J/K from the device density dP into dJ, dK (and F = H + J - K/2 when H
+  // given; F may alias H).  dmax < 1 (the largest |element| of a density
+  // change) raises the Schwarz threshold to screen / dmax for this build.
+  void jk_device(const double* dP, double* dJ, double* dK, const double* dH = nullptr, double* dF = nullptr,
+                 double dmax = 1.0);
   void sync();
   void set_profile(bool on);
 };

@@ -321,6 +321,7 @@ void Engine::build_tasks() {
   up_load(d_kcoef, plan.kcoef.data(), plan.kcoef.size() * sizeof(double));
   up_load(d_groups, plan.groups.data(), plan.groups.size() * sizeof(KetGroup));
   up_load(d_views, plan.views.data(), plan.views.size() * sizeof(ViewDev));
+  up_load(d_vq, plan.view_q.data(), plan.view_q.size() * sizeof(double));
   up_load(d_combos, plan.combos.data(), plan.combos.size() * sizeof(ComboDev));
   up_load(d_tasks, plan.tasks.data(), plan.tasks.size() * sizeof(JKTask));
   up_load(d_jobs, plan.jobs.data(), plan.jobs.size() * sizeof(int));
@@ -384,7 +385,7 @@ void Engine::count_work() {
   }
 }
 
-void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* dH, double* dF) {
+void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* dH, double* dF, double dmax) {
   const size_t nn = (size_t)n * n;
   const int tb = 256;
   const size_t nacc = nn * NXCC_SLOTS;
@@ -399,7 +400,9 @@ void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* d
   a.brec = d_vb.as<double>();
   a.kt = kt_view();
   a.kcoef = d_kcoef.as<double>();
-  a.screen = screen;
+  a.screen0 = screen;
+  a.screen = dmax > 0.0 && dmax < 1.0 ? screen / dmax : screen;
+  a.vq = d_vq.as<double>();
   a.scale = d_scale.as<double>();
   a.boys = d_boys.as<double>();
   a.D = dP;

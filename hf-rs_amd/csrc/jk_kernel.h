@@ -100,8 +100,24 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
   const JKTask t = a.tasks[task];
   const int bra = __builtin_amdgcn_readfirstlane(t.bra);
   const int ket0 = __builtin_amdgcn_readfirstlane(t.ket0);
-  const int nket = __builtin_amdgcn_readfirstlane(t.nket);
+  int nket = __builtin_amdgcn_readfirstlane(t.nket);
   const int grp = __builtin_amdgcn_readfirstlane(t.grp);
+  if (a.screen > a.screen0) {
+    // A raised threshold (a density change): the task's kets, sorted by
+    // bound, keep the prefix with qmax_bra * qmax_ket > screen.
+    const cdptr vq = (cdptr)a.vq;
+    const double qb = vq[bra];
+    int lo = 0, hi = nket;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (qb * vq[ket0 + mid] > a.screen)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    nket = lo;
+    if (nket == 0) return;  // wave-uniform
+  }
   const bool act = lane < nket;
   const int ket = ket0 + (act ? lane : 0);
   const ViewDev& vb = a.views[bra];

@@ -341,6 +341,7 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
       d.same = P.same ? 1 : 0;
       for (int k = 0; k < 3; ++k) d.AB[k] = P.AB[k];
       vd.push_back(d);
+      plan.view_q.push_back(v.qmax);
       for (int ci : v.combos) {
         // a missing member repeats the view's first combination with bound 0
         const auto& mm = P.combos[ci >= 0 ? ci : v.combos[0]];
@@ -465,7 +466,10 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
   plan.vb = std::move(brec);
   plan.kt.assign(9 * std::max<size_t>(1, ktot), 0.0);
   for (int c = 0; c < 9; ++c) std::copy(kt_cols[c].begin(), kt_cols[c].begin() + ktot, plan.kt.begin() + c * ktot);
-  if (vd.empty()) vd.emplace_back();
+  if (vd.empty()) {
+    vd.emplace_back();
+    plan.view_q.push_back(0.0);
+  }
   if (cd.empty()) cd.push_back(ComboDev{0, 0, 0.0});
   plan.kcoef = std::move(kcoef);
   plan.views = std::move(vd);

@@ -66,6 +66,7 @@ constexpr int GC_CAPS[HFG_NCLASS][4] = {HFG_GC_CAPS};
 // Everything the J/K kernels read besides the density, built on the host.
 struct JKPlan {
   std::vector<ViewDev> views;
+  std::vector<double> view_q;  // per view: the largest Schwarz bound of its combinations
   std::vector<ComboDev> combos;
   std::vector<KetGroup> groups;
   std::vector<double> vb;     // bra primitive records, HFG_BRA_REC doubles each (vb_size records)

@@ -32,6 +32,10 @@ __global__ inline void k_sub(const double* a, const double* b, double* out, size
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] - b[i];
 }
+__global__ inline void k_add(const double* a, const double* b, double* out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
 __global__ inline void k_zero(double* a, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.0;

@@ -460,6 +460,37 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
     t_fock += std::chrono::duration<double>(clk::now() - t0).count();
     ++builds;
   };
+  // Incremental builds inside the iterations (opt->incremental_fock): G, the
+  // two-electron part of the last build, and Pb, its density, are kept; a
+  // build of P adds J - K/2 of dP = P - Pb, screened at 1e-12 / max|dP|.
+  // Every 12th build, and the first, is full.
+  const bool incremental = opt->incremental_fock != 0;
+  Dev dG2, dPb;
+  int since_full = -1;  // -1: no G yet
+  auto fock_iter = [&](const double* P, double* F) {
+    if (!incremental) return fock(P, F);
+    if (since_full < 0 || since_full >= 12) {
+      fock(P, F);
+      if (!dG2.p) {
+        dG2.alloc(nn);
+        dPb.alloc(nn);
+      }
+      hipLaunchKernelGGL(k_sub, grid(nn), dim3(tb), 0, st, F, dH.p, dG2.p, nn);
+      HIP_CHECK(hipMemcpyAsync(dPb.p, P, nn * 8, hipMemcpyDeviceToDevice, st));
+      since_full = 0;
+      return;
+    }
+    const auto t0 = clk::now();
+    hipLaunchKernelGGL(k_sub, grid(nn), dim3(tb), 0, st, P, dPb.p, dPb.p, nn);  // dPb <- P - Pb
+    const double dmax = bl.amax(nn, dPb.p);
+    if (dmax > 0.0) eng.jk_device(dPb.p, nullptr, nullptr, dG2.p, dG2.p, dmax);
+    hipLaunchKernelGGL(k_add, grid(nn), dim3(tb), 0, st, dH.p, dG2.p, F, nn);
+    HIP_CHECK(hipMemcpyAsync(dPb.p, P, nn * 8, hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    t_fock += std::chrono::duration<double>(clk::now() - t0).count();
+    ++builds;
+    ++since_full;
+  };
   // diagonalise X^T F X, C = X C', P = 2 C_occ C_occ^T; eigenvalues into eps
   std::vector<double> eps(m);
   auto diag_density = [&](const double* F, double* P, bool want_eps) {
@@ -537,7 +568,7 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   HIP_CHECK(hipMemcpyAsync(dPprev.p, dP.p, nn * 8, hipMemcpyDeviceToDevice, st));
   for (int it = 1; it <= opt->max_iter; ++it) {
     iters = it;
-    fock(dP.p, dF.p);
+    fock_iter(dP.p, dF.p);
     const double tot = energy(dP.p, dF.p) + enuc;
     const double de = std::fabs(tot - prev_e);
     prev_e = tot;

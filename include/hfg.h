@@ -120,6 +120,11 @@ typedef struct {
   int32_t conv_algo; /* 0 ADIIS->CDIIS (default), 1 CDIIS, 2 ADIIS, 3 ADIIS->CDIIS at 0.08 */
   int32_t spherical; /* BasisKind::Spherical for the orthogonaliser */
   int32_t verbose;
+  /* 1: inside the iterations, build J and K of the density change dP only
+   * (F = H + G_prev + G(dP)) with the Schwarz threshold divided by max|dP|
+   * when that is below 1; the first build and the final rebuild are full.
+   * 0: every build full, as the reference. */
+  int32_t incremental_fock;
 } hfg_scf_options;
 
 typedef struct {

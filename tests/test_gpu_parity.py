@@ -188,3 +188,32 @@ def test_rhf_h2o16_ccpvdz_matches_oracle_at_full_size(hf):
     assert r["converged"]
     assert abs(r["total_energy"] - ref["total_energy"]) < 1e-8
     assert abs(r["iterations"] - ref["iterations"]) <= 1
+
+
+@pytest.mark.parametrize("inc", [False, True])
+def test_rhf_water_ccpvdz_spherical_incremental_builds(hf, inc):
+    """With incremental Fock builds (J/K of the density change, threshold
+    1e-12 / max|dP|, full first build and final rebuild) and without, the
+    PySCF energy of spherical water/cc-pVDZ (tests/data/spherical_reference.json:
+    2283-2285) within 1e-9."""
+    sysd = json.load(open(os.path.join(GOLDEN, "scf_energies.json")))["systems"]["water_ccpvdz_spherical"]
+    syms, xyz = parse_geometry(sysd["geometry_bohr"])
+    mol = M.Molecule(syms, xyz)
+    r = hf.scf_rhf(hf.Basis.from_g94(M.basis_text("cc-pvdz"), syms, xyz), mol.zxyz, max_iter=150, tol=1e-13,
+                   conv="cdiis", spherical=True, incremental=inc)
+    assert r["converged"]
+    assert abs(r["total_energy"] - sysd["results"]["cc-pvdz"]["energy"]) < 1e-9
+
+
+def test_rhf_h2o16_incremental_matches_oracle(hf):
+    """The config-2 solve with incremental builds: energy within 1e-8 Eh and
+    iterations within 1 of the oracle's in-core run."""
+    ref = json.load(open(os.path.join(GOLDEN, "h2o16_ccpvdz_scf.json")))
+    mol = M.water_cluster((4, 2, 2), seed=2)
+    r = hf.scf_rhf(hf.Basis.from_g94(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz), mol.zxyz, tol=1e-10,
+                   conv="adiissoscf", spherical=True, incremental=True)
+    print("(H2O)16 incremental: E %.12f vs %.12f, iterations %d vs %d" % (r["total_energy"], ref["total_energy"],
+                                                                           r["iterations"], ref["iterations"]))
+    assert r["converged"]
+    assert abs(r["total_energy"] - ref["total_energy"]) < 1e-8
+    assert abs(r["iterations"] - ref["iterations"]) <= 1
