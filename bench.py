@@ -53,6 +53,40 @@ def cpu_baseline(mol, text, stride, threads, density):
                                                                           threads)}
 
 
+def batch_report(hf, M, count, steps=5):
+    """The batched workload: `count` molecules (half all-trans alkanes
+    CnH2n+2, n ~ U{1..12}, half water clusters (H2O)m, m ~ U{1..8}; PCG64 seed
+    3) in def2-SVP; one step = J/K of every molecule in fused launches, next to
+    the same builds molecule after molecule."""
+    mols = M.batch_config3(count, seed=3)
+    text = M.basis_text("def2-svp")
+    bases = [hf.Basis.from_g94(text, m.symbols, m.xyz) for m in mols]
+    t0 = time.perf_counter()
+    batch = hf.Batch(bases)
+    setup = time.perf_counter() - t0
+    info = batch.info()
+    rng = np.random.default_rng(6)
+    for i, b in enumerate(bases):
+        c = rng.standard_normal((b.n, max(1, b.n // 4))) * 0.3
+        batch.set_density(i, c @ c.T)
+    batch.jk()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.jk()
+    fused = (time.perf_counter() - t0) / steps
+    batch.jk_each()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.jk_each()
+    each = (time.perf_counter() - t0) / steps
+    return {"workload": "%d molecules def2-SVP (alkanes C1-C12, water clusters 1-8; seed 3), Cartesian n %d..%d"
+                        % (count, min(b.n for b in bases), max(b.n for b in bases)),
+            "fock_builds_per_s": count / fused, "ms_per_batch": 1e3 * fused,
+            "algorithmic_tflops": info["flops"] / fused / 1e12, "shell_quartets": info["quartets"],
+            "unbatched_fock_builds_per_s": count / each, "unbatched_ms_per_batch": 1e3 * each,
+            "speedup_vs_unbatched": each / fused, "setup_s": setup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +96,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--batch", action="store_true",
+                    help="also time the batched multi-molecule workload (off: see DESIGN.md, batches)")
+    ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="run only the counter-calibration copy kernel over BYTES bytes and exit")
     args = ap.parse_args()
@@ -169,6 +206,8 @@ def main():
             line["scf"]["dE_vs_oracle"] = r["total_energy"] - o["total_energy"]
             line["scf"]["full_build_solve"]["dE_vs_oracle"] = full["total_energy"] - o["total_energy"]
             line["scf"]["oracle_iterations"] = o["iterations"]
+    if args.batch:
+        line["batch"] = batch_report(hf, M, args.batch_count)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mol, text, args.cpu_stride, args.cpu_threads, dens)
     print(json.dumps(line), flush=True)

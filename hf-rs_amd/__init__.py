@@ -54,7 +54,9 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_contract_jk_device", "hfg_scf_rhf", "hfg_basis_from_g94", "hfg_basis_free",
             "hfg_basis_get_view", "hfg_set_density", "hfg_jk_resident", "hfg_get_jk", "hfg_profile",
             "hfg_class_stats", "hfg_fock_flops", "hfg_md_flops", "hfg_persistent_blocks", "hfg_df_build",
-            "hfg_df_destroy", "hfg_df_info", "hfg_df_jk", "hfg_df_rhf", "hfg_calibration_copy"]
+            "hfg_df_destroy", "hfg_df_info", "hfg_df_jk", "hfg_df_rhf", "hfg_calibration_copy",
+            "hfg_batch_create", "hfg_batch_destroy", "hfg_batch_info", "hfg_batch_set_density", "hfg_batch_jk",
+            "hfg_batch_get_jk", "hfg_batch_jk_each"]
 
 
 class DfhfOptions(C.Structure):
@@ -107,6 +109,13 @@ def lib():
         L.hfg_df_jk.argtypes = [vp, vp, i32, vp, vp]
         L.hfg_df_rhf.argtypes = [C.POINTER(BasisView), C.POINTER(BasisView), vp, i32, i32, C.POINTER(DfhfOptions),
                                  C.POINTER(DfhfResult), vp, vp, vp, vp]
+        L.hfg_batch_create.argtypes = [vp, i32, C.POINTER(vp)]
+        L.hfg_batch_destroy.argtypes = [vp]
+        L.hfg_batch_info.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3 + [C.POINTER(dbl)]
+        L.hfg_batch_set_density.argtypes = [vp, i32, vp]
+        L.hfg_batch_jk.argtypes = [vp]
+        L.hfg_batch_jk_each.argtypes = [vp]
+        L.hfg_batch_get_jk.argtypes = [vp, i32, vp, vp]
         _LIB = L
     return _LIB
 
@@ -173,6 +182,50 @@ class Basis:
                    coefficient=arr(v.coefficient, int(po[-1]), C.c_double))
         lib().hfg_basis_free(h)
         return cls(out)
+
+
+class Batch:
+    """Many molecules whose J/K builds run in fused launches (hfg_batch_*)."""
+
+    def __init__(self, bases):
+        self.bases = list(bases)
+        arr = (BasisView * len(self.bases))(*[b.view for b in self.bases])
+        self._arr = arr
+        self.h = C.c_void_p()
+        _check(lib().hfg_batch_create(C.cast(arr, C.c_void_p), len(self.bases), C.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value and _LIB is not None:
+            _LIB.hfg_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        v = [C.c_int64() for _ in range(3)]
+        f = C.c_double()
+        _check(lib().hfg_batch_info(self.h, *[C.byref(x) for x in v], C.byref(f)))
+        return dict(count=v[0].value, n_max=v[1].value, quartets=v[2].value, flops=f.value)
+
+    def set_density(self, mol, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        _check(lib().hfg_batch_set_density(self.h, mol, _p(p)))
+
+    def jk(self):
+        """J/K of every molecule's resident density; results stay in HBM."""
+        _check(lib().hfg_batch_jk(self.h))
+
+    def jk_each(self):
+        """The same builds molecule after molecule (unbatched launches)."""
+        _check(lib().hfg_batch_jk_each(self.h))
+
+    def get_jk(self, mol):
+        n = self.bases[mol].n
+        j = np.zeros((n, n))
+        k = np.zeros((n, n))
+        _check(lib().hfg_batch_get_jk(self.h, mol, _p(j), _p(k)))
+        return j, k
 
 
 class Context:

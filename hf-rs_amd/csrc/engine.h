@@ -16,6 +16,17 @@ namespace hfg {
 // Number of per-XCC copies of the J/K accumulators (see jk_kernel.h).
 constexpr int NXCC_SLOTS = 16;
 
+// One molecule of a batch as the J/K kernels address it: its density, the
+// first of its per-XCC accumulator copies, its AO component factors.
+struct MolDev {
+  const double* D;
+  double* Jacc;
+  double* Kacc;
+  const double* scale;
+  size_t nn;
+  int n, pad;
+};
+
 // Arguments of the J/K kernels (jk_kernel.h).
 struct JKArgs {
   const JKTask* tasks;   // all classes' tasks; this class's start at task0
@@ -40,6 +51,10 @@ struct JKArgs {
   const int* jobs;
   int njob;
   int* job_counter;
+  // Batch: per-molecule addressing (ViewDev::mol); null for one molecule,
+  // which then uses n, nn, D, Jacc, Kacc and scale above (n is the largest
+  // molecule's for a batch: it sizes the LDS rows).
+  const MolDev* mols;
 };
 // Launch of class cls's J/K kernel over ntask tasks with at most max_blocks
 // workgroups; the kernels are split over three translation units.
@@ -134,6 +149,7 @@ struct Engine {
               const PairDev* pair_table = nullptr, const PrimPairs* prims = nullptr);
   double prim_screen = 1e-20;
   size_t prim_pairs_kept = 0;  // primitive pairs over all views
+  JKPlan plan;                 // host copy of the J/K plan (a batch concatenates them)
   int n_gen_shells = 0;
   int n_views = 0;
   // J/K from the device density dP into dJ, dK (and F = H + J - K/2 when H

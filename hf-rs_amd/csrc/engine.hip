@@ -311,7 +311,7 @@ void Engine::build_tasks() {
   };
   std::vector<double> Qhf(pairs.size());
   for (size_t i = 0; i < pairs.size(); ++i) Qhf[pairs[i].hf_index] = Q[i];
-  JKPlan plan = make_jk_plan(shells, Qhf, ao_scale, screen, prim_screen, gpu_blocks);
+  plan = make_jk_plan(shells, Qhf, ao_scale, screen, prim_screen, gpu_blocks);
   auto up_load = [&](DevBuf& d, const void* src, size_t bytes) {
     d.alloc(std::max<size_t>(bytes, 16));
     if (bytes) HIP_CHECK(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream));
@@ -408,6 +408,7 @@ void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* d
   a.D = dP;
   a.Jacc = d_Jacc.as<double>();
   a.Kacc = d_Kacc.as<double>();
+  a.mols = nullptr;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, d_counters.as<int>(), HFG_NCLASS);
   a.tasks = d_tasks.as<JKTask>();
   // (dp|pp), (dp|ds), (dp|dp) and (dd|pp) .. (dd|dd): few, long quartets with

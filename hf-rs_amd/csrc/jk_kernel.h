@@ -77,6 +77,35 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Per-molecule addresses of a task or job: the molecule's density, this
+// XCC's accumulator copies, its AO factors and size.
+struct MolCtx {
+  const double* D;
+  double* Jx;
+  double* Kx;
+  const double* scale;
+  int n;
+};
+__device__ __forceinline__ MolCtx mol_ctx(const JKArgs& a, int mol) {
+  const double* D = a.D;
+  double* J = a.Jacc;
+  double* K = a.Kacc;
+  const double* sc = a.scale;
+  size_t nn = a.nn;
+  int n = a.n;
+  if (a.mols) {
+    const MolDev& m = a.mols[mol];
+    D = m.D;
+    J = m.Jacc;
+    K = m.Kacc;
+    sc = m.scale;
+    nn = m.nn;
+    n = __builtin_amdgcn_readfirstlane(m.n);
+  }
+  const size_t slot = (size_t)xcc_id() * nn;
+  return MolCtx{D, J + slot, K + slot, sc, n};
+}
+
 // One task: the bra view (wave-uniform) against up to 64 ket views.  The
 // CA x CC blocks of a lane are scaled (weight, component factors, and 0 for a
 // combination whose hf-rs quartet fails the Schwarz test or whose member is
@@ -145,7 +174,8 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
   for (int z = 0; z < RC; ++z) Co[z] = cket[z * RD].ao_a;
 #pragma unroll
   for (int u = 0; u < RD; ++u) Do[u] = cket[u].ao_b;
-  const double* __restrict__ sc_ = a.scale;
+  const MolCtx mc = mol_ctx(a, __builtin_amdgcn_readfirstlane(vb.mol));
+  const double* __restrict__ sc_ = mc.scale;
   // scaling
 #pragma unroll
   for (int xy = 0; xy < CA; ++xy) {
@@ -169,11 +199,10 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
     }
   }
 
-  const double* __restrict__ Dm = a.D;
-  const int n = a.n;
-  const size_t slot = (size_t)xcc_id() * a.nn;
-  double* __restrict__ Jx = a.Jacc + slot;
-  double* __restrict__ Kx = a.Kacc + slot;
+  const double* __restrict__ Dm = mc.D;
+  const int n = mc.n;
+  double* __restrict__ Jx = mc.Jx;
+  double* __restrict__ Kx = mc.Kx;
   // Each term is its own pass over the blocks, one output element at a time,
   // so that only the blocks, one density block and one sum are live.
   // Coulomb, bra side: J[a_i, b_j] over c, d and the wave (all lanes share the bra).
@@ -374,14 +403,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
   __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
   __shared__ int job_sh;
   extern __shared__ double rows[];
-  const int n = a.n;
-  const int nrow = NR * n + CA * NAB;
-  for (int e = threadIdx.x; e < nrow; e += blockDim.x) rows[e] = 0.0;
+  // a.n is the largest molecule's size: it bounds every job's rows
+  for (int e = threadIdx.x; e < NR * a.n + CA * NAB; e += blockDim.x) rows[e] = 0.0;
   load_boys_slice<LA + LB + LC + LD>(a.boys, boys);  // ends with a barrier
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  const size_t slot = (size_t)xcc_id() * a.nn;
-  double* __restrict__ Jx = a.Jacc + slot;
-  double* __restrict__ Kx = a.Kacc + slot;
   for (;;) {
     if (threadIdx.x == 0) job_sh = atomicAdd(a.job_counter, 1);
     __syncthreads();
@@ -393,6 +418,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     __syncthreads();
     const ViewDev& vb = a.views[a.tasks[t0].bra];
     const ComboDev* __restrict__ cbra = a.combos + vb.c0;
+    const MolCtx mc = mol_ctx(a, __builtin_amdgcn_readfirstlane(vb.mol));
+    const int n = mc.n;
+    const int nrow = NR * n + CA * NAB;
+    double* __restrict__ Jx = mc.Jx;
+    double* __restrict__ Kx = mc.Kx;
     for (int e = threadIdx.x; e < nrow; e += blockDim.x) {
       const double v = rows[e];
       if (v == 0.0) continue;

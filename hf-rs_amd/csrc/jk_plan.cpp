@@ -453,6 +453,7 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
                         const std::pair<double, std::vector<JKTask>>& y) { return x.first > y.first; });
     for (auto& jb : jobs_c) {
       plan.jobs.push_back((int)all.size());
+      plan.job_cost.push_back(jb.first);
       all.insert(all.end(), jb.second.begin(), jb.second.end());
     }
   }

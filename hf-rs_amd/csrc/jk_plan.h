@@ -29,7 +29,7 @@ struct PairDev {
 // the combination table (missing members: Schwarz bound 0); same = 1 when
 // both sides are one shell.
 struct ViewDev {
-  int p0, p1, na, nb, c0, coef0, same, pad0;
+  int p0, p1, na, nb, c0, coef0, same, mol;  // mol: molecule of a batch (0 alone)
   double AB[3], pad2;
 };
 
@@ -78,6 +78,7 @@ struct JKPlan {
   // indices, one sentinel at the end); class c owns jobs job_begin[c] ..
   // job_begin[c + 1] - 1.
   std::vector<int> jobs;
+  std::vector<double> job_cost;   // estimated cost per job (same order)
   std::vector<size_t> job_begin;  // HFG_NCLASS + 1 entries
   size_t vb_size = 0, kt_size = 0, prim_pairs_kept = 0;
   int n_gen_shells = 0;

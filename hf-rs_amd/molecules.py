@@ -98,37 +98,45 @@ def water_cluster_m(m, seed, spacing_angstrom=2.9):
 
 
 def alkane(n):
-    """All-trans CnH2n+2 zig-zag: C-C 1.54 A, C-H 1.09 A, tetrahedral angles."""
+    """All-trans CnH2n+2: planar C zig-zag with C-C 1.54 A and C-C-C
+    109.471 deg, C-H 1.09 A, every carbon tetrahedral.  A terminal carbon gets
+    a hydrogen where the zig-zag would continue."""
     cc, ch = 1.54 * ANGSTROM_TO_BOHR, 1.09 * ANGSTROM_TO_BOHR
-    theta = math.acos(-1.0 / 3.0)
-    dx = cc * math.sin(theta / 2)
-    dy = cc * math.cos(theta / 2)
-    carbons = [np.array([i * dx, (i % 2) * dy, 0.0]) for i in range(n)]
-    symbols, xyz = [], []
+    phi = (math.pi - math.acos(-1.0 / 3.0)) / 2  # bond direction against the chain axis
+    bond = [np.array([math.cos(phi), math.sin(phi), 0.0]), np.array([math.cos(phi), -math.sin(phi), 0.0])]
+    carbons = [np.zeros(3)]
+    for i in range(n - 1):
+        carbons.append(carbons[-1] + cc * bond[i % 2])
+    symbols = ["C"] * n
+    xyz = list(carbons)
+    s = math.sqrt(2.0 / 3.0)
     for i, c in enumerate(carbons):
-        symbols.append("C")
-        xyz.append(c)
-    for i, c in enumerate(carbons):
-        up = 1.0 if i % 2 == 0 else -1.0
-        # two out-of-plane hydrogens per carbon, bisecting the C-C-C plane
-        out = ch * math.cos(theta / 2)
-        inplane = ch * math.sin(theta / 2)
-        base = c + np.array([0.0, -up * out * 0.0, 0.0])
-        for s in (1.0, -1.0):
-            symbols.append("H")
-            xyz.append(base + np.array([0.0, -up * out, s * inplane]))
+        u_prev = -bond[(i - 1) % 2]  # toward C(i-1), or where it would be
+        u_next = bond[i % 2]         # toward C(i+1), or where it would be
         if i == 0:
             symbols.append("H")
-            xyz.append(c + ch * np.array([-math.sin(theta / 2), up * math.cos(theta / 2), 0.0]) *
-                       np.array([1.0, -1.0, 1.0]))
-        if i == n - 1 and n > 1:
-            d = carbons[i] - carbons[i - 1]
+            xyz.append(c + ch * u_prev)
+        if i == n - 1:
             symbols.append("H")
-            xyz.append(c + ch * d / np.linalg.norm(d))
-    if n == 1:
-        symbols, xyz = ["C"], [carbons[0]]
-        t = ch / math.sqrt(3.0)
-        for v in ((1, 1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1)):
+            xyz.append(c + ch * u_next)
+        normal = np.cross(u_prev, u_next)
+        normal /= np.linalg.norm(normal)
+        mid = -(u_prev + u_next) / 2
+        for sign in (1.0, -1.0):
             symbols.append("H")
-            xyz.append(carbons[0] + t * np.array(v, float))
+            xyz.append(c + ch * (mid + sign * s * normal))
     return Molecule(symbols, xyz, name="C%dH%d" % (n, 2 * n + 2))
+
+
+def batch_config3(count=512, seed=3):
+    """The batched workload: half all-trans alkanes CnH2n+2 with n ~ U{1..12},
+    half water clusters (H2O)m with m ~ U{1..8} on the smallest near-cubic
+    lattice, in a seeded order (PCG64)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mols = []
+    for i in range(count):
+        if i % 2 == 0:
+            mols.append(alkane(int(rng.integers(1, 13))))
+        else:
+            mols.append(water_cluster_m(int(rng.integers(1, 9)), seed=int(rng.integers(0, 2 ** 31))))
+    return mols

@@ -174,6 +174,27 @@ int hfg_df_rhf(const hfg_basis_view* basis, const hfg_basis_view* aux, const dou
                int charge, const hfg_dfhf_options* opt, hfg_dfhf_result* res, double* p_out,
                double* f_out, double* j_out, double* k_out);
 
+/* Batches: J and K of many molecules in fused launches (one launch per
+ * shell-quartet class for the whole batch).  Each molecule keeps its own
+ * engine (as hfg_ctx_create); the results per molecule equal hfg_contract_jk's.
+ * No reference counterpart: the reference builds each molecule's Fock matrix
+ * on its own (src/scf.rs:366-369). */
+typedef struct hfg_batch hfg_batch;
+int hfg_batch_create(const hfg_basis_view* bases, int count, hfg_batch** out);
+void hfg_batch_destroy(hfg_batch* batch);
+/* count: molecules; n_max: largest basis; quartets: surviving shell quartets
+ * over the batch; flops: algorithmic FLOPs of one build of the batch. */
+int hfg_batch_info(const hfg_batch* batch, int64_t* count, int64_t* n_max, int64_t* quartets, double* flops);
+/* Density of molecule mol (n x n row-major), kept in HBM. */
+int hfg_batch_set_density(hfg_batch* batch, int mol, const double* p);
+/* J and K of every molecule's resident density, into HBM. */
+int hfg_batch_jk(hfg_batch* batch);
+/* The same J and K, molecule after molecule with each one's own launches
+ * (the unbatched path, for comparison). */
+int hfg_batch_jk_each(hfg_batch* batch);
+/* Copies molecule mol's J and K (either may be NULL) to the host. */
+int hfg_batch_get_jk(hfg_batch* batch, int mol, double* j, double* k);
+
 /* Host-side basis construction from Gaussian-94 text, mirroring build_basis
  * (src/basis.rs:296-496).  The arrays are owned by the returned handle. */
 typedef struct hfg_basis hfg_basis;
