@@ -53,6 +53,39 @@ def cpu_baseline(mol, text, stride, threads, density):
                                                                           threads)}
 
 
+def df_report(hf, M, steps=5):
+    """Density-fitted J/K on (H2O)32/cc-pVDZ (4x4x2 lattice, seed 4) with the
+    def2-universal-JKFIT auxiliary basis (Cartesian), 160 occupied orbitals
+    (random, fixed seed): the exchange contraction in FP64, FP32 and split
+    BF16 on the matrix cores, each with its error against FP64."""
+    mol = M.water_cluster((4, 4, 2), seed=4)
+    b = hf.Basis.from_g94(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz)
+    a = hf.Basis.from_g94(M.basis_text("def2-universal-jkfit"), mol.symbols, mol.xyz)
+    t0 = time.perf_counter()
+    df = hf.DfTensor(b, a)
+    build = time.perf_counter() - t0
+    o = 5 * 32
+    c = np.random.default_rng(7).standard_normal((b.n, o)) * 0.1
+    n, rank = b.n, df.rank
+    k_flops = 2.0 * rank * n * n * o + 2.0 * n * n * rank * o  # A = B C and K = 2 A A^T as computed
+    out = {"workload": "DF-J/K (H2O)32 cc-pVDZ / def2-universal-JKFIT, n %d, n_aux %d, rank %d, %d occupied"
+                       % (n, df.n_aux, rank, o), "build_s": build, "k_flops_per_build": k_flops}
+    j0, k0, _ = df.jk_precision(c, "fp64")
+    scale = float(np.abs(k0).max())
+    for prec, peak in (("fp64", 78.6), ("fp32", 157.3), ("bf16x3", 2500.0)):
+        ms = []
+        for _ in range(steps):
+            j, k, t = df.jk_precision(c, prec)
+            ms.append(t)
+        t = float(np.median(ms))
+        mult = 3 if prec == "bf16x3" else 1
+        out[prec] = {"ms_per_build": t, "fock_builds_per_s": 1e3 / t,
+                     "k_tflops_upper": k_flops / (t * 1e-3) / 1e12,
+                     "mfma_peak_tflops": peak, "bf16_products": mult,
+                     "max_abs_dK_rel": float(np.abs(k - k0).max()) / scale}
+    return out
+
+
 def batch_report(hf, M, count, steps=5):
     """The batched workload: `count` molecules (half all-trans alkanes
     CnH2n+2, n ~ U{1..12}, half water clusters (H2O)m, m ~ U{1..8}; PCG64 seed
@@ -99,6 +132,7 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="also time the batched multi-molecule workload (off: see DESIGN.md, batches)")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-df", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="run only the counter-calibration copy kernel over BYTES bytes and exit")
     args = ap.parse_args()
@@ -208,6 +242,8 @@ def main():
             line["scf"]["oracle_iterations"] = o["iterations"]
     if args.batch:
         line["batch"] = batch_report(hf, M, args.batch_count)
+    if not args.no_df:
+        line["df"] = df_report(hf, M)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mol, text, args.cpu_stride, args.cpu_threads, dens)
     print(json.dumps(line), flush=True)

@@ -56,7 +56,7 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_class_stats", "hfg_fock_flops", "hfg_md_flops", "hfg_persistent_blocks", "hfg_df_build",
             "hfg_df_destroy", "hfg_df_info", "hfg_df_jk", "hfg_df_rhf", "hfg_calibration_copy",
             "hfg_batch_create", "hfg_batch_destroy", "hfg_batch_info", "hfg_batch_set_density", "hfg_batch_jk",
-            "hfg_batch_get_jk", "hfg_batch_jk_each"]
+            "hfg_batch_get_jk", "hfg_batch_jk_each", "hfg_df_jk_ex"]
 
 
 class DfhfOptions(C.Structure):
@@ -107,6 +107,7 @@ def lib():
         L.hfg_df_destroy.argtypes = [vp]
         L.hfg_df_info.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3
         L.hfg_df_jk.argtypes = [vp, vp, i32, vp, vp]
+        L.hfg_df_jk_ex.argtypes = [vp, vp, i32, vp, vp, i32, C.POINTER(dbl)]
         L.hfg_df_rhf.argtypes = [C.POINTER(BasisView), C.POINTER(BasisView), vp, i32, i32, C.POINTER(DfhfOptions),
                                  C.POINTER(DfhfResult), vp, vp, vp, vp]
         L.hfg_batch_create.argtypes = [vp, i32, C.POINTER(vp)]
@@ -343,6 +344,19 @@ class DfTensor:
         k = np.zeros((self.n, self.n))
         _check(lib().hfg_df_jk(self.h, _p(cocc), cocc.shape[1], _p(j), _p(k)))
         return j, k
+
+    PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+    def jk_precision(self, cocc, precision="fp64"):
+        """J (FP64) and K with the exchange contraction in the given precision;
+        also returns the device milliseconds of the contraction."""
+        cocc = np.ascontiguousarray(cocc, dtype=np.float64)
+        j = np.zeros((self.n, self.n))
+        k = np.zeros((self.n, self.n))
+        ms = C.c_double()
+        _check(lib().hfg_df_jk_ex(self.h, _p(cocc), cocc.shape[1], _p(j), _p(k), self.PRECISION[precision],
+                                  C.byref(ms)))
+        return j, k, ms.value
 
 
 def df_rhf(basis, aux, zxyz, charge=0, max_iter=100, e_tol=1e-12, r_tol=1e-10, d_tol=1e-10, metric_cutoff=1e-10):

@@ -11,6 +11,7 @@
 // [rank][n][n], so that DF-J is two FP64 GEMVs over B (HBM-bound) and DF-K is
 // one batched GEMM A_Q = B_Q C plus one GEMM K = 2 A' A'^T over the
 // rank*n_occ index (FP64 matrix cores through rocBLAS).
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -101,6 +102,48 @@ __global__ void k_axpby_fock(const double* H, const double* J, const double* K, 
   if (i < nn) F[i] = H[i] + J[i] - 0.5 * K[i];
 }
 
+// Precision conversions for DF-K on the matrix cores.
+__global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi) (round to nearest): the
+// pair carries about 16 significant bits.
+template <class T>
+__global__ void k_split_bf16(const T* __restrict__ x, __hip_bfloat16* __restrict__ hi, __hip_bfloat16* __restrict__ lo,
+                             size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float v = (float)x[i];
+    const __hip_bfloat16 h = __float2bfloat16(v);
+    hi[i] = h;
+    lo[i] = __float2bfloat16(v - __bfloat162float(h));
+  }
+}
+__global__ void k_f32_to_f64_scaled(const float* __restrict__ x, double* __restrict__ y, size_t n, double s) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = s * (double)x[i];
+}
+// A[Q][m][i] -> At[m][Q][i] for float data.
+__global__ void k_permute_qmi_f32(const float* __restrict__ A, float* __restrict__ At, int rank, int n, int o) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t tot = (size_t)rank * n * o;
+  if (idx >= tot) return;
+  const int i = (int)(idx % o);
+  const int m = (int)((idx / o) % n);
+  const int q = (int)(idx / ((size_t)o * n));
+  At[((size_t)m * rank + q) * o + i] = A[idx];
+}
+
+// K = 2 sum_b part[b] over the split-K partial products.
+template <class T>
+__global__ void k_sum_parts(const T* __restrict__ part, int nb, size_t nn, double* __restrict__ K) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += (double)part[(size_t)b * nn + i];
+    K[i] = 2.0 * s;
+  }
+}
+
 struct DfEngine {
   hipStream_t stream = nullptr;
   std::unique_ptr<Blas> bl;
@@ -109,6 +152,17 @@ struct DfEngine {
   DevBuf d_B;  // rank x n x n
   DevBuf d_work, d_A, d_At;
   size_t work_o = 0;
+  // reduced-precision copies of B (made on first use) and work arrays
+  DevBuf d_B32, d_Bh, d_Bl, d_C32, d_Ch, d_Cl, d_A32, d_At32, d_Ah, d_Al, d_K32, d_parts;
+  size_t work_o32 = 0;
+  // Split-K of the reduced-precision K = 2 At At^T (n x n output, inner
+  // dimension rank*o): the output alone gives too few tiles for the chip, so
+  // the auxiliary index is cut into nsplit blocks computed as one strided
+  // batch, then summed in FP64.
+  int nsplit() const {
+    const int tiles = std::max(1, ((n + 127) / 128) * ((n + 127) / 128));
+    return std::max(1, std::min(rank, (1024 + tiles - 1) / tiles));
+  }
 
   DfEngine(const hfg_basis_view& basis, const hfg_basis_view& auxv, double cutoff) {
     if (!(cutoff > 0) || !std::isfinite(cutoff))
@@ -343,13 +397,106 @@ struct DfEngine {
                            (rocblas_int)nn, P, 1, &zero, dq.p, 1));
     RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_none, (rocblas_int)nn, rank, &one, d_B.as<double>(),
                            (rocblas_int)nn, dq.p, 1, &zero, dJ, 1));
+    if (!dK) return;  // J only
     // A = B C: rows (Q, m), (rank*n) x o
     bl->gemm(false, false, rank * n, o, n, 1.0, d_B.as<double>(), n, dC, o, 0.0, d_A.as<double>(), o);
     const size_t tot = (size_t)rank * n * o;
     hipLaunchKernelGGL(k_permute_qmi, dim3((tot + 255) / 256), dim3(256), 0, stream, d_A.as<double>(),
                        d_At.as<double>(), rank, n, o);
     HIP_CHECK(hipGetLastError());
+    // FP64: one GEMM (split-K measured slower here)
     bl->gemm(false, true, n, n, rank * o, 2.0, d_At.as<double>(), rank * o, d_At.as<double>(), rank * o, 0.0, dK, n);
+  }
+
+  // DF-K in reduced precision on the matrix cores (J stays FP64, HBM-bound):
+  // prec 1: FP32 operands, FP32 accumulation; prec 2: each operand split into
+  // two BF16 parts, three BF16 products (hi*hi + hi*lo + lo*hi) with FP32
+  // accumulation, for both A = B C and K = 2 At At^T.  No reference
+  // counterpart: the reference's DF-K is FP64 (src/df.rs:143-173).
+  void gemm_ex(bool ta, bool tb, int m, int nn_, int k, const void* A, int lda, const void* B, int ldb, float* C,
+               int ldc, float alpha, float beta, rocblas_datatype in) {
+    // row-major C = op(A) op(B) through column-major C^T = op(B)^T op(A)^T
+    RB_CHECK(rocblas_gemm_ex(bl->h, tb ? rocblas_operation_transpose : rocblas_operation_none,
+                             ta ? rocblas_operation_transpose : rocblas_operation_none, nn_, m, k, &alpha, B, in, ldb,
+                             A, in, lda, &beta, C, rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
+                             rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
+  }
+  void k_reduced(const double* dC, int o, double* dK, int prec) {
+    const size_t nn = (size_t)n * n, bsz = (size_t)rank * nn, csz = (size_t)n * o, asz = (size_t)rank * n * o;
+    const dim3 g(2048), b(256);
+    if (prec == 1 && !d_B32.p) {
+      d_B32.alloc(bsz * 4);
+      hipLaunchKernelGGL(k_to_f32, g, b, 0, stream, d_B.as<double>(), d_B32.as<float>(), bsz);
+    }
+    if (prec == 2 && !d_Bh.p) {
+      d_Bh.alloc(bsz * 2);
+      d_Bl.alloc(bsz * 2);
+      hipLaunchKernelGGL(k_split_bf16<double>, g, b, 0, stream, d_B.as<double>(), d_Bh.as<__hip_bfloat16>(),
+                         d_Bl.as<__hip_bfloat16>(), bsz);
+    }
+    if ((size_t)o != work_o32) {
+      d_C32.alloc(csz * 4);
+      d_Ch.alloc(csz * 2);
+      d_Cl.alloc(csz * 2);
+      d_A32.alloc(asz * 4);
+      d_At32.alloc(asz * 4);
+      d_Ah.alloc(asz * 2);
+      d_Al.alloc(asz * 2);
+      d_K32.alloc(nn * 4);
+      work_o32 = o;
+    }
+    float* A = d_A32.as<float>();
+    float* At = d_At32.as<float>();
+    float* K = d_K32.as<float>();
+    if (prec == 1) {
+      hipLaunchKernelGGL(k_to_f32, g, b, 0, stream, dC, d_C32.as<float>(), csz);
+      gemm_ex(false, false, rank * n, o, n, d_B32.p, n, d_C32.p, o, A, o, 1.f, 0.f, rocblas_datatype_f32_r);
+    } else {
+      hipLaunchKernelGGL(k_split_bf16<double>, g, b, 0, stream, dC, d_Ch.as<__hip_bfloat16>(),
+                         d_Cl.as<__hip_bfloat16>(), csz);
+      const auto bf = rocblas_datatype_bf16_r;
+      gemm_ex(false, false, rank * n, o, n, d_Bh.p, n, d_Ch.p, o, A, o, 1.f, 0.f, bf);
+      gemm_ex(false, false, rank * n, o, n, d_Bh.p, n, d_Cl.p, o, A, o, 1.f, 1.f, bf);
+      gemm_ex(false, false, rank * n, o, n, d_Bl.p, n, d_Ch.p, o, A, o, 1.f, 1.f, bf);
+    }
+    hipLaunchKernelGGL(k_permute_qmi_f32, dim3((asz + 255) / 256), dim3(256), 0, stream, A, At, rank, n, o);
+    (void)K;
+    const int nb = nsplit(), qb = (rank + nb - 1) / nb, nbe = (rank + qb - 1) / qb, full = rank / qb;
+    const int kc = qb * o, ld = rank * o;
+    d_parts.alloc((size_t)nbe * nn * 4);
+    float* parts = d_parts.as<float>();
+    // partial[b] (+)= X_b Y_b^T over aux block b, X, Y the row-major n x (rank*o)
+    // operands; column-major: partial^T = Y_b^T-ish, same as gemm_ex above.
+    auto split_gemm = [&](const void* X, const void* Y, float beta, rocblas_datatype in, size_t esz) {
+      const float one = 1.f;
+      if (full > 0)
+        RB_CHECK(rocblas_gemm_strided_batched_ex(bl->h, rocblas_operation_transpose, rocblas_operation_none, n, n, kc,
+                                                 &one, Y, in, ld, kc, X, in, ld, kc, &beta, parts,
+                                                 rocblas_datatype_f32_r, n, (rocblas_stride)nn, parts,
+                                                 rocblas_datatype_f32_r, n, (rocblas_stride)nn, full,
+                                                 rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
+      if (full < nbe) {
+        const int kt = (rank - full * qb) * o;
+        const char* Xt = (const char*)X + (size_t)full * kc * esz;
+        const char* Yt = (const char*)Y + (size_t)full * kc * esz;
+        RB_CHECK(rocblas_gemm_ex(bl->h, rocblas_operation_transpose, rocblas_operation_none, n, n, kt, &one, Yt, in,
+                                 ld, Xt, in, ld, &beta, parts + (size_t)full * nn, rocblas_datatype_f32_r, n,
+                                 parts + (size_t)full * nn, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
+                                 rocblas_gemm_algo_standard, 0, 0));
+      }
+    };
+    if (prec == 1) {
+      split_gemm(At, At, 0.f, rocblas_datatype_f32_r, 4);
+    } else {
+      hipLaunchKernelGGL(k_split_bf16<float>, g, b, 0, stream, At, d_Ah.as<__hip_bfloat16>(),
+                         d_Al.as<__hip_bfloat16>(), asz);
+      const auto bf = rocblas_datatype_bf16_r;
+      split_gemm(d_Ah.p, d_Ah.p, 0.f, bf, 2);
+      split_gemm(d_Ah.p, d_Al.p, 1.f, bf, 2);
+      split_gemm(d_Al.p, d_Ah.p, 1.f, bf, 2);
+    }
+    hipLaunchKernelGGL(k_sum_parts<float>, dim3(1024), dim3(256), 0, stream, parts, nbe, nn, dK);
+    HIP_CHECK(hipGetLastError());
   }
 };
 
@@ -572,6 +719,49 @@ int hfg_df_jk(hfg_df* df, const double* cocc, int nocc, double* j, double* k) {
     dk.alloc(nn);
     upload(dc.p, cocc, (size_t)e.n * nocc, e.stream);
     e.jk(dc.p, nocc, dj.p, dk.p);
+    download(j, dj.p, nn, e.stream);
+    download(k, dk.p, nn, e.stream);
+    return HFG_OK;
+  } catch (HfgError& ex) {
+    return dferr(ex.code, ex.msg);
+  }
+}
+
+int hfg_df_jk_ex(hfg_df* df, const double* cocc, int nocc, double* j, double* k, int precision,
+                 double* device_ms) {
+  if (!df || !cocc || !j || !k || nocc <= 0) return dferr(HFG_INVALID_INPUT, "invalid occupied coefficients for DF J/K");
+  if (precision < 0 || precision > 2) return dferr(HFG_INVALID_INPUT, "precision must be 0 (FP64), 1 (FP32) or 2 (BF16x3)");
+  try {
+    DfEngine& e = *df->e;
+    const size_t nn = (size_t)e.n * e.n;
+    for (size_t x = 0; x < (size_t)e.n * nocc; ++x)
+      if (!std::isfinite(cocc[x])) return dferr(HFG_INVALID_INPUT, "invalid occupied coefficients for DF J/K");
+    Dev dc, dj, dk;
+    dc.alloc((size_t)e.n * nocc);
+    dj.alloc(nn);
+    dk.alloc(nn);
+    upload(dc.p, cocc, (size_t)e.n * nocc, e.stream);
+    if (precision != 0) {  // first use: make the reduced copies of B outside the timed region
+      e.k_reduced(dc.p, nocc, dk.p, precision);
+      HIP_CHECK(hipStreamSynchronize(e.stream));
+    }
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, e.stream));
+    if (precision == 0) {
+      e.jk(dc.p, nocc, dj.p, dk.p);
+    } else {
+      e.jk(dc.p, nocc, dj.p, nullptr);  // J in FP64
+      e.k_reduced(dc.p, nocc, dk.p, precision);
+    }
+    HIP_CHECK(hipEventRecord(e1, e.stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (device_ms) *device_ms = ms;
     download(j, dj.p, nn, e.stream);
     download(k, dk.p, nn, e.stream);
     return HFG_OK;

@@ -157,6 +157,15 @@ int hfg_df_info(const hfg_df* df, int64_t* n, int64_t* n_aux, int64_t* rank);
 /* J and K of the RHF density P = 2 C_occ C_occ^T from occupied coefficients
  * (n x nocc).  Replaces DfTensor::coulomb_exchange_rhf (src/df.rs:133-174). */
 int hfg_df_jk(hfg_df* df, const double* cocc, int nocc, double* j, double* k);
+/* The same with the precision of the exchange contraction chosen: 0 FP64 (as
+ * hfg_df_jk), 1 FP32 operands with FP32 accumulation, 2 each operand split
+ * into two BF16 parts with three BF16 products and FP32 accumulation, both
+ * on the matrix cores; J stays FP64.  device_ms (may be NULL): device time
+ * of the J/K contraction.  No reference counterpart (the reference's DF-K is
+ * FP64, src/df.rs:143-173): its error against precision 0 is the caller's to
+ * report. */
+int hfg_df_jk_ex(hfg_df* df, const double* cocc, int nocc, double* j, double* k, int precision,
+                 double* device_ms);
 
 /* DFHFOptions (src/hf_variants/ri_df_hf.rs:21-45) and DFHFResult (:47-64). */
 typedef struct {

@@ -91,3 +91,26 @@ def test_df_rejects_bad_inputs(hf):
         hf.df_rhf(b, a, zxyz, charge=1)  # odd electron count
     with pytest.raises(hf.HfError):
         hf.df_rhf(b, a, zxyz, max_iter=1)  # cannot converge in one iteration
+
+
+def test_df_k_reduced_precision_error_bounds(hf):
+    """DF-K on the matrix cores in FP32 and in split BF16 (three products,
+    FP32 accumulation) against the FP64 contraction: J unchanged, K within
+    the stated relative bounds (FP32 1e-5, BF16x3 1e-3 of max|K|)."""
+    mol = M.water_cluster((2, 2, 1), seed=2)
+    b = hf.Basis.from_g94(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz)
+    a = hf.Basis.from_g94(M.basis_text("def2-universal-jkfit"), mol.symbols, mol.xyz)
+    df = hf.DfTensor(b, a)
+    rng = np.random.default_rng(12)
+    c = rng.standard_normal((b.n, 20)) * 0.2
+    j0, k0, _ = df.jk_precision(c, "fp64")
+    jr, kr = df.coulomb_exchange_rhf(c)
+    assert np.abs(j0 - jr).max() < 1e-13 * max(1, np.abs(jr).max()) and np.abs(k0 - kr).max() < 1e-13 * max(1, np.abs(kr).max())
+    scale = np.abs(k0).max()
+    for prec, bound in (("fp32", 1e-5), ("bf16x3", 1e-3)):
+        j, k, ms = df.jk_precision(c, prec)
+        err = np.abs(k - k0).max() / scale
+        print("DF-K %s: max|dK|/max|K| = %.2e, %.3f ms" % (prec, err, ms))
+        assert np.abs(j - j0).max() < 1e-13 * max(1, np.abs(j0).max())
+        assert err < bound
+        assert np.allclose(k, k.T, rtol=0, atol=bound * scale)
