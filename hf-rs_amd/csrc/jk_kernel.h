@@ -175,7 +175,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
 #pragma unroll
   for (int u = 0; u < RD; ++u) Do[u] = cket[u].ao_b;
   const MolCtx mc = mol_ctx(a, __builtin_amdgcn_readfirstlane(vb.mol));
-  const double* __restrict__ sc_ = mc.scale;
+  const cdptr sc_ = (cdptr)mc.scale;  // read-only in the kernel: loads survive the atomics
   // scaling
 #pragma unroll
   for (int xy = 0; xy < CA; ++xy) {
@@ -199,7 +199,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
     }
   }
 
-  const double* __restrict__ Dm = mc.D;
+  const cdptr Dm = (cdptr)mc.D;  // read-only in the kernel: loads survive the atomics
   const int n = mc.n;
   double* __restrict__ Jx = mc.Jx;
   double* __restrict__ Kx = mc.Kx;
