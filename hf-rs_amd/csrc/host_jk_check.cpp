@@ -221,7 +221,13 @@ extern "C" const char* hfgc_jk_last_error() { return g_err.c_str(); }
 // compiled ones), without evaluating it: per class, out[4c] = tasks,
 // out[4c+1] = primitive quartets of the active lanes, out[4c+2] = combination
 // blocks of the active lanes, out[4c+3] = J/K atomics.
+extern "C" int hfgc_jk_plan_work2(const hfg_basis_view* b, const int* caps, double* out, double* wave_out);
 extern "C" int hfgc_jk_plan_work(const hfg_basis_view* b, const int* caps, double* out) {
+  return hfgc_jk_plan_work2(b, caps, out, nullptr);
+}
+// As hfgc_jk_plan_work; wave_out[c] (if not null): primitive-quartet
+// iterations of the waves (every lane of a task runs its kets' count).
+extern "C" int hfgc_jk_plan_work2(const hfg_basis_view* b, const int* caps, double* out, double* wave_out) {
   try {
     std::vector<Shell> sh = group_shells(*b);
     const int n = b->n_functions;
@@ -265,13 +271,14 @@ extern "C" int hfgc_jk_plan_work(const hfg_basis_view* b, const int* caps, doubl
 #undef X
       };
       const int NA = ncart(LC[c][0]), NB = ncart(LC[c][1]), NC = ncart(LC[c][2]), ND = ncart(LC[c][3]);
-      double nt = 0, pq = 0, nb = 0, at = 0;
+      double nt = 0, pq = 0, nb = 0, at = 0, wave = 0;
       for (size_t t = plan.task_begin[c]; t < plan.task_begin[c + 1]; ++t) {
         const JKTask& task = plan.tasks[t];
         const ViewDev& vb = plan.views[task.bra];
         const KetGroup& g = plan.groups[task.grp];
         nt += 1;
         pq += (double)(vb.p1 - vb.p0) * g.npp * task.nket;
+        wave += (double)(vb.p1 - vb.p0) * g.npp;
         at += (double)vb.na * vb.nb * NA * NB;  // bra Coulomb: one per wave
         for (int l = 0; l < task.nket; ++l) {
           const ViewDev& vk = plan.views[task.ket0 + l];
@@ -288,6 +295,7 @@ extern "C" int hfgc_jk_plan_work(const hfg_basis_view* b, const int* caps, doubl
       out[4 * c + 1] = pq;
       out[4 * c + 2] = nb;
       out[4 * c + 3] = at;
+      if (wave_out) wave_out[c] = wave;
     }
     return 0;
   } catch (HfgError& e) {

@@ -275,23 +275,34 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
         }
     }
 
-  // ---- ket groups per set: (prim count), views sorted by qmax descending;
-  // heavy groups first.  Global view ids: groups are contiguous.
+  // ---- ket groups per set: views whose primitive counts fall in one bucket
+  // (steps of at most 25%), padded to the largest count of the group with
+  // zero-weight primitives, sorted by qmax descending; heavy groups first.
+  // Few, large groups keep the lanes of a task full (exact counts left most
+  // groups small and the surviving prefix of a bra short).  Global view ids:
+  // groups are contiguous.
   struct Group {
-    int set, npp;
+    int set, npp, bucket;
     std::vector<int> v;  // view indices in the set
+  };
+  auto bucket_of = [](int npp) {
+    static const int B[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 56, 64, 72, 81};
+    for (int b : B)
+      if (npp <= b) return b;
+    return npp;
   };
   std::vector<Group> groups;
   std::vector<std::vector<int>> set_groups(sets.size());
   for (size_t s = 0; s < sets.size(); ++s) {
     std::vector<Group> gl;
     for (int i = 0; i < (int)sets[s].views.size(); ++i) {
-      const int npp = (int)sets[s].views[i].prims.size();
-      auto it = std::find_if(gl.begin(), gl.end(), [&](const Group& g) { return g.npp == npp; });
+      const int npp = (int)sets[s].views[i].prims.size(), bk = bucket_of(npp);
+      auto it = std::find_if(gl.begin(), gl.end(), [&](const Group& g) { return g.bucket == bk; });
       if (it == gl.end()) {
-        gl.push_back({(int)s, npp, {}});
+        gl.push_back({(int)s, 0, bk, {}});
         it = gl.end() - 1;
       }
+      it->npp = std::max(it->npp, npp);
       it->v.push_back(i);
     }
     for (auto& g : gl)
@@ -332,8 +343,9 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
       const GenPair& P = gp[v.pair];
       ViewDev d;
       std::memset(&d, 0, sizeof d);
+      const int npv = (int)v.prims.size();  // this view's primitives; the group pads to g.npp
       d.p0 = (int)nbrec;
-      d.p1 = d.p0 + g.npp;
+      d.p1 = d.p0 + npv;
       d.na = v.na;
       d.nb = v.nb;
       d.c0 = (int)cd.size();
@@ -350,7 +362,11 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
       }
       const int coff = S.pc == 0 ? HFG_BRA_COEF_SS : HFG_BRA_COEF;
       if (cap > 1 && coff + cap > HFG_BRA_REC) throw HfgError{HFG_INVALID_INPUT, "contraction cap beyond the bra record"};
-      for (int ip = 0; ip < g.npp; ++ip) {
+      for (int ip = npv; ip < g.npp; ++ip) {  // padding: p = 1, K = 0 (contributes exactly 0)
+        kt_cols[0][kb + (size_t)ip * size + j] = 1.0;
+        kt_cols[8][kb + (size_t)ip * size + j] = 0.5;
+      }
+      for (int ip = 0; ip < npv; ++ip) {
         const int q = v.prims[ip];
         double x[9];
         for (int c = 0; c < 9; ++c) {
@@ -380,7 +396,7 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
           }
         ++nbrec;
       }
-      plan.prim_pairs_kept += g.npp;
+      plan.prim_pairs_kept += npv;
     }
     kb += (size_t)g.npp * size;
     cb += (size_t)g.npp * cap * size;
