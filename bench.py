@@ -188,13 +188,22 @@ def main():
     # The J/K kernels keep K rows in LDS (k_jk_rows) unless HFG_JK_ROWS=0.
     kname = ("k_jk_rows<%d, %d, %d, %d>" if os.environ.get("HFG_JK_ROWS", "1") != "0"
              else "k_jk<%d, %d, %d, %d>") % tuple(dom["l"])
-    traffic, traffic_src = None, None
+    traffic, traffic_src, executed = None, None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_jk_h2o16_ccpvdz.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))["per_kernel"].get(kname.replace(" ", ""))
-        if t:
+        if t and "fetch_bytes" in t:
             traffic = t["fetch_bytes"] + t["write_bytes"]
             traffic_src = "profiles/traffic_jk_h2o16_ccpvdz.json (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+        if t and "executed" in t and dom_ms > 0:
+            # FP64 work the kernel executes (SQ instruction counters x 64 lanes)
+            # over this run's launch time: the hardware utilisation, where the
+            # algorithmic figure above counts the reference's per-primitive
+            # work, which general contraction avoids in part.
+            ef = t["executed"]["f64_flops"]
+            executed = {"flops_per_launch": ef, "achieved": ef / (dom_ms * 1e-3) / 1e12,
+                        "frac": ef / (dom_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                        "source": "profiles/traffic_jk_h2o16_ccpvdz.json (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64)"}
     line = {
         "metric": "Fock builds/s (direct FP64 J+K, |dE|<=1e-8 Eh vs FP64 oracle)",
         "value": value, "unit": "Fock builds/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -208,7 +217,9 @@ def main():
         "roofline": {"bound": "fp64-valu", "kernel": kname,
                      "achieved": dom_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": dom_tflops / PEAK_FP64_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
-                     "flops_per_launch": dom["flops"], "ms_per_launch": dom_ms},
+                     "flops_per_launch": dom["flops"], "ms_per_launch": dom_ms,
+                     "flop_model": "MD per primitive quartet of each contracted quartet (the reference's loops)",
+                     "executed": executed},
         "fock_build_roofline": {"achieved": whole_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                                 "frac": whole_tflops / PEAK_FP64_TFLOPS, "algorithmic_flops": total_flops,
                                 "wall_ms": ms_step, "class_kernel_ms_summed": ms_classes},

@@ -63,7 +63,7 @@ struct Batch {
   std::vector<size_t> task_begin, job_begin;
   size_t vb_size = 0, kt_size = 0;
   int nmax = 0;
-  int side_blocks = 256;
+  int side_blocks = 512;
   double flops = 0;
   int64_t quartets = 0;
 
@@ -221,7 +221,7 @@ void Batch::jk() {
   a.boys = eng[0]->d_boys.as<double>();
   a.mols = d_mols.as<MolDev>();
   a.tasks = d_tasks.as<JKTask>();
-  auto on_side = [&](int c) { return side_blocks > 0 && (c == 12 || c == 13 || c == 14 || c >= 17); };
+  auto on_side = [&](int c) { return side_blocks > 0 && side_class(c); };
   const bool two = side_blocks > 0;
   if (two) {
     HIP_CHECK(hipEventRecord(ev_fork, stream));

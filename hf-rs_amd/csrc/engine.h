@@ -59,6 +59,10 @@ struct JKArgs {
 // Launch of class cls's J/K kernel over ntask tasks with at most max_blocks
 // workgroups; the kernels are split over three translation units.
 void launch_jk_class(int cls, const JKArgs& a, int ntask, int max_blocks, hipStream_t s);
+// Whether class cls runs on the side stream: the classes with a dp or dd bra,
+// (dp|ss) .. (dd|dd), unless HFG_SIDE_CLASSES lists class indices
+// (comma-separated; measured on (H2O)16/cc-pVDZ).
+bool side_class(int cls);
 
 // Arguments of the block kernels: one thread per task computes the unscaled
 // block of pair `bra` with pair `ket` (kernel orientation) at out + off.
@@ -108,7 +112,7 @@ struct Engine {
   // which then overlap the large low-L classes; joined before symmetrisation.
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int side_blocks = 256;  // workgroup cap of a side-stream class (HFG_SIDE_BLOCKS; 0: one stream)
+  int side_blocks = 512;  // workgroup cap of a side-stream class (HFG_SIDE_BLOCKS; 0: one stream)
   int n = 0;
   std::vector<Shell> shells;
   std::vector<PairInfo> pairs;  // hf-rs pair order

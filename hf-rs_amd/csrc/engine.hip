@@ -91,6 +91,25 @@ __global__ void k_symmetrize(const double* Jacc, const double* Kacc, double* J, 
 
 // ---------------------------------------------------------------------------
 // Host side.
+
+bool side_class(int cls) {
+  static const unsigned mask = [] {
+    unsigned m = 0;
+    for (int c = 10; c < HFG_NCLASS; ++c) m |= 1u << c;  // (dp|ss) .. (dd|dd)
+    if (const char* env = std::getenv("HFG_SIDE_CLASSES")) {
+      m = 0;
+      for (const char* p = env; *p;) {
+        char* end = nullptr;
+        const long c = std::strtol(p, &end, 10);
+        if (end == p) break;
+        if (c >= 0 && c < HFG_NCLASS) m |= 1u << c;
+        p = *end ? end + 1 : end;
+      }
+    }
+    return m;
+  }();
+  return cls >= 0 && cls < HFG_NCLASS && ((mask >> cls) & 1u);
+}
 // ---------------------------------------------------------------------------
 typedef void (*BlockLaunch)(const BlockArgs&, hipStream_t);
 
@@ -411,10 +430,9 @@ void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* d
   a.mols = nullptr;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, d_counters.as<int>(), HFG_NCLASS);
   a.tasks = d_tasks.as<JKTask>();
-  // (dp|pp), (dp|ds), (dp|dp) and (dd|pp) .. (dd|dd): few, long quartets with
-  // register spills; on the side stream, with a capped grid, they fill the
-  // room the low-L classes leave.
-  auto on_side = [&](int c) { return side_blocks > 0 && (c == 12 || c == 13 || c == 14 || c >= 17); };
+  // The high-L classes, few and long quartets, run on the side stream with a
+  // capped grid and fill the room the low-L classes leave.
+  auto on_side = [&](int c) { return side_blocks > 0 && side_class(c); };
   const bool two = side_blocks > 0;
   if (two) {
     HIP_CHECK(hipEventRecord(ev_fork, stream));

@@ -144,10 +144,12 @@ def levels(lo, hi):
     return out
 
 
-def hrr_lines(la, lb, lc, ld, acc, pre, base):
+def hrr_lines(la, lb, lc, ld, acc, pre, base, keep=None, used=None):
     """Horizontal recurrences on contracted [e0|f0] values `acc` (names keyed
     by (e, f)), writing the Cartesian block to I[base ...]; `pre` keeps the
-    temporaries of different contraction combinations apart."""
+    temporaries of different contraction combinations apart.  With `keep`,
+    only the outputs whose a and c component indices pass keep(ia, ic) are
+    formed; `used` collects the (e, f) keys they reach."""
     hl = []
     memo = {}
 
@@ -157,6 +159,8 @@ def hrr_lines(la, lb, lc, ld, acc, pre, base):
         if key in memo:
             return memo[key]
         if sum(b) == 0:
+            if used is not None:
+                used.add((a, f))
             memo[key] = acc[(a, f)]
             return memo[key]
         i = choose_axis(b, (0, 0, 0))
@@ -184,12 +188,13 @@ def hrr_lines(la, lb, lc, ld, acc, pre, base):
         return name
 
     idx = base
-    for a in cart(la):
+    for ia, a in enumerate(cart(la)):
         for b in cart(lb):
-            for c in cart(lc):
+            for ic, c in enumerate(cart(lc)):
                 for d in cart(ld):
-                    v = ket(a, b, c, d)
-                    hl.append("  I[%d] = %s;" % (idx, v))
+                    if keep is None or keep(ia, ic):
+                        v = ket(a, b, c, d)
+                        hl.append("  I[%d] = %s;" % (idx, v))
                     idx += 1
     return hl
 
@@ -289,6 +294,49 @@ def gc_caps(c):
     return GC_CAPS.get(tuple(c), (1, 1, 1, 1))
 
 
+# Output slices of the high-L classes.  A slice is the part of a contracted
+# quartet's Cartesian block with the a component (mode 1), the c component
+# (mode 2) or both (mode 3) fixed; each slice runs in its own wave with only
+# the [e0|f0] accumulators and recurrence nodes its outputs reach.  Slices
+# repeat the per-primitive setup (Boys values, prefactors) and some of the
+# vertical recurrences (x2-x10 for these classes) but hold a fraction of the
+# registers: unsliced, (dp|dp) keeps 256 contracted accumulators, (dd|dd) 961,
+# far beyond the 512 registers of a lane, and the spills bound those classes
+# by memory latency at one wave per SIMD.  Classes with many primitive
+# quartets per contracted quartet and little spilling ((pp|pp), (dp|pp),
+# (dp|ds), (dd|ps)) were measured slower sliced and stay whole.
+SLICE_MODES = {
+    (2, 1, 2, 1): 1,
+    (2, 2, 1, 1): 3,
+    (2, 2, 2, 0): 2,
+    (2, 2, 2, 1): 3,
+    (2, 2, 2, 2): 3,
+}
+
+
+def slice_mode(c):
+    return SLICE_MODES.get(tuple(c), 0)
+
+
+def slice_count(c):
+    m = slice_mode(c)
+    na, nc = ncart(c[0]), ncart(c[2])
+    return {0: 1, 1: na, 2: nc, 3: na * nc}[m]
+
+
+def slice_keep(c, S):
+    """keep(ia, ic) of slice S of class c (None: the whole block)."""
+    m = slice_mode(c)
+    nc = ncart(c[2])
+    if m == 0:
+        return None
+    if m == 1:
+        return lambda ia, ic: ia == S
+    if m == 2:
+        return lambda ia, ic: ic == S
+    return lambda ia, ic: ia * nc + ic == S
+
+
 # Bra primitive record of the J/K kernel: 16 doubles, read with wide scalar
 # loads from one pointer: p, K, Px, Py, Pz, then for pair classes beyond ss
 # PAx, PAy, PAz, 1/(2p); the contraction coefficients of the view's
@@ -298,8 +346,8 @@ BRA_COEF_SS = 5
 BRA_COEF = 9
 
 
-def gen_class_gc(la, lb, lc, ld):
-    """eri_gc<la,lb,lc,ld>: blocks of CA x CC contraction combinations of one
+def gen_class_gc(la, lb, lc, ld, S=0):
+    """eri_gc<la,lb,lc,ld,S>: slice S of the blocks of CA x CC contraction combinations of one
     bra pair and one ket pair that share their primitive pairs.
 
     Bra primitives are records of BRA_REC doubles at br (wave-uniform, read
@@ -310,7 +358,8 @@ def gen_class_gc(la, lb, lc, ld):
     combination's coefficient is folded into K as in eri_block.  Ket
     primitives are the outer loop, bra primitives the inner one: per ket
     primitive the bra sums t[gab] = sum_b c_b[gab] [e0|f0]_b are formed, then
-    a[gab][gcd] += c_k[gcd] t[gab].  Output: I[(gab * CC + gcd) * NAB * NCD + ...]."""
+    a[gab][gcd] += c_k[gcd] t[gab].  Output: I[(gab * CC + gcd) * NAB * NCD + ...],
+    only the entries of slice S (slice_keep) written."""
     RA, RB, RC, RD = gc_caps((la, lb, lc, ld))
     CA, CC = RA * RB, RC * RD
     coff = BRA_COEF_SS if la + lb == 0 else BRA_COEF
@@ -320,18 +369,22 @@ def gen_class_gc(la, lb, lc, ld):
     E = levels(la, la + lb)
     Fk = levels(lc, lc + ld)
     nab, ncd = ncart(la) * ncart(lb), ncart(lc) * ncart(ld)
+    keep = slice_keep((la, lb, lc, ld), S)
+    used = set()
+    hrr_lines(la, lb, lc, ld, {(e, f): "x" for e in E for f in Fk}, "", 0, keep, used)
+    E_f = [(e, f) for e in E for f in Fk if (e, f) in used]  # accumulators the slice reaches
     out = []
-    out.append("// (%s%s|%s%s) x %d x %d combinations" % ("spdfg"[la], "spdfg"[lb], "spdfg"[lc], "spdfg"[ld], CA, CC))
-    out.append("template <> HFG_DEV void eri_gc<%d,%d,%d,%d>(" % (la, lb, lc, ld))
+    out.append("// (%s%s|%s%s) x %d x %d combinations, slice %d of %d: %d accumulators per combination" % (
+        "spdfg"[la], "spdfg"[lb], "spdfg"[lc], "spdfg"[ld], CA, CC, S, slice_count((la, lb, lc, ld)), len(E_f)))
+    out.append("template <> HFG_DEV void eri_gc<%d,%d,%d,%d,%d>(" % (la, lb, lc, ld, S))
     out.append("    const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,")
     out.append("    const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],")
     out.append("    const double* __restrict__ boys, double* __restrict__ I) {")
     acc = {}
     for ga in range(CA):
         for gc in range(CC):
-            for e in E:
-                for f in Fk:
-                    acc[(ga, gc, e, f)] = "a%d_%d_%s_%s" % (ga, gc, tag(e), tag(f))
+            for e, f in E_f:
+                acc[(ga, gc, e, f)] = "a%d_%d_%s_%s" % (ga, gc, tag(e), tag(f))
     out.append("  double " + ", ".join("%s = 0.0" % v for v in acc.values()) + ";")
     # Ket primitives outer (one ket pair per lane; every lane runs the same
     # count, idle lanes on a valid pair, masked at digestion), bra primitives
@@ -356,9 +409,8 @@ def gen_class_gc(la, lb, lc, ld):
         out.append("    const double " + ", ".join("ck%d = kc[(jk * %d + %d) * ks]" % (g, CC, g) for g in range(CC)) + ";")
         tmp = {}
         for ga in range(CA):
-            for e in E:
-                for f in Fk:
-                    tmp[(ga, e, f)] = "t%d_%s_%s" % (ga, tag(e), tag(f))
+            for e, f in E_f:
+                tmp[(ga, e, f)] = "t%d_%s_%s" % (ga, tag(e), tag(f))
         out.append("    double " + ", ".join("%s = 0.0" % v for v in tmp.values()) + ";")
         tgt = lambda ga, e, f: tmp[(ga, e, f)]
     else:
@@ -388,27 +440,25 @@ def gen_class_gc(la, lb, lc, ld):
             out.append("      const double rq = p * inv_s, oo2s = 0.5 * inv_s;")
             out.append("      const double WQx = rq * PQx, WQy = rq * PQy, WQz = rq * PQz;")
     vrr = VRR(em)
-    for e in E:
-        for f in Fk:
-            n = vrr.node(e, f, 0)
-            for ga in range(CA):
-                if CA > 1:
-                    em.emit("      %s += cb%d * %s;" % (tgt(ga, e, f), ga, n))
-                else:
-                    em.emit("      %s += %s;" % (tgt(ga, e, f), n))
+    for e, f in E_f:
+        n = vrr.node(e, f, 0)
+        for ga in range(CA):
+            if CA > 1:
+                em.emit("      %s += cb%d * %s;" % (tgt(ga, e, f), ga, n))
+            else:
+                em.emit("      %s += %s;" % (tgt(ga, e, f), n))
     out += em.lines
     out.append("    }")
     if CC > 1:
         for ga in range(CA):
             for gc in range(CC):
-                for e in E:
-                    for f in Fk:
-                        out.append("    %s += ck%d * %s;" % (acc[(ga, gc, e, f)], gc, tmp[(ga, e, f)]))
+                for e, f in E_f:
+                    out.append("    %s += ck%d * %s;" % (acc[(ga, gc, e, f)], gc, tmp[(ga, e, f)]))
     out.append("  }")
     for ga in range(CA):
         for gc in range(CC):
-            sub_acc = {(e, f): acc[(ga, gc, e, f)] for e in E for f in Fk}
-            out += hrr_lines(la, lb, lc, ld, sub_acc, "g%d_%d_" % (ga, gc), (ga * CC + gc) * nab * ncd)
+            sub_acc = {(e, f): acc[(ga, gc, e, f)] for e, f in E_f}
+            out += hrr_lines(la, lb, lc, ld, sub_acc, "g%d_%d_" % (ga, gc), (ga * CC + gc) * nab * ncd, keep)
     out.append("}")
     return "\n".join(out) + "\n"
 
@@ -441,7 +491,8 @@ def main(outdir):
     with open(os.path.join(outdir, "eri_gc.inc"), "w") as f:
         f.write("// Generated by gen_eri.py -- do not edit.\n")
         for c in jk:
-            f.write(gen_class_gc(*c))
+            for S in range(slice_count(c)):
+                f.write(gen_class_gc(*c, S=S))
     with open(os.path.join(outdir, "eri_classes.txt"), "w") as f:
         f.write("\n".join(report) + "\n")
     # class list for the C++ side, in quartet_class() order
@@ -451,6 +502,8 @@ def main(outdir):
         f.write("#define HFG_NCLASS_ALL %d\n" % len(classes()))
         f.write("// Contraction-member caps (RA, RB, RC, RD) of the J/K classes, quartet_class() order.\n")
         f.write("#define HFG_GC_CAPS " + ", ".join("{%d, %d, %d, %d}" % gc_caps(c) for c in jk) + "\n")
+        f.write("// Output-slice mode of the J/K classes (0 whole block, 1 per a component, 2 per c, 3 per (a, c)).\n")
+        f.write("#define HFG_SLICE_MODES " + ", ".join("%d" % slice_mode(c) for c in jk) + "\n")
         f.write("// Bra primitive record of the J/K kernels: doubles per record, first coefficient (ss / other pairs).\n")
         f.write("#define HFG_BRA_REC %d\n#define HFG_BRA_COEF_SS %d\n#define HFG_BRA_COEF %d\n" % (BRA_REC, BRA_COEF_SS, BRA_COEF))
 

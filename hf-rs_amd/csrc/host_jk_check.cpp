@@ -16,7 +16,7 @@
 #include "jk_plan.h"
 
 namespace hfg {
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int S>
 HFG_DEV void eri_gc(const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,
                     const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],
                     const double* __restrict__ boys, double* __restrict__ I);
@@ -76,6 +76,15 @@ struct Replay {
   std::vector<double>& Kacc;
 };
 
+// Every output slice of a block, so that I holds the whole block.
+template <int LA, int LB, int LC, int LD, int CLS, int S = 0>
+void eri_gc_all(const double* br, int b0, int b1, const PrimPairs& kp, const double* kc, int k0, int nk, int ks,
+                const double AB[3], const double CD[3], const double* boys, double* I) {
+  eri_gc<LA, LB, LC, LD, S>(br, b0, b1, kp, kc, k0, nk, ks, AB, CD, boys, I);
+  if constexpr (S + 1 < slice_count(CLS, LA, LC))
+    eri_gc_all<LA, LB, LC, LD, CLS, S + 1>(br, b0, b1, kp, kc, k0, nk, ks, AB, CD, boys, I);
+}
+
 template <int LA, int LB, int LC, int LD, int CLS>
 void replay_class(Replay& r) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -93,8 +102,8 @@ void replay_class(Replay& r) {
       const int ket = task.ket0 + lane;
       const ViewDev& vk = P.views[ket];
       const int j = ket - g.start;
-      eri_gc<LA, LB, LC, LD>(P.vb.data(), vb.p0, vb.p1, kp, P.kcoef.data() + g.cbase + j,
-                             g.base + j, g.npp, g.size, vb.AB, vk.AB, slice<LA + LB + LC + LD>(), I.data());
+      eri_gc_all<LA, LB, LC, LD, CLS>(P.vb.data(), vb.p0, vb.p1, kp, P.kcoef.data() + g.cbase + j, g.base + j,
+                                      g.npp, g.size, vb.AB, vk.AB, slice<LA + LB + LC + LD>(), I.data());
       double w = 1.0;
       if (vb.same) w *= 0.5;
       if (vk.same) w *= 0.5;

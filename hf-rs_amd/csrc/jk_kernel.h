@@ -26,8 +26,9 @@
 
 namespace hfg {
 
-// Blocks of CA x CC contraction combinations (generated, see gen_eri.py).
-template <int LA, int LB, int LC, int LD>
+// Slice S of the blocks of CA x CC contraction combinations (generated, see
+// gen_eri.py).
+template <int LA, int LB, int LC, int LD, int S>
 HFG_DEV void eri_gc(const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,
                     const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],
                     const double* __restrict__ boys, double* __restrict__ I);
@@ -42,6 +43,35 @@ constexpr int class_index() {
   HFG_FOR_EACH_CLASS(X)
 #undef X
   return found;
+}
+
+// Output slices of a class: the Cartesian block of a contracted quartet is
+// split by its a component (mode 1), its c component (mode 2) or both (mode
+// 3), and each slice is evaluated and digested by its own wave with only the
+// recurrence terms its outputs need (high-L classes, whose whole blocks do
+// not fit the registers of a lane).  A task of such a class is run once per
+// slice; the partial sums of the slices add up to the quartet's J and K.
+template <int LA, int LB, int LC, int LD>
+constexpr int slice_count() {
+  return hfg::slice_count(class_index<LA, LB, LC, LD>(), LA, LC);
+}
+// Whether slice S holds the outputs with a component i and c component k.
+template <int LA, int LB, int LC, int LD, int S>
+constexpr bool in_slice(int i, int k) {
+  constexpr int m = SLICE_MODE[class_index<LA, LB, LC, LD>()];
+  return m == 0 || (m == 1 && i == S) || (m == 2 && k == S) || (m == 3 && i * ncart(LC) + k == S);
+}
+template <int LA, int LB, int LC, int LD, int S>
+constexpr bool slice_has_a(int i) {
+  for (int k = 0; k < ncart(LC); ++k)
+    if (in_slice<LA, LB, LC, LD, S>(i, k)) return true;
+  return false;
+}
+template <int LA, int LB, int LC, int LD, int S>
+constexpr bool slice_has_c(int k) {
+  for (int i = 0; i < ncart(LA); ++i)
+    if (in_slice<LA, LB, LC, LD, S>(i, k)) return true;
+  return false;
 }
 
 // Waves per SIMD the register allocator must leave room for in k_jk_rows:
@@ -117,7 +147,7 @@ __device__ __forceinline__ MolCtx mol_ctx(const JKArgs& a, int mol) {
 // tuple remains.
 // rows: the workgroup's LDS accumulator of the bra's K rows and bra Coulomb
 // block (see k_jk_rows), or null to update J and K in global memory only.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int S>
 __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restrict__ boys, int task, int lane,
                                         double* __restrict__ rows) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -154,7 +184,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
   const KetGroup g = a.groups[grp];
   const int j0 = ket - g.start;
   double I[CA * CC * NBLK];
-  eri_gc<LA, LB, LC, LD>(a.brec, vb.p0, vb.p1, a.kt, a.kcoef + g.cbase + j0, g.base + j0,
+  eri_gc<LA, LB, LC, LD, S>(a.brec, vb.p0, vb.p1, a.kt, a.kcoef + g.cbase + j0, g.base + j0,
                          g.npp, g.size, vb.AB, vk.AB, boys, I);  // idle lanes run a valid pair, masked below
 
   double w = 1.0;
@@ -192,9 +222,11 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
         for (int jj = 0; jj < NB; ++jj) {
           const double fab = f * sc_[Ao[x] + i] * sc_[Bo[y] + jj];
 #pragma unroll
-          for (int k = 0; k < NC; ++k)
+          for (int k = 0; k < NC; ++k) {
+            if (!in_slice<LA, LB, LC, LD, S>(i, k)) continue;
 #pragma unroll
             for (int l = 0; l < ND; ++l) blk[((i * NB + jj) * NC + k) * ND + l] *= fab * sc_[Co[z] + k] * sc_[Do[u] + l];
+          }
         }
     }
   }
@@ -215,6 +247,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
       for (int i = 0; i < NA; ++i)
 #pragma unroll
         for (int jj = 0; jj < NB; ++jj) {
+          if (!slice_has_a<LA, LB, LC, LD, S>(i)) continue;
           double s = 0.0;
 #pragma unroll
           for (int z = 0; z < RC; ++z)
@@ -222,9 +255,11 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
             for (int u = 0; u < RD; ++u) {
               const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK + (i * NB + jj) * NCD;
 #pragma unroll
-              for (int k = 0; k < NC; ++k)
+              for (int k = 0; k < NC; ++k) {
+                if (!in_slice<LA, LB, LC, LD, S>(i, k)) continue;
 #pragma unroll
                 for (int l = 0; l < ND; ++l) s += Dm[(size_t)(Co[z] + k) * n + Do[u] + l] * blk[k * ND + l];
+              }
             }
           s = wave_sum(s);
           if (lane == 0) {
@@ -246,6 +281,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
       for (int k = 0; k < NC; ++k)
 #pragma unroll
         for (int l = 0; l < ND; ++l) {
+          if (!slice_has_c<LA, LB, LC, LD, S>(k)) continue;
           double s = 0.0;
 #pragma unroll
           for (int x = 0; x < RA; ++x)
@@ -253,9 +289,11 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
             for (int y = 0; y < RB; ++y) {
               const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK + k * ND + l;
 #pragma unroll
-              for (int i = 0; i < NA; ++i)
+              for (int i = 0; i < NA; ++i) {
+                if (!in_slice<LA, LB, LC, LD, S>(i, k)) continue;
 #pragma unroll
                 for (int jj = 0; jj < NB; ++jj) s += Dm[(size_t)(Ao[x] + i) * n + Bo[y] + jj] * blk[(i * NB + jj) * NCD];
+              }
             }
           add_local(&Jx[(size_t)(Co[z] + k) * n + Do[u] + l], 2.0 * s);
         }
@@ -270,6 +308,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
       for (int i = 0; i < NA; ++i)
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
+          if (!in_slice<LA, LB, LC, LD, S>(i, k)) continue;
           double s = 0.0;
 #pragma unroll
           for (int y = 0; y < RB; ++y)
@@ -298,6 +337,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
       for (int jj = 0; jj < NB; ++jj)
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
+          if (!slice_has_c<LA, LB, LC, LD, S>(k)) continue;
           double s = 0.0;
 #pragma unroll
           for (int x = 0; x < RA; ++x)
@@ -305,10 +345,12 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
             for (int u = 0; u < RD; ++u) {
               const double* blk = I + ((x * RB + y) * CC + z * RD + u) * NBLK;
 #pragma unroll
-              for (int i = 0; i < NA; ++i)
+              for (int i = 0; i < NA; ++i) {
+                if (!in_slice<LA, LB, LC, LD, S>(i, k)) continue;
 #pragma unroll
                 for (int l = 0; l < ND; ++l)
                   s += Dm[(size_t)(Ao[x] + i) * n + Do[u] + l] * blk[((i * NB + jj) * NC + k) * ND + l];
+              }
             }
           if (rows)
             lds_add(&rows[(size_t)(RA * NA + y * NB + jj) * n + Co[z] + k], s);
@@ -326,6 +368,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
       for (int i = 0; i < NA; ++i)
 #pragma unroll
         for (int l = 0; l < ND; ++l) {
+          if (!slice_has_a<LA, LB, LC, LD, S>(i)) continue;
           double s = 0.0;
 #pragma unroll
           for (int y = 0; y < RB; ++y)
@@ -335,8 +378,10 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
 #pragma unroll
               for (int jj = 0; jj < NB; ++jj)
 #pragma unroll
-                for (int k = 0; k < NC; ++k)
+                for (int k = 0; k < NC; ++k) {
+                  if (!in_slice<LA, LB, LC, LD, S>(i, k)) continue;
                   s += Dm[(size_t)(Bo[y] + jj) * n + Co[z] + k] * blk[((i * NB + jj) * NC + k) * ND + l];
+                }
             }
           if (rows)
             lds_add(&rows[(size_t)(x * NA + i) * n + Do[u] + l], s);
@@ -363,8 +408,10 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
 #pragma unroll
               for (int i = 0; i < NA; ++i)
 #pragma unroll
-                for (int k = 0; k < NC; ++k)
+                for (int k = 0; k < NC; ++k) {
+                  if (!in_slice<LA, LB, LC, LD, S>(i, k)) continue;
                   s += Dm[(size_t)(Ao[x] + i) * n + Co[z] + k] * blk[((i * NB + jj) * NC + k) * ND + l];
+                }
             }
           if (rows)
             lds_add(&rows[(size_t)(RA * NA + y * NB + jj) * n + Do[u] + l], s);
@@ -372,6 +419,19 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
             add_local(&Kx[(size_t)(Bo[y] + jj) * n + Do[u] + l], s);
         }
     }
+}
+
+// Slice s (wave-uniform) of a task.
+template <int LA, int LB, int LC, int LD, int S = 0>
+__device__ __forceinline__ void jk_task_slice(const JKArgs& a, const double* __restrict__ boys, int task, int s,
+                                              int lane, double* __restrict__ rows) {
+  if constexpr (S + 1 < slice_count<LA, LB, LC, LD>()) {
+    if (s != S) {
+      jk_task_slice<LA, LB, LC, LD, S + 1>(a, boys, task, s, lane, rows);
+      return;
+    }
+  }
+  jk_task<LA, LB, LC, LD, S>(a, boys, task, lane, rows);
 }
 
 // Persistent waves: every wave walks the class's task list with a stride of
@@ -382,10 +442,13 @@ template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
   __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
   load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
+  constexpr int NS = slice_count<LA, LB, LC, LD>();
   const int lane = threadIdx.x & 63;
   const int waves = (int)(gridDim.x * (blockDim.x >> 6));
-  for (int task = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); task < a.ntask; task += waves)
-    jk_task<LA, LB, LC, LD>(a, boys, task, lane, nullptr);
+  for (int v = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); v < a.ntask * NS; v += waves) {
+    const int task = __builtin_amdgcn_readfirstlane(v / NS);
+    jk_task_slice<LA, LB, LC, LD>(a, boys, task, __builtin_amdgcn_readfirstlane(v - task * NS), lane, nullptr);
+  }
 }
 
 // Persistent workgroups over jobs (tasks of one bra view), taken from a
@@ -400,6 +463,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
   constexpr int CLS = class_index<LA, LB, LC, LD>();
   constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1], CA = RA * RB;
   constexpr int NR = RA * NA + RB * NB;
+  constexpr int NS = slice_count<LA, LB, LC, LD>();
   __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
   __shared__ int job_sh;
   extern __shared__ double rows[];
@@ -414,7 +478,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     __syncthreads();
     if (job >= a.njob) break;
     const int t0 = a.jobs[job], t1 = a.jobs[job + 1];
-    for (int task = t0 + wave; task < t1; task += nwave) jk_task<LA, LB, LC, LD>(a, boys, task, lane, rows);
+    for (int v = t0 * NS + wave; v < t1 * NS; v += nwave) {
+      const int task = __builtin_amdgcn_readfirstlane(v / NS);
+      jk_task_slice<LA, LB, LC, LD>(a, boys, task, __builtin_amdgcn_readfirstlane(v - task * NS), lane, rows);
+    }
     __syncthreads();
     const ViewDev& vb = a.views[a.tasks[t0].bra];
     const ComboDev* __restrict__ cbra = a.combos + vb.c0;

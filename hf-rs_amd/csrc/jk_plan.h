@@ -63,6 +63,16 @@ struct BlockTask {
 // Contraction-member caps (RA, RB, RC, RD) of the J/K classes.
 constexpr int GC_CAPS[HFG_NCLASS][4] = {HFG_GC_CAPS};
 
+// Output-slice mode of the J/K classes: the Cartesian block of a quartet is
+// evaluated whole (0) or in slices of one a component (1), one c component
+// (2) or one (a, c) pair (3); slice_count is the number of slices of a class
+// with bra/ket first shells of angular momentum la and lc.
+constexpr int SLICE_MODE[HFG_NCLASS] = {HFG_SLICE_MODES};
+constexpr int slice_count(int cls, int la, int lc) {
+  const int na = (la + 1) * (la + 2) / 2, nc = (lc + 1) * (lc + 2) / 2;
+  return SLICE_MODE[cls] == 0 ? 1 : SLICE_MODE[cls] == 1 ? na : SLICE_MODE[cls] == 2 ? nc : na * nc;
+}
+
 // Everything the J/K kernels read besides the density, built on the host.
 struct JKPlan {
   std::vector<ViewDev> views;
