@@ -80,8 +80,45 @@ HFG_DEV void boys_slice_row(const double* __restrict__ tab, int r, double* __res
   }
 }
 
-template <int M>
+template <int M, bool SELECT = false>
 HFG_DEV void boys_eval(double T, const double* __restrict__ slice, double* __restrict__ F) {
+  if constexpr (SELECT && M <= 4) {
+    // Both branches below, evaluated without control flow and selected per
+    // lane: the loop body stays one basic block, so that the scheduler can
+    // overlap the table reads of consecutive primitives (used where the bra
+    // loop is unrolled; it costs registers elsewhere).
+    const bool near = T < BOYS_TMAX;
+    const double Tn = near ? T : 0.0;
+    const int r = (int)(Tn * BOYS_INV_STEP + 0.5);
+    const double x = r * (1.0 / BOYS_INV_STEP) - Tn;
+    const double* row = slice + r * BOYS_SLICE;
+    double acc = row[BOYS_TAYLOR];
+#pragma unroll
+    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = fma(acc, x, row[k]);
+    double Fn[M + 1];
+    Fn[M] = acc;
+    if (M > 0) {
+      constexpr double c[10] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040,
+                                1.0 / 40320, 1.0 / 362880};
+      double e = c[9];
+#pragma unroll
+      for (int k = 8; k >= 0; --k) e = fma(e, x, c[k]);
+      const double emt = row[0] * e;
+      const double t2 = 2.0 * Tn;
+#pragma unroll
+      for (int m = M - 1; m >= 0; --m) Fn[m] = (t2 * Fn[m + 1] + emt) * (1.0 / (2 * m + 1));
+    }
+    const double rt = rsqrt_nr(near ? 1.0 : T);
+    double Ff = HALF_SQRT_PI * rt;
+    const double h = 0.5 * rt * rt;
+    F[0] = near ? Fn[0] : Ff;
+#pragma unroll
+    for (int m = 1; m <= M; ++m) {
+      Ff = (2 * m - 1) * Ff * h;
+      F[m] = near ? Fn[m] : Ff;
+    }
+    return;
+  }
   if (T < BOYS_TMAX) {
     const int r = (int)(T * BOYS_INV_STEP + 0.5);
     const double x = r * (1.0 / BOYS_INV_STEP) - T;  // -dT, |dT| <= 1/16

@@ -314,6 +314,13 @@ SLICE_MODES = {
 }
 
 
+# Classes whose bra-primitive loop is unrolled by two: two independent Boys
+# evaluations per iteration let the scheduler overlap the table reads of one
+# primitive with the arithmetic of the other (at one or two waves per SIMD
+# the low-L classes otherwise wait on those reads most of the time).
+UNROLL_BRA = {(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0)}
+
+
 def slice_mode(c):
     return SLICE_MODES.get(tuple(c), 0)
 
@@ -417,6 +424,8 @@ def gen_class_gc(la, lb, lc, ld, S=0):
         tgt = lambda ga, e, f: acc[(ga, 0, e, f)]
     out.append("    const cdptr rb = rec0 + (size_t)b0 * %d;" % BRA_REC)
     out.append("    double " + ", ".join(bra_loads("rb", "n_")) + ";")
+    if (la, lb, lc, ld) in UNROLL_BRA:
+        out.append("#pragma unroll 2")
     out.append("    for (int ib = b0; ib < b1; ++ib) {")
     names = [v for v, k in bra_fields]
     out.append("      const double " + ", ".join("%s = n_%s" % (v, v) for v in names) + ";")
@@ -428,7 +437,7 @@ def gen_class_gc(la, lb, lc, ld, S=0):
     out.append("      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
     out.append("      const double pref = TWO_PI_2_5 * Kp * Kq * rs;")
     out.append("      double Fm[%d];" % (M + 1))
-    out.append("      boys_eval<%d>(T, boys, Fm);" % M)
+    out.append("      boys_eval<%d, %s>(T, boys, Fm);" % (M, "true" if (la, lb, lc, ld) in UNROLL_BRA else "false"))
     for m in range(M + 1):
         out.append("      const double F%d = pref * Fm[%d];" % (m, m))
     out.append("      " + "; ".join(bra_loads("rn", "n_")) + ";")
