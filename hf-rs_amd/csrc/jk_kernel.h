@@ -75,13 +75,14 @@ constexpr bool slice_has_c(int k) {
 }
 
 // Waves per SIMD the register allocator must leave room for in k_jk_rows:
-// two for (ss|ss), (ps|ss) and (pp|ss), whose few spills cost less than the
-// latency a second wave hides; one (no constraint) elsewhere (measured on
-// (H2O)16/cc-pVDZ).
+// two for (ss|ss), (ps|ss), (ps|ps) and (pp|ss), whose few spills cost less
+// than the latency a second wave hides; one (no constraint) elsewhere.
+// Measured on (H2O)16/cc-pVDZ: three waves for (ss|ss) and (ps|ss) spill
+// far more and run 1.5-3.7x slower; two for (pp|ps) cost 15%.
 template <int LA, int LB, int LC, int LD>
 constexpr int jk_min_waves() {
   constexpr int c = class_index<LA, LB, LC, LD>();
-  return (c == 0 || c == 1 || c == 3) ? 2 : 1;
+  return (c == 0 || c == 1 || c == 2 || c == 3) ? 2 : 1;
 }
 
 // J/K partial sums live in one copy per XCC (accelerator complex die, each
