@@ -36,6 +36,85 @@ __global__ inline void k_add(const double* a, const double* b, double* out, size
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] + b[i];
 }
+// Second-order spectral projection (SP2, trace-correcting purification;
+// A. M. N. Niklasson, Phys. Rev. B 66, 155115 (2002)) of a symmetric m x m
+// matrix H onto its nocc lowest eigenvectors, in three kernels per step
+// around one GEMM X2 = X X.
+//
+// Gershgorin bounds of H: b[0] = min_i (H_ii - sum_{j!=i} |H_ij|),
+// b[1] = max_i (H_ii + sum_{j!=i} |H_ij|).  One workgroup of 256 threads.
+__global__ inline void k_gershgorin(const double* __restrict__ h, int m, double* __restrict__ b) {
+  __shared__ double lo[256], hi[256];
+  double l = 1e300, u = -1e300;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    double r = 0.0;
+    for (int j = 0; j < m; ++j)
+      if (j != i) r += fabs(h[(size_t)i * m + j]);
+    l = fmin(l, h[(size_t)i * m + i] - r);
+    u = fmax(u, h[(size_t)i * m + i] + r);
+  }
+  lo[threadIdx.x] = l;
+  hi[threadIdx.x] = u;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + s]);
+      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    b[0] = lo[0];
+    b[1] = hi[0];
+  }
+}
+// X = (emax I - H) / (emax - emin): the spectrum mapped into [0, 1], lowest
+// energies nearest 1.
+__global__ inline void k_sp2_init(const double* __restrict__ h, const double* __restrict__ b, double* __restrict__ x,
+                                  int m) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)m * m) return;
+  const double emin = b[0], emax = b[1];
+  const double s = 1.0 / fmax(emax - emin, 1e-300);
+  const size_t r = i / m, c = i % m;
+  x[i] = ((r == c ? emax : 0.0) - h[i]) * s;
+}
+// t[0] = tr X, t[1] = tr X2; err[k] = tr X - tr X2 (= sum lambda (1 - lambda)).
+// One workgroup of 256 threads.
+__global__ inline void k_sp2_traces(const double* __restrict__ x, const double* __restrict__ x2, int m,
+                                    double* __restrict__ t, double* __restrict__ err, int k) {
+  __shared__ double a[256], q[256];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    s0 += x[(size_t)i * m + i];
+    s1 += x2[(size_t)i * m + i];
+  }
+  a[threadIdx.x] = s0;
+  q[threadIdx.x] = s1;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      a[threadIdx.x] += a[threadIdx.x + s];
+      q[threadIdx.x] += q[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    t[0] = a[0];
+    t[1] = q[0];
+    err[k] = a[0] - q[0];
+  }
+}
+// X <- X2 when that brings the trace nearer nocc, else X <- 2X - X2.
+__global__ inline void k_sp2_update(double* __restrict__ x, const double* __restrict__ x2,
+                                    const double* __restrict__ t, int m, double nocc) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)m * m) return;
+  const double t0 = t[0], t1 = t[1];
+  const bool square = fabs(t1 - nocc) < fabs(2.0 * t0 - t1 - nocc);
+  x[i] = square ? x2[i] : 2.0 * x[i] - x2[i];
+}
+
 __global__ inline void k_zero(double* a, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.0;

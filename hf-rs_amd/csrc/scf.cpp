@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -491,11 +492,57 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
     ++builds;
     ++since_full;
   };
+  // Inside the iterations the density needs only the projector onto the nocc
+  // lowest eigenvectors of F' = X^T F X, not the eigenpairs: it comes from
+  // SP2 purification (one m x m GEMM per step, about 20-30 steps), which on
+  // the GPU costs a fraction of the tridiagonalisation-bound dsyevd.  The
+  // steps stop once tr(X - X^2) < 1e-9, then three more (quadratic
+  // convergence takes the error to rounding); without convergence in 100
+  // steps (a vanishing HOMO-LUMO gap) the eigensolver is used.  The final
+  // density, the orbital energies and the atomic SAD solves always use the
+  // eigensolver.  HFG_SCF_EIGH=1 uses it everywhere.
+  const bool use_sp2 = std::getenv("HFG_SCF_EIGH") == nullptr && nocc > 0 && nocc < m;
+  Dev dSx, dSb, dSt, dSerr;
+  const int SP2_MAX = 100;
+  if (use_sp2) {
+    dSx.alloc((size_t)m * m);
+    dSb.alloc(2);
+    dSt.alloc(2);
+    dSerr.alloc(SP2_MAX);
+  }
+  auto sp2_density = [&](double* P) {  // F' in dTmp2 (m x m); false: not converged
+    const size_t mm = (size_t)m * m;
+    hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(256), 0, st, dTmp2.p, m, dSb.p);
+    hipLaunchKernelGGL(k_sp2_init, grid(mm), dim3(tb), 0, st, dTmp2.p, dSb.p, dSx.p, m);
+    int extra = -1;
+    for (int k = 0; k < SP2_MAX; ++k) {
+      bl.gemm(false, false, m, m, m, 1.0, dSx.p, m, dSx.p, m, 0.0, dTmp2.p, m);  // X^2
+      hipLaunchKernelGGL(k_sp2_traces, dim3(1), dim3(256), 0, st, dSx.p, dTmp2.p, m, dSt.p, dSerr.p, k);
+      hipLaunchKernelGGL(k_sp2_update, grid(mm), dim3(tb), 0, st, dSx.p, dTmp2.p, dSt.p, m, (double)nocc);
+      if (extra > 0 && --extra == 0) {
+        // P = 2 X D X^T
+        bl.gemm(false, false, n, m, m, 1.0, dX.p, m, dSx.p, m, 0.0, dTmp.p, m);
+        bl.gemm(false, true, n, n, m, 2.0, dTmp.p, m, dX.p, m, 0.0, P, n);
+        return true;
+      }
+      if (extra < 0 && k >= 11 && (k + 1) % 4 == 0) {
+        double e = 0.0;
+        download(&e, dSerr.p + k, 1, st);
+        if (std::fabs(e) < 1e-9) extra = 3;
+      }
+    }
+    return false;
+  };
   // diagonalise X^T F X, C = X C', P = 2 C_occ C_occ^T; eigenvalues into eps
   std::vector<double> eps(m);
   auto diag_density = [&](const double* F, double* P, bool want_eps) {
     bl.gemm(false, false, n, m, n, 1.0, F, n, dX.p, m, 0.0, dTmp.p, m);     // F X
     bl.gemm(true, false, m, m, n, 1.0, dX.p, m, dTmp.p, m, 0.0, dTmp2.p, m);  // X^T F X
+    if (use_sp2 && !want_eps) {
+      HIP_CHECK(hipMemcpyAsync(dC.p, dTmp2.p, (size_t)m * m * 8, hipMemcpyDeviceToDevice, st));
+      if (sp2_density(P)) return;
+      HIP_CHECK(hipMemcpyAsync(dTmp2.p, dC.p, (size_t)m * m * 8, hipMemcpyDeviceToDevice, st));
+    }
     eig.run(bl, dTmp2.p, dW.p, m);
     // C = X V where the row-major view of dTmp2 is V^T
     bl.gemm(false, true, n, m, m, 1.0, dX.p, m, dTmp2.p, m, 0.0, dC.p, m);
@@ -635,11 +682,13 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
       HIP_CHECK(hipMemcpyAsync(aP.p + s * nn, dP.p, nn * 8, hipMemcpyDeviceToDevice, st));
       const int k = (int)aslot.size();
       if (k >= 2) {
-        // G[s][t] = P_s . F_t over all slots; the column-major MAXS x MAXS
-        // result of P^T F is G^T in row-major terms.
+        // G[s][t] = P_s . F_t over all slots, stored column-major (dG[t * MAXS + s]):
+        // one transposed GEMV per F_t.  (As one MAXS x MAXS GEMM over the n^2
+        // inner dimension rocBLAS used a single workgroup, 25 ms at n = 400.)
         const double one = 1.0, zero = 0.0;
-        RB_CHECK(rocblas_dgemm(bl.h, rocblas_operation_transpose, rocblas_operation_none, MAXS, MAXS, (rocblas_int)nn,
-                               &one, aP.p, (rocblas_int)nn, aF.p, (rocblas_int)nn, &zero, dG.p, MAXS));
+        for (int t = 0; t < MAXS; ++t)
+          RB_CHECK(rocblas_dgemv(bl.h, rocblas_operation_transpose, (rocblas_int)nn, MAXS, &one, aP.p, (rocblas_int)nn,
+                                 aF.p + (size_t)t * nn, 1, &zero, dG.p + t * MAXS, 1));
         std::vector<double> G(MAXS * MAXS);
         download(G.data(), dG.p, MAXS * MAXS, st);
         auto g = [&](int i, int j) { return G[(size_t)aslot[j] * MAXS + aslot[i]]; };  // P_i . F_j
