@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -63,6 +64,26 @@ __global__ void k_scatter3(const Scatter3* __restrict__ d, int ntask, const doub
     T[(size_t)P * nn + (size_t)mu * n + nu] = v;
     T[(size_t)P * nn + (size_t)nu * n + mu] = v;
   }
+}
+
+// Tasks i < cnt of one (orbital pair class, auxiliary class) product,
+// t = t0 + i = (orbital pair io, auxiliary shell ix) with ix fastest: the
+// block task (kernel orientation) and its scatter record, at output offset
+// i * NT.  Generated on the device: the product has ~1e8 entries for
+// (H2O)32/cc-pVDZ with def2-universal-JKFIT, far too many to build and sort
+// on the host.
+__global__ void k_expand3(const int* __restrict__ opair, const int* __restrict__ oao_a, const int* __restrict__ oao_b,
+                          const int* __restrict__ xsh, const int* __restrict__ xao, int nx, long long t0, int cnt,
+                          int NT, int aux_bra, int norb_pairs, int na, int nb, int np, BlockTask* __restrict__ tasks,
+                          Scatter3* __restrict__ sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const long long t = t0 + i;
+  const int io = (int)(t / nx), ix = (int)(t - (long long)io * nx);
+  const int op = opair[io], xp = norb_pairs + xsh[ix];
+  const long long off = (long long)i * NT;
+  tasks[i] = BlockTask{aux_bra ? xp : op, aux_bra ? op : xp, off};
+  sc[i] = Scatter3{off, aux_bra, na, nb, np, oao_a[io], oao_b[io], xao[ix]};
 }
 
 // Metric block (P 1|Q 1) -> V[P][Q] and V[Q][P].
@@ -336,39 +357,73 @@ struct DfEngine {
     HIP_CHECK(hipMemcpyAsync(d_Xt.p, Xt.data(), Xt.size() * 8, hipMemcpyHostToDevice, stream));
     d_V.free();
 
-    // 3-index tensor T[P][mu][nu].
+    // 3-index tensor T[P][mu][nu]: every orbital pair against every
+    // auxiliary shell (src/df.rs:60-78), one (pair class, auxiliary class)
+    // product at a time, in batches of at most `budget` block doubles.
     DevBuf d_T;
     const size_t nn = (size_t)n * n;
     d_T.alloc((size_t)naux * nn * 8);
     {
-      std::vector<BlockTask> tasks;
-      std::vector<int> cls;
-      std::vector<Scatter3> s3;
-      for (int i = 0; i < norb_pairs; ++i) {
-        const PairInfo& o = pairs[i];
-        const int pco = pair_class(o.la, o.lb);
-        for (int p = 0; p < (int)aux.size(); ++p) {
-          const int pcx = pair_class(aux[p].L, 0);
+      std::map<int, std::vector<int>> by_oc, by_xc;  // class -> orbital pairs / auxiliary shells
+      for (int i = 0; i < norb_pairs; ++i) by_oc[pair_class(pairs[i].la, pairs[i].lb)].push_back(i);
+      for (int p = 0; p < (int)aux.size(); ++p) by_xc[pair_class(aux[p].L, 0)].push_back(p);
+      const long long budget = 1ll << 26;
+      DevBuf d_opair, d_oa, d_ob, d_xsh, d_xao, dt, ds3, dout;
+      size_t cap_tasks = 0, cap_out = 0;
+      for (auto& oc : by_oc) {
+        const std::vector<int>& ol = oc.second;
+        std::vector<int> oa(ol.size()), ob(ol.size());
+        for (size_t k = 0; k < ol.size(); ++k) {
+          oa[k] = pairs[ol[k]].ao_a;
+          ob[k] = pairs[ol[k]].ao_b;
+        }
+        d_opair.alloc(ol.size() * sizeof(int));
+        d_oa.alloc(ol.size() * sizeof(int));
+        d_ob.alloc(ol.size() * sizeof(int));
+        HIP_CHECK(hipMemcpyAsync(d_opair.p, ol.data(), ol.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d_oa.p, oa.data(), ol.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d_ob.p, ob.data(), ol.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        const PairInfo& o0 = pairs[ol[0]];
+        for (auto& xc : by_xc) {
+          const std::vector<int>& xl = xc.second;
+          std::vector<int> xao(xl.size());
+          for (size_t k = 0; k < xl.size(); ++k) xao[k] = aux[xl[k]].ao;
+          d_xsh.alloc(xl.size() * sizeof(int));
+          d_xao.alloc(xl.size() * sizeof(int));
+          HIP_CHECK(hipMemcpyAsync(d_xsh.p, xl.data(), xl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+          HIP_CHECK(hipMemcpyAsync(d_xao.p, xao.data(), xl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+          const int pco = oc.first, pcx = xc.first;
           const bool aux_bra = pcx > pco;
-          const int na = nc(o.la), nb = nc(o.lb), nx = nc(aux[p].L);
-          tasks.push_back({aux_bra ? norb_pairs + p : i, aux_bra ? i : norb_pairs + p, (long long)na * nb * nx});
-          cls.push_back(aux_bra ? quartet_class(pcx, pco) : quartet_class(pco, pcx));
-          s3.push_back({0, aux_bra ? 1 : 0, na, nb, nx, o.ao_a, o.ao_b, aux[p].ao});
+          const int cls = aux_bra ? quartet_class(pcx, pco) : quartet_class(pco, pcx);
+          const int na = nc(o0.la), nb = nc(o0.lb), np_ = nc(aux[xl[0]].L), NT = na * nb * np_;
+          const long long total = (long long)ol.size() * (long long)xl.size();
+          const long long chunk = std::max<long long>(1, std::min<long long>(budget / NT, 1ll << 24));
+          for (long long t0 = 0; t0 < total; t0 += chunk) {
+            const int cnt = (int)std::min(chunk, total - t0);
+            if ((size_t)cnt > cap_tasks) {
+              dt.alloc((size_t)cnt * sizeof(BlockTask));
+              ds3.alloc((size_t)cnt * sizeof(Scatter3));
+              cap_tasks = cnt;
+            }
+            if ((size_t)cnt * NT > cap_out) {
+              dout.alloc((size_t)cnt * NT * 8);
+              cap_out = (size_t)cnt * NT;
+            }
+            hipLaunchKernelGGL(k_expand3, dim3((cnt + 255) / 256), dim3(256), 0, stream, d_opair.as<int>(),
+                               d_oa.as<int>(), d_ob.as<int>(), d_xsh.as<int>(), d_xao.as<int>(), (int)xl.size(), t0,
+                               cnt, NT, aux_bra ? 1 : 0, norb_pairs, na, nb, np_, dt.as<BlockTask>(),
+                               ds3.as<Scatter3>());
+            HIP_CHECK(hipGetLastError());
+            BlockArgs args{dt.as<BlockTask>(), cnt, d_pairs.as<PairDev>(), view, d_boys.as<double>(), dout.as<double>()};
+            launch_block_class(cls, args, stream);
+            hipLaunchKernelGGL(k_scatter3, dim3(cnt), dim3(64), 0, stream, ds3.as<Scatter3>(), cnt, dout.as<double>(),
+                               d_so.as<double>(), d_sx.as<double>(), d_T.as<double>(), n);
+            HIP_CHECK(hipGetLastError());
+          }
+          // the lists are rewritten for the next product
+          HIP_CHECK(hipStreamSynchronize(stream));
         }
       }
-      DevBuf ds3;
-      run_blocks(tasks, cls, [&](std::vector<BlockTask>& bt, std::vector<size_t>& which, double* blk) {
-        std::vector<Scatter3> h(bt.size());
-        for (size_t i = 0; i < bt.size(); ++i) {
-          h[i] = s3[which[i]];
-          h[i].off = bt[i].off;
-        }
-        ds3.alloc(h.size() * sizeof(Scatter3));
-        HIP_CHECK(hipMemcpyAsync(ds3.p, h.data(), h.size() * sizeof(Scatter3), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_scatter3, dim3(h.size()), dim3(64), 0, stream, ds3.as<Scatter3>(), (int)h.size(), blk,
-                           d_so.as<double>(), d_sx.as<double>(), d_T.as<double>(), n);
-        HIP_CHECK(hipGetLastError());
-      });
     }
     // B = X^T T  (rank x naux) (naux x n^2)
     d_B.alloc((size_t)rank * nn * 8);
