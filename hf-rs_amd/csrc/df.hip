@@ -647,7 +647,18 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
   auto energy = [&](const double* P, const double* F) { return 0.5 * (bl.dot(nn, P, dH.p) + bl.dot(nn, P, F)); };
   diag(dH.p, dC.p, dP.p);
   occ_copy(dC.p);
-  std::vector<std::vector<double>> fh, eh;  // CDIIS on the host (m x m errors, n x n Focks)
+  // CDIIS histories on the device (8 slots of n x n Focks and m x m
+  // errors); the error inner products are kept per slot pair, so each
+  // iteration adds at most 8 dot products; the (k+1)^2 system is solved on
+  // the host.
+  const int MAXS = 8;
+  Dev hF, hE, dCoef;
+  hF.alloc(MAXS * nn);
+  hE.alloc((size_t)MAXS * (size_t)m * m);
+  dCoef.alloc(MAXS);
+  hipLaunchKernelGGL(k_zero, grid(MAXS * nn), dim3(tb), 0, st, hF.p, MAXS * nn);
+  std::vector<int> order;  // logical order (oldest first) -> slot
+  std::map<std::pair<int, int>, double> edot;
   double last = INFINITY;
   const size_t mm2 = (size_t)m * m;
   for (int it = 1; it <= opt->max_iter; ++it) {
@@ -660,8 +671,15 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
     bl.gemm(false, false, n, m, n, 1.0, dErr.p, n, dX.p, m, 0.0, dTmp.p, m);
     bl.gemm(true, false, m, m, n, 1.0, dX.p, m, dTmp.p, m, 0.0, dTmp2.p, m);
     const double resid = bl.amax(mm2, dTmp2.p);
-    std::vector<double> err(mm2);
-    download(err.data(), dTmp2.p, mm2, st);
+    // this iteration's error, kept before diag() reuses dTmp2
+    int slot;
+    if ((int)order.size() >= MAXS) {
+      slot = order.front();
+      order.erase(order.begin());
+    } else {
+      slot = (int)order.size();
+    }
+    HIP_CHECK(hipMemcpyAsync(hE.p + (size_t)slot * mm2, dTmp2.p, mm2 * 8, hipMemcpyDeviceToDevice, st));
     diag(dF.p, dCn.p, dPn.p);
     hipLaunchKernelGGL(k_sub, grid(nn), dim3(tb), 0, st, dPn.p, dP.p, dTmp.p, nn);
     const double dch = bl.amax(nn, dTmp.p);
@@ -683,32 +701,32 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
       return HFG_OK;
     }
     last = ce;
-    std::vector<double> F(nn);
-    download(F.data(), dF.p, nn, st);
-    if (fh.size() >= 8) {
-      fh.erase(fh.begin());
-      eh.erase(eh.begin());
-    }
-    fh.push_back(F);
-    eh.push_back(err);
+    HIP_CHECK(hipMemcpyAsync(hF.p + (size_t)slot * nn, dF.p, nn * 8, hipMemcpyDeviceToDevice, st));
+    for (auto e = edot.begin(); e != edot.end();)
+      if (e->first.first == slot || e->first.second == slot)
+        e = edot.erase(e);
+      else
+        ++e;
+    order.push_back(slot);
+    for (int o : order)
+      edot[{std::min(slot, o), std::max(slot, o)}] = bl.dot(mm2, hE.p + (size_t)slot * mm2, hE.p + (size_t)o * mm2);
     const double* fuse = dF.p;
-    if (it >= 2 && eh.size() >= 2) {
-      const int k = (int)eh.size();
+    if (it >= 2 && order.size() >= 2) {
+      const int k = (int)order.size();
       std::vector<double> B((size_t)(k + 1) * (k + 1), 0.0), rhs(k + 1, 0.0), c;
       for (int i = 0; i < k; ++i) {
-        for (int j = 0; j < k; ++j) {
-          double d = 0;
-          for (size_t x = 0; x < mm2; ++x) d += eh[i][x] * eh[j][x];
-          B[(size_t)i * (k + 1) + j] = d;
-        }
+        for (int j = 0; j < k; ++j)
+          B[(size_t)i * (k + 1) + j] = edot[{std::min(order[i], order[j]), std::max(order[i], order[j])}];
         B[(size_t)i * (k + 1) + k] = B[(size_t)k * (k + 1) + i] = -1.0;
       }
       rhs[k] = -1.0;
       if (solve_small(B, rhs, k + 1, c)) {
-        std::vector<double> fu(nn, 0.0);
-        for (int i = 0; i < k; ++i)
-          for (size_t x = 0; x < nn; ++x) fu[x] += c[i] * fh[i][x];
-        upload(dFu.p, fu.data(), nn, st);
+        std::vector<double> cs(MAXS, 0.0);
+        for (int i = 0; i < k; ++i) cs[order[i]] = c[i];
+        upload(dCoef.p, cs.data(), MAXS, st);
+        const double one = 1.0, zero = 0.0;
+        RB_CHECK(rocblas_dgemv(bl.h, rocblas_operation_none, (rocblas_int)nn, MAXS, &one, hF.p, (rocblas_int)nn,
+                               dCoef.p, 1, &zero, dFu.p, 1));
         fuse = dFu.p;
       }
     }
