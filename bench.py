@@ -83,6 +83,19 @@ def df_report(hf, M, steps=5):
                      "k_tflops_upper": k_flops / (t * 1e-3) / 1e12,
                      "mfma_peak_tflops": peak, "bf16_products": mult,
                      "max_abs_dK_rel": float(np.abs(k - k0).max()) / scale}
+    df.close()
+    # One DF-RHF solve (run_df_hf_with_aux: core-H guess, CDIIS, FP64), DF
+    # build included.
+    r = hf.df_rhf(b, a, mol.zxyz)
+    out["df_rhf_solve"] = {"seconds": r["seconds_total"], "seconds_fock": r["seconds_fock"],
+                           "iterations": r["iterations"], "converged": bool(r["converged"]),
+                           "energy": r["total_energy"], "solves_per_s": 1.0 / r["seconds_total"]}
+    # The same with split-BF16 exchange until the residual is below 1e-4, FP64 after.
+    m = hf.df_rhf(b, a, mol.zxyz, k_precision="bf16x3", switch_residual=1e-4)
+    out["df_rhf_solve_bf16x3_then_fp64"] = {
+        "seconds": m["seconds_total"], "seconds_fock": m["seconds_fock"], "iterations": m["iterations"],
+        "converged": bool(m["converged"]), "energy": m["total_energy"], "solves_per_s": 1.0 / m["seconds_total"],
+        "dE_vs_fp64_solve": m["total_energy"] - r["total_energy"]}
     return out
 
 

@@ -61,7 +61,8 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
 
 class DfhfOptions(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("energy_tol", C.c_double), ("residual_tol", C.c_double),
-                ("density_tol", C.c_double), ("metric_cutoff", C.c_double)]
+                ("density_tol", C.c_double), ("metric_cutoff", C.c_double), ("k_precision", C.c_int32),
+                ("switch_residual", C.c_double)]
 
 
 class DfhfResult(C.Structure):
@@ -359,9 +360,20 @@ class DfTensor:
         return j, k, ms.value
 
 
-def df_rhf(basis, aux, zxyz, charge=0, max_iter=100, e_tol=1e-12, r_tol=1e-10, d_tol=1e-10, metric_cutoff=1e-10):
-    """DF-RHF (run_df_hf_with_aux, src/hf_variants/ri_df_hf.rs:122-263); defaults of DFHFOptions."""
-    opt = DfhfOptions(max_iter, e_tol, r_tol, d_tol, metric_cutoff)
+DF_PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+def df_rhf(basis, aux, zxyz, charge=0, max_iter=100, e_tol=1e-12, r_tol=1e-10, d_tol=1e-10, metric_cutoff=1e-10,
+           k_precision="fp64", switch_residual=1e-4):
+    """DF-RHF (run_df_hf_with_aux, src/hf_variants/ri_df_hf.rs:122-263); defaults of DFHFOptions.
+
+    k_precision "fp32" or "bf16x3" runs the exchange contraction of the early
+    iterations in that precision until the projected residual is below
+    switch_residual, then finishes with FP64 builds (no reference
+    counterpart; "fp64" is the reference's scheme)."""
+    if k_precision not in DF_PRECISION:
+        raise ValueError("k_precision must be one of %s" % sorted(DF_PRECISION))
+    opt = DfhfOptions(max_iter, e_tol, r_tol, d_tol, metric_cutoff, DF_PRECISION[k_precision], switch_residual)
     res = DfhfResult()
     zxyz = np.ascontiguousarray(zxyz, dtype=np.float64)
     n = basis.n

@@ -567,6 +567,8 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
   if (natom <= 0 || opt->max_iter <= 0 || !(opt->energy_tol > 0) || !(opt->residual_tol > 0) ||
       !(opt->density_tol > 0) || !(opt->metric_cutoff > 0))
     throw HfgError{HFG_INVALID_INPUT, "DF-HF requires finite geometry and positive convergence controls"};
+  if (opt->k_precision < 0 || opt->k_precision > 2 || !(opt->switch_residual > 0))
+    throw HfgError{HFG_INVALID_INPUT, "DF-HF k_precision must be 0, 1 or 2 and switch_residual positive"};
   std::vector<std::array<double, 4>> atoms(natom);
   double el = -charge;
   for (int a = 0; a < natom; ++a) {
@@ -636,9 +638,17 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
   };
   double t_fock = 0;
   int builds = 0;
+  int kprec = opt->k_precision;  // nonzero until the switch to FP64 builds
+  int fp64_builds = 0;
   auto fock = [&](double* F) {
     const auto tf = clk::now();
-    df.jk(dCo.p, no, dJ.p, dK.p);
+    if (kprec != 0) {
+      df.jk(dCo.p, no, dJ.p, nullptr);  // J in FP64
+      df.k_reduced(dCo.p, no, dK.p, kprec);
+    } else {
+      df.jk(dCo.p, no, dJ.p, dK.p);
+      ++fp64_builds;
+    }
     hipLaunchKernelGGL(k_axpby_fock, grid(nn), dim3(tb), 0, st, dH.p, dJ.p, dK.p, F, nn);
     HIP_CHECK(hipStreamSynchronize(st));
     t_fock += std::chrono::duration<double>(clk::now() - tf).count();
@@ -684,7 +694,8 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
     hipLaunchKernelGGL(k_sub, grid(nn), dim3(tb), 0, st, dPn.p, dP.p, dTmp.p, nn);
     const double dch = bl.amax(nn, dTmp.p);
     if (!std::isfinite(ce) || !std::isfinite(resid)) throw HfgError{HFG_INTEGRAL, "DF-HF energy/residual became nonfinite"};
-    if (std::fabs(ce - last) <= opt->energy_tol && resid <= opt->residual_tol && dch <= opt->density_tol) {
+    if (kprec == 0 && fp64_builds >= 2 && std::fabs(ce - last) <= opt->energy_tol && resid <= opt->residual_tol &&
+        dch <= opt->density_tol) {
       occ_copy(dCn.p);
       fock(dF.p);
       res->total_energy = energy(dPn.p, dF.p) + enuc;
@@ -701,6 +712,17 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
       return HFG_OK;
     }
     last = ce;
+    if (kprec != 0 && resid < opt->switch_residual) {
+      // from here on FP64 builds only; the reduced-precision Focks leave the
+      // extrapolation history, this one included
+      kprec = 0;
+      order.clear();
+      edot.clear();
+      last = INFINITY;
+      diag(dF.p, dC.p, dP.p);
+      occ_copy(dC.p);
+      continue;
+    }
     HIP_CHECK(hipMemcpyAsync(hF.p + (size_t)slot * nn, dF.p, nn * 8, hipMemcpyDeviceToDevice, st));
     for (auto e = edot.begin(); e != edot.end();)
       if (e->first.first == slot || e->first.second == slot)

@@ -167,10 +167,18 @@ int hfg_df_jk(hfg_df* df, const double* cocc, int nocc, double* j, double* k);
 int hfg_df_jk_ex(hfg_df* df, const double* cocc, int nocc, double* j, double* k, int precision,
                  double* device_ms);
 
-/* DFHFOptions (src/hf_variants/ri_df_hf.rs:21-45) and DFHFResult (:47-64). */
+/* DFHFOptions (src/hf_variants/ri_df_hf.rs:21-45) and DFHFResult (:47-64),
+ * with two fields that have no reference counterpart: k_precision selects
+ * the exchange contraction of the early iterations (0 FP64 throughout, as
+ * the reference; 1 FP32; 2 split BF16, as in hfg_df_jk_ex), used until the
+ * projected residual falls below switch_residual; from then on every build
+ * is FP64, the DIIS history restarts, and convergence is only accepted on
+ * FP64 builds (at least two of them). */
 typedef struct {
   int32_t max_iter;
   double energy_tol, residual_tol, density_tol, metric_cutoff;
+  int32_t k_precision;
+  double switch_residual;
 } hfg_dfhf_options;
 typedef struct {
   double total_energy;

@@ -77,6 +77,24 @@ def test_df_rhf_water_dimer_matches_oracle(hf):
     assert abs(r["iterations"] - o.iterations) <= 1
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_df_rhf_mixed_precision_reaches_the_fp64_energy(hf, prec):
+    """Reduced-precision exchange in the early iterations, FP64 at the end:
+    the energy is the FP64 oracle's to 1e-8 Eh (no reference counterpart; the
+    reference is FP64 throughout)."""
+    mol = M.water_cluster((2, 1, 1), seed=2)
+    ob, oa = M.basis_text("cc-pvdz"), M.basis_text("def2-universal-jkfit")
+    b, a = hf.Basis.from_g94(ob, mol.symbols, mol.xyz), hf.Basis.from_g94(oa, mol.symbols, mol.xyz)
+    r = hf.df_rhf(b, a, mol.zxyz, k_precision=prec)
+    o, _, _, _, _ = O.df_rhf(O.Basis.build(ob, mol.symbols, mol.xyz), O.Basis.build(oa, mol.symbols, mol.xyz),
+                             mol.zxyz)
+    print("DF dimer %s: E %.12f vs %.12f, iters %d vs %d" % (prec, r["total_energy"], o.total_energy,
+                                                            r["iterations"], o.iterations))
+    assert r["converged"]
+    assert abs(r["total_energy"] - o.total_energy) < 1e-8
+    assert r["iterations"] <= o.iterations + 4
+
+
 def test_df_rejects_bad_inputs(hf):
     _, syms, xyz = water_fixture()
     b = hf.Basis.from_g94(M.basis_text("sto-3g"), syms, xyz)
