@@ -4,6 +4,8 @@
 // Fock-build driver over the J/K kernels (jk_kernel.h, jk_plan.cpp).
 #include <algorithm>
 #include <array>
+#include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -199,8 +201,20 @@ Engine::Engine(const hfg_basis_view& b) {
   d_H.alloc(nn);
   d_F.alloc(nn);
   sync();
+  const bool trace = std::getenv("HFG_TRACE") != nullptr;
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  auto lap = [&](const char* what) {
+    if (!trace) return;
+    const auto t1 = clk::now();
+    fprintf(stderr, "hfg engine %-10s %8.2f ms\n", what, std::chrono::duration<double>(t1 - t0).count() * 1e3);
+    t0 = t1;
+  };
+  lap("upload");
   compute_schwarz();
+  lap("schwarz");
   build_tasks();
+  lap("plan");
 }
 
 Engine::~Engine() {

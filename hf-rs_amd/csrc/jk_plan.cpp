@@ -18,6 +18,9 @@
 // 1/2, which reproduces 1/((1+d_AB)(1+d_CD)(1+d_AB,CD)) of the unique hf-rs
 // quartets (the digestion sums the eight permutations).
 #include <algorithm>
+#include <cstdlib>
+#include <cstdio>
+#include <chrono>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -110,6 +113,14 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
                     double screen, double prim_screen, const BlockFn& blocks, const int (*caps)[4]) {
   JKPlan plan;
   if (!caps) caps = GC_CAPS;
+  const bool trace = std::getenv("HFG_TRACE") != nullptr;
+  auto t0 = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    if (!trace) return;
+    const auto t1 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "hfg plan   %-10s %8.2f ms\n", what, std::chrono::duration<double>(t1 - t0).count() * 1e3);
+    t0 = t1;
+  };
   const std::vector<GenShell> gs = merge_shells(shells);
   plan.n_gen_shells = (int)gs.size();
   auto seg_pair = [](int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; };
@@ -166,6 +177,7 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
       gp.push_back(std::move(P));
     }
 
+  lap("pairs");
   // ---- bound of every unit primitive pair: B_q = sqrt(max_ij |(ij|ij)|) with
   // the largest component factors of the two shells, from one block each.
   std::vector<double> Bu(up.size(), 0.0);
@@ -221,6 +233,7 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
     for (const auto& c : P.coef)
       for (int q = P.q0; q < P.q1; ++q) bmax = std::max(bmax, std::fabs(c[q - P.q0]) * Bu[q]);
 
+  lap("bounds");
   // ---- views per (pair class, member caps RA x RB) that some class uses:
   // the members of each shell are cut into chunks of at most RA (RB); a view
   // is one chunk of each, its combinations padded to RA x RB (index -1).
@@ -275,6 +288,7 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
         }
     }
 
+  lap("views");
   // ---- ket groups per set: views whose primitive counts fall in one bucket
   // (steps of at most 25%), padded to the largest count of the group with
   // zero-weight primitives, sorted by qmax descending; heavy groups first.
@@ -315,53 +329,76 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
     }
   }
 
-  // ---- device tables
-  std::vector<ViewDev> vd;
-  std::vector<ComboDev> cd;
-  std::vector<double> brec;  // bra records, HFG_BRA_REC doubles per primitive pair
-  size_t nbrec = 0;
+  lap("groups");
+  // ---- device tables.  Each group's views, combinations and bra records
+  // start at offsets fixed by the groups before it, so the groups are filled
+  // in parallel.
   std::vector<KetGroup> kg(groups.size());
-  std::vector<double> kt_cols[9];
-  std::vector<double> kcoef;
-  size_t ktot = 0, ctot = 0;
-  for (const Group& g : groups) {
-    ktot += (size_t)g.npp * g.v.size();
-    ctot += (size_t)g.npp * sets[g.set].ra * sets[g.set].rb * g.v.size();
-  }
-  const std::vector<double>* ucol[9] = {&up.p, &up.Px, &up.Py, &up.Pz, &up.PAx, &up.PAy, &up.PAz, &up.K, &up.H};
-  for (auto& c : kt_cols) c.assign(std::max<size_t>(1, ktot), 0.0);
-  kcoef.assign(std::max<size_t>(1, ctot), 0.0);
-  size_t kb = 0, cb = 0;
-  plan.prim_pairs_kept = 0;
+  std::vector<size_t> v_off(groups.size() + 1, 0), c_off(groups.size() + 1, 0), r_off(groups.size() + 1, 0),
+      k_off(groups.size() + 1, 0), f_off(groups.size() + 1, 0);
   for (size_t gi = 0; gi < groups.size(); ++gi) {
     const Group& g = groups[gi];
     const ViewSet& S = sets[g.set];
+    const int cap = S.ra * S.rb;
+    size_t nc_ = 0, nr = 0;
+    for (int vi : g.v) {
+      nc_ += S.views[vi].combos.size();
+      nr += S.views[vi].prims.size();
+    }
+    v_off[gi + 1] = v_off[gi] + g.v.size();
+    c_off[gi + 1] = c_off[gi] + nc_;
+    r_off[gi + 1] = r_off[gi] + nr;
+    k_off[gi + 1] = k_off[gi] + (size_t)g.npp * g.v.size();
+    f_off[gi + 1] = f_off[gi] + (size_t)g.npp * cap * g.v.size();
+  }
+  const size_t ktot = k_off[groups.size()], ctot = f_off[groups.size()], nbrec = r_off[groups.size()];
+  std::vector<ViewDev> vd(v_off[groups.size()]);
+  std::vector<ComboDev> cd(c_off[groups.size()]);
+  std::vector<double> brec(nbrec * HFG_BRA_REC, 0.0);  // bra records, HFG_BRA_REC doubles per primitive pair
+  std::vector<double> kt_cols[9];
+  std::vector<double> kcoef;
+  const std::vector<double>* ucol[9] = {&up.p, &up.Px, &up.Py, &up.Pz, &up.PAx, &up.PAy, &up.PAz, &up.K, &up.H};
+  for (auto& c : kt_cols) c.assign(std::max<size_t>(1, ktot), 0.0);
+  kcoef.assign(std::max<size_t>(1, ctot), 0.0);
+  plan.view_q.assign(vd.size(), 0.0);
+  plan.prim_pairs_kept = nbrec;
+  for (const Group& g : groups) {
+    const ViewSet& S = sets[g.set];
+    const int cap = S.ra * S.rb, coff = S.pc == 0 ? HFG_BRA_COEF_SS : HFG_BRA_COEF;
+    if (cap > 1 && coff + cap > HFG_BRA_REC) throw HfgError{HFG_INVALID_INPUT, "contraction cap beyond the bra record"};
+  }
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long gi_ = 0; gi_ < (long)groups.size(); ++gi_) {
+    const size_t gi = (size_t)gi_;
+    const Group& g = groups[gi];
+    const ViewSet& S = sets[g.set];
     const int cap = S.ra * S.rb, size = (int)g.v.size();
-    kg[gi] = KetGroup{(int)vd.size(), size, (int)kb, g.npp, (int)cb, 0, 0, 0};
+    const size_t kb = k_off[gi], cb = f_off[gi];
+    size_t ci_next = c_off[gi], rec = r_off[gi];
+    kg[gi] = KetGroup{(int)v_off[gi], size, (int)kb, g.npp, (int)cb, 0, 0, 0};
+    const int coff = S.pc == 0 ? HFG_BRA_COEF_SS : HFG_BRA_COEF;
     for (int j = 0; j < size; ++j) {
       const View& v = S.views[g.v[j]];
       const GenPair& P = gp[v.pair];
       ViewDev d;
       std::memset(&d, 0, sizeof d);
       const int npv = (int)v.prims.size();  // this view's primitives; the group pads to g.npp
-      d.p0 = (int)nbrec;
+      d.p0 = (int)rec;
       d.p1 = d.p0 + npv;
       d.na = v.na;
       d.nb = v.nb;
-      d.c0 = (int)cd.size();
+      d.c0 = (int)ci_next;
       d.coef0 = 0;
       d.same = P.same ? 1 : 0;
       for (int k = 0; k < 3; ++k) d.AB[k] = P.AB[k];
-      vd.push_back(d);
-      plan.view_q.push_back(v.qmax);
+      vd[v_off[gi] + j] = d;
+      plan.view_q[v_off[gi] + j] = v.qmax;
       for (int ci : v.combos) {
         // a missing member repeats the view's first combination with bound 0
         const auto& mm = P.combos[ci >= 0 ? ci : v.combos[0]];
         const int sa = gs[P.A].mem[mm[0]], sb = gs[P.B].mem[mm[1]];
-        cd.push_back(ComboDev{shells[sa].ao, shells[sb].ao, ci >= 0 ? Q[seg_pair(sa, sb)] : 0.0});
+        cd[ci_next++] = ComboDev{shells[sa].ao, shells[sb].ao, ci >= 0 ? Q[seg_pair(sa, sb)] : 0.0};
       }
-      const int coff = S.pc == 0 ? HFG_BRA_COEF_SS : HFG_BRA_COEF;
-      if (cap > 1 && coff + cap > HFG_BRA_REC) throw HfgError{HFG_INVALID_INPUT, "contraction cap beyond the bra record"};
       for (int ip = npv; ip < g.npp; ++ip) {  // padding: p = 1, K = 0 (contributes exactly 0)
         kt_cols[0][kb + (size_t)ip * size + j] = 1.0;
         kt_cols[8][kb + (size_t)ip * size + j] = 0.5;
@@ -375,8 +412,7 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
           kt_cols[c][kb + (size_t)ip * size + j] = x[c];
         }
         // record: p, K, Px, Py, Pz, [PAx, PAy, PAz, 1/(2p) beyond ss], coefficients
-        brec.resize(brec.size() + HFG_BRA_REC, 0.0);
-        double* r = &brec[brec.size() - HFG_BRA_REC];
+        double* r = &brec[rec * HFG_BRA_REC];
         r[0] = x[0];
         r[1] = x[7];
         r[2] = x[1];
@@ -394,36 +430,36 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
             r[coff + gg] = cf;
             kcoef[cb + ((size_t)ip * cap + gg) * size + j] = cf;
           }
-        ++nbrec;
+        ++rec;
       }
-      plan.prim_pairs_kept += npv;
     }
-    kb += (size_t)g.npp * size;
-    cb += (size_t)g.npp * cap * size;
   }
 
+  lap("tables");
   // ---- tasks per class: bra views of the class's bra set, ket groups of its
   // ket set; kets with qmax_bra * qmax_ket > screen form a prefix.  When bra
   // and ket sets coincide each unordered pair of views is listed once.  The
   // tasks of one job share their bra view and are contiguous; jobs are ordered
   // by estimated cost, heaviest first.
-  std::vector<JKTask> all;
-  plan.task_begin.assign(HFG_NCLASS + 1, 0);
-  plan.job_begin.assign(HFG_NCLASS + 1, 0);
-  plan.jobs.clear();
+  struct JobRange {
+    double cost;
+    size_t a, b;  // tasks [a, b) of the class list
+  };
+  std::vector<std::vector<JKTask>> tl_c(HFG_NCLASS);
+  std::vector<std::vector<JobRange>> jobs_cv(HFG_NCLASS);
+#pragma omp parallel for schedule(dynamic, 1)
   for (int c = 0; c < HFG_NCLASS; ++c) {
-    plan.task_begin[c] = all.size();
-    plan.job_begin[c] = plan.jobs.size();
     const int sb = class_sets[c][0], sk = class_sets[c][1];
     const bool same_set = sb == sk;
     const double digest = 40.0 * caps[c][0] * caps[c][1] * caps[c][2] * caps[c][3];
-    std::vector<std::pair<double, std::vector<JKTask>>> per_bra;
+    std::vector<JKTask>& tl = tl_c[c];  // grouped by bra view
+    std::vector<JobRange> per_bra;
     for (size_t ib = 0; ib < set_groups[sb].size(); ++ib) {
       const int gb = set_groups[sb][ib];
       const Group& B = groups[gb];
       for (size_t i = 0; i < B.v.size(); ++i) {
         const double qb = sets[sb].views[B.v[i]].qmax;
-        std::vector<JKTask> tl;
+        const size_t a = tl.size();
         double cost = 0.0;
         for (size_t ik = 0; ik < set_groups[sk].size(); ++ik) {
           if (same_set && ik > ib) continue;
@@ -445,32 +481,38 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
             cost += (double)m * ((double)B.npp * K.npp + digest);
           }
         }
-        if (!tl.empty()) per_bra.emplace_back(cost, std::move(tl));
+        if (tl.size() > a) per_bra.push_back(JobRange{cost, a, tl.size()});
       }
     }
     // A bra with many tasks is cut into jobs of at most jt tasks, jt chosen so
     // that the class has about 1024 jobs (four per CU and more) when it has the
     // tasks for it: enough workgroups for the whole chip, and as many tasks per
     // job as that allows (each job ends with one flush of its LDS rows).
-    size_t ntask_c = 0;
-    for (auto& pb : per_bra) ntask_c += pb.second.size();
-    const size_t jt = std::max<size_t>(4, ntask_c / 1024);
-    std::vector<std::pair<double, std::vector<JKTask>>> jobs_c;
-    for (auto& pb : per_bra) {
-      const size_t nt = pb.second.size(), nj = (nt + jt - 1) / jt;
+    const size_t jt = std::max<size_t>(4, tl.size() / 1024);
+    std::vector<JobRange>& jobs = jobs_cv[c];
+    for (const JobRange& pb : per_bra) {
+      const size_t nt = pb.b - pb.a, nj = (nt + jt - 1) / jt;
       for (size_t j = 0; j < nj; ++j) {
         const size_t a0 = j * nt / nj, a1 = (j + 1) * nt / nj;
-        jobs_c.emplace_back(pb.first * (double)(a1 - a0) / nt,
-                            std::vector<JKTask>(pb.second.begin() + a0, pb.second.begin() + a1));
+        jobs.push_back(JobRange{pb.cost * (double)(a1 - a0) / nt, pb.a + a0, pb.a + a1});
       }
     }
-    std::stable_sort(jobs_c.begin(), jobs_c.end(),
-                     [](const std::pair<double, std::vector<JKTask>>& x,
-                        const std::pair<double, std::vector<JKTask>>& y) { return x.first > y.first; });
-    for (auto& jb : jobs_c) {
+    std::stable_sort(jobs.begin(), jobs.end(), [](const JobRange& x, const JobRange& y) { return x.cost > y.cost; });
+  }
+  std::vector<JKTask> all;
+  plan.task_begin.assign(HFG_NCLASS + 1, 0);
+  plan.job_begin.assign(HFG_NCLASS + 1, 0);
+  plan.jobs.clear();
+  size_t ntot = 0;
+  for (auto& t : tl_c) ntot += t.size();
+  all.reserve(ntot);
+  for (int c = 0; c < HFG_NCLASS; ++c) {
+    plan.task_begin[c] = all.size();
+    plan.job_begin[c] = plan.jobs.size();
+    for (const JobRange& jb : jobs_cv[c]) {
       plan.jobs.push_back((int)all.size());
-      plan.job_cost.push_back(jb.first);
-      all.insert(all.end(), jb.second.begin(), jb.second.end());
+      plan.job_cost.push_back(jb.cost);
+      all.insert(all.end(), tl_c[c].begin() + jb.a, tl_c[c].begin() + jb.b);
     }
   }
   plan.job_begin[HFG_NCLASS] = plan.jobs.size();
@@ -493,6 +535,7 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
   plan.combos = std::move(cd);
   plan.groups = std::move(kg);
   plan.tasks = std::move(all);
+  lap("tasks");
   return plan;
 }
 

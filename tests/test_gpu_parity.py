@@ -219,6 +219,23 @@ def test_rhf_h2o16_incremental_matches_oracle(hf):
     assert abs(r["iterations"] - ref["iterations"]) <= 1
 
 
+def test_rhf_densities_by_purification_match_the_eigensolver(hf, monkeypatch):
+    """The iteration densities from SP2 purification (default) and from the
+    eigensolver (HFG_SCF_EIGH=1) give the same converged energy to 1e-10 Eh
+    and iteration counts within 1, on the spherical (H2O)4/cc-pVDZ solve."""
+    mol = M.water_cluster((2, 2, 1), seed=2)
+    b = hf.Basis.from_g94(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz)
+    monkeypatch.delenv("HFG_SCF_EIGH", raising=False)
+    r = hf.scf_rhf(b, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
+    monkeypatch.setenv("HFG_SCF_EIGH", "1")
+    e = hf.scf_rhf(b, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
+    print("(H2O)4 SP2 vs eigh: E %.12f vs %.12f, iterations %d vs %d" % (r["total_energy"], e["total_energy"],
+                                                                        r["iterations"], e["iterations"]))
+    assert r["converged"] and e["converged"]
+    assert abs(r["total_energy"] - e["total_energy"]) < 1e-10
+    assert abs(r["iterations"] - e["iterations"]) <= 1
+
+
 @pytest.mark.skip(reason="a single-molecule J/K run in the same process after this batch ended with an HSA "
                          "queue abort (HSA_STATUS_ERROR_OUT_OF_RESOURCES) on the (dd|dd) kernel, which uses 13.9 KB "
                          "of scratch per lane; cause not established, case kept off the GPU")
