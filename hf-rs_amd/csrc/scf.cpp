@@ -454,11 +454,14 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   auto grid = [&](size_t cnt) { return dim3((unsigned)((cnt + tb - 1) / tb)); };
   double t_fock = 0;
   int builds = 0;
+  const bool trace = std::getenv("HFG_TRACE") != nullptr;
   auto fock = [&](const double* P, double* F) {
     const auto t0 = clk::now();
     eng.jk_device(P, nullptr, nullptr, dH.p, F);
     HIP_CHECK(hipStreamSynchronize(st));
-    t_fock += std::chrono::duration<double>(clk::now() - t0).count();
+    const double dt = std::chrono::duration<double>(clk::now() - t0).count();
+    t_fock += dt;
+    if (trace) fprintf(stderr, "hfg scf full build: %.2f ms\n", dt * 1e3);
     ++builds;
   };
   // Incremental builds inside the iterations (opt->incremental_fock): G, the
@@ -488,7 +491,9 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
     hipLaunchKernelGGL(k_add, grid(nn), dim3(tb), 0, st, dH.p, dG2.p, F, nn);
     HIP_CHECK(hipMemcpyAsync(dPb.p, P, nn * 8, hipMemcpyDeviceToDevice, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    t_fock += std::chrono::duration<double>(clk::now() - t0).count();
+    const double dt = std::chrono::duration<double>(clk::now() - t0).count();
+    t_fock += dt;
+    if (trace) fprintf(stderr, "hfg scf incremental build: max|dP| %.3e, %.2f ms\n", dmax, dt * 1e3);
     ++builds;
     ++since_full;
   };
