@@ -459,7 +459,8 @@ struct DfEngine {
     hipLaunchKernelGGL(k_permute_qmi, dim3((tot + 255) / 256), dim3(256), 0, stream, d_A.as<double>(),
                        d_At.as<double>(), rank, n, o);
     HIP_CHECK(hipGetLastError());
-    // FP64: one GEMM (split-K measured slower here)
+    // FP64: one GEMM (split-K measured slower here, and rocBLAS dsyrk, with
+    // half the flops, 2.1x slower on (H2O)32/cc-pVDZ with JKFIT)
     bl->gemm(false, true, n, n, rank * o, 2.0, d_At.as<double>(), rank * o, d_At.as<double>(), rank * o, 0.0, dK, n);
   }
 
