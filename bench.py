@@ -142,6 +142,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--solve-repeats", type=int, default=8, help="warm SCF solves for the solve-latency percentiles")
     ap.add_argument("--batch", action="store_true",
                     help="also time the batched multi-molecule workload (off: see DESIGN.md, batches)")
     ap.add_argument("--batch-count", type=int, default=512)
@@ -246,7 +247,15 @@ def main():
         cold = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
         full = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
         r = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True, incremental=True)
+        # Solve latency over repeated warm solves (the same molecule, each a
+        # complete solve from the geometry: set-up, guess, iterations).
+        sl = [r["seconds_total"]]
+        for _ in range(args.solve_repeats - 1):
+            sl.append(hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True,
+                                 incremental=True)["seconds_total"])
         line["scf"] = {"solves_per_s": 1.0 / r["seconds_total"], "seconds": r["seconds_total"],
+                       "solve_latency_ms": {"p50": 1e3 * float(np.percentile(sl, 50)),
+                                            "p99": 1e3 * float(np.percentile(sl, 99)), "solves": len(sl)},
                        "cold_seconds": cold["seconds_total"],
                        "iterations": r["iterations"], "converged": r["converged"], "fock_builds": r["fock_builds"],
                        "energy": r["total_energy"], "cold_energy": cold["total_energy"],
