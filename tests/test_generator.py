@@ -1,0 +1,67 @@
+"""The recurrence generator (hf-rs_amd/csrc/gen_eri.py): output slices of
+the J/K classes partition each Cartesian block, every slice's horizontal
+recurrences reach only accumulators it declares, and the members caps and
+slice tables written for the C++ side agree with the generator's."""
+import importlib.util
+import os
+import re
+
+import pytest
+
+from conftest import PKG
+
+spec = importlib.util.spec_from_file_location("gen_eri", os.path.join(PKG, "csrc", "gen_eri.py"))
+G = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(G)
+
+JK = G.classes()[:21]
+
+
+@pytest.mark.parametrize("c", JK, ids=lambda c: "".join(map(str, c)))
+def test_slices_partition_the_block(c):
+    la, lb, lc, ld = c
+    na, nc = G.ncart(la), G.ncart(lc)
+    ns = G.slice_count(c)
+    owner = {}
+    for S in range(ns):
+        keep = G.slice_keep(c, S)
+        for ia in range(na):
+            for ic in range(nc):
+                if keep is None or keep(ia, ic):
+                    assert (ia, ic) not in owner, "output (%d, %d) in two slices" % (ia, ic)
+                    owner[(ia, ic)] = S
+    assert len(owner) == na * nc
+
+
+@pytest.mark.parametrize("c", [c for c in JK if G.slice_mode(c)], ids=lambda c: "".join(map(str, c)))
+def test_slice_code_declares_what_it_reads(c):
+    """Each slice's generated function writes exactly its outputs, and every
+    accumulator its horizontal recurrences read is one it accumulates."""
+    la, lb, lc, ld = c
+    nab, ncd = G.ncart(la) * G.ncart(lb), G.ncart(lc) * G.ncart(ld)
+    RA, RB, RC, RD = G.gc_caps(c)
+    written = set()
+    for S in range(G.slice_count(c)):
+        src = G.gen_class_gc(*c, S=S)
+        decl = set(re.findall(r"\b(a\d+_\d+_\d{3}_\d{3})\s*=\s*0\.0", src))
+        used = set(re.findall(r"\b(a\d+_\d+_\d{3}_\d{3})\b", src))
+        assert used <= decl
+        idx = [int(x) for x in re.findall(r"\bI\[(\d+)\]\s*=", src)]
+        assert len(idx) == len(set(idx))
+        assert not (written & set(idx))
+        written |= set(idx)
+    assert written == set(range(RA * RB * RC * RD * nab * ncd))
+
+
+def test_slices_cut_the_accumulators_of_the_high_classes():
+    """The sliced classes hold at most about 100 contracted accumulators per slice
+    (unsliced, (dd|dd) needs 961)."""
+    for c in JK:
+        if not G.slice_mode(c):
+            continue
+        E = G.levels(c[0], c[0] + c[1])
+        F = G.levels(c[2], c[2] + c[3])
+        for S in range(G.slice_count(c)):
+            used = set()
+            G.hrr_lines(*c, {(e, f): "x" for e in E for f in F}, "", 0, G.slice_keep(c, S), used)
+            assert len(used) <= 100, (c, S, len(used))
