@@ -1,0 +1,13 @@
+// jk_d.hip -- J/K kernels of the class (dd|dp) (36 output slices), a
+// translation unit of its own so that the sliced classes compile in parallel.
+#include "jk_kernel.h"
+
+namespace hfg {
+
+bool launch_jk_d(int cls, const JKArgs& a, int ntask, int max_blocks, hipStream_t s) {
+  if (cls != 19) return false;
+  launch_jk<2, 2, 2, 1>(a, ntask, max_blocks, s);
+  return true;
+}
+
+}  // namespace hfg
