@@ -93,6 +93,21 @@ def test_direct_jk_matches_oracle(hf, basis, grid):
     assert np.abs(j - j.T).max() == 0.0 and np.abs(k - k.T).max() == 0.0
 
 
+@pytest.mark.parametrize("n_carbon,basis", [(3, "def2-svp"), (2, "cc-pvdz"), (4, "6-31g*")])
+def test_direct_jk_alkane_matches_oracle(hf, n_carbon, basis):
+    """All-trans alkanes (the config-3 molecule family): carbon d shells with
+    other contraction patterns than water's, through every sliced class."""
+    mol = M.alkane(n_carbon)
+    text = M.basis_text(basis)
+    ctx = hf.Context(hf.Basis.from_g94(text, mol.symbols, mol.xyz))
+    p = sym_density(ctx.n, 43)
+    j, k = ctx.contract_jk(p)
+    jr, kr, _ = O.Basis.build(text, mol.symbols, mol.xyz).direct_jk(p)
+    ej, ek = np.abs(j - jr).max(), np.abs(k - kr).max()
+    print("C%dH%d %s n=%d  J %.2e  K %.2e" % (n_carbon, 2 * n_carbon + 2, basis, ctx.n, ej, ek))
+    assert ej <= 1e-10 and ek <= 1e-10
+
+
 def test_build_fock_is_h_plus_j_minus_half_k(hf):
     mol = M.water_config1()
     ctx = hf.Context(hf.Basis.from_g94(M.basis_text("6-31g*"), mol.symbols, mol.xyz))
