@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -380,7 +381,19 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
     t_phase = now;
     return d;
   };
-  Engine eng(*basis);
+  // The one-electron matrices (host) are computed on a second thread while
+  // the engine is set up (Schwarz bounds on the GPU, J/K plan on the host).
+  HMat S, T, V;
+  const std::vector<Shell> sh1 = group_shells(*basis);
+  auto one_e = std::async(std::launch::async, [&] { one_electron(sh1, n, atoms, S, T, V); });
+  std::unique_ptr<Engine> eng_p;
+  try {
+    eng_p.reset(new Engine(*basis));
+  } catch (...) {
+    one_e.wait();
+    throw;
+  }
+  Engine& eng = *eng_p;
   res->seconds_setup = lap();
   hipStream_t st = eng.stream;
   Blas bl(st);
@@ -391,8 +404,7 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
       const double dx = atoms[i][1] - atoms[j][1], dy = atoms[i][2] - atoms[j][2], dz = atoms[i][3] - atoms[j][3];
       enuc += atoms[i][0] * atoms[j][0] / std::sqrt(dx * dx + dy * dy + dz * dz);
     }
-  HMat S, T, V;
-  one_electron(eng.shells, n, atoms, S, T, V);
+  one_e.get();  // the remaining wait, if any, is the one-electron phase
   HMat H(nn);
   for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
   res->seconds_one_electron = lap();
