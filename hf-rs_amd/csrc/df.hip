@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -578,7 +579,20 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
   }
   if (!(el > 0) || std::fabs(el - std::round(el)) > 1e-8 || ((long)std::lround(el)) % 2)
     throw HfgError{HFG_INVALID_INPUT, "DF-RHF requires a positive even electron count"};
-  DfEngine df(*basis, *auxv, opt->metric_cutoff);
+  // One-electron matrices (host) on a second thread during the DF build.
+  std::vector<double> S, T, V;
+  const std::vector<Shell> sh1 = group_shells(*basis);
+  const int n1 = basis->n_functions;
+  auto one_e = std::async(std::launch::async, [&] { one_electron(sh1, n1, atoms, S, T, V); });
+  std::unique_ptr<DfEngine> df_p;
+  try {
+    df_p.reset(new DfEngine(*basis, *auxv, opt->metric_cutoff));
+  } catch (...) {
+    one_e.wait();
+    throw;
+  }
+  DfEngine& df = *df_p;
+  one_e.get();
   const int n = df.n, no = (int)std::lround(el) / 2;
   hipStream_t st = df.stream;
   Blas& bl = *df.bl;
@@ -589,8 +603,6 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
       const double dx = atoms[i][1] - atoms[j][1], dy = atoms[i][2] - atoms[j][2], dz = atoms[i][3] - atoms[j][3];
       enuc += atoms[i][0] * atoms[j][0] / std::sqrt(dx * dx + dy * dy + dz * dz);
     }
-  std::vector<double> S, T, V;
-  one_electron(df.orb, n, atoms, S, T, V);
   std::vector<double> H(nn);
   for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
   Dev dS, dH, dX, dTmp, dTmp2, dF, dP, dPn, dJ, dK, dC, dCo, dCn, dW, dErr, dFu;
