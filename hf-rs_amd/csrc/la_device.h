@@ -178,6 +178,46 @@ static void download(double* h, const double* d, size_t n, hipStream_t s) {
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Projector onto the nocc lowest eigenvectors of a symmetric m x m matrix by
+// SP2 purification (the k_sp2_* kernels): h holds the matrix and is
+// overwritten (it serves as the X^2 buffer); the projector is left in x.
+// The steps stop three after tr(X - X^2) < 1e-9 (checked every fourth step
+// from the twelfth); false when 100 steps have not converged.
+struct Sp2 {
+  Dev x, b, t, err;
+  size_t cap = 0;
+  static constexpr int MAX_STEPS = 100;
+  bool run(Blas& bl, hipStream_t st, double* h, int m, int nocc) {
+    if (nocc <= 0 || nocc >= m) return false;
+    const size_t mm = (size_t)m * m;
+    if (mm > cap) {
+      x.alloc(mm);
+      cap = mm;
+    }
+    if (!b.p) {
+      b.alloc(2);
+      t.alloc(2);
+      err.alloc(MAX_STEPS);
+    }
+    const dim3 g((unsigned)((mm + 255) / 256));
+    hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(256), 0, st, h, m, b.p);
+    hipLaunchKernelGGL(k_sp2_init, g, dim3(256), 0, st, h, b.p, x.p, m);
+    int extra = -1;
+    for (int k = 0; k < MAX_STEPS; ++k) {
+      bl.gemm(false, false, m, m, m, 1.0, x.p, m, x.p, m, 0.0, h, m);  // X^2
+      hipLaunchKernelGGL(k_sp2_traces, dim3(1), dim3(256), 0, st, x.p, h, m, t.p, err.p, k);
+      hipLaunchKernelGGL(k_sp2_update, g, dim3(256), 0, st, x.p, h, t.p, m, (double)nocc);
+      if (extra > 0 && --extra == 0) return true;
+      if (extra < 0 && k >= 11 && (k + 1) % 4 == 0) {
+        double e = 0.0;
+        download(&e, err.p + k, 1, st);
+        if (std::fabs(e) < 1e-9) extra = 3;
+      }
+    }
+    return false;
+  }
+};
+
 // Symmetric eigensolver: eigenvalues ascending into w, eigenvectors as
 // columns of the row-major result V (computed as V^T by rocSOLVER's
 // column-major output and transposed in the following GEMM by the caller).

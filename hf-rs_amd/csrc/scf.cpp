@@ -519,36 +519,13 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   // density, the orbital energies and the atomic SAD solves always use the
   // eigensolver.  HFG_SCF_EIGH=1 uses it everywhere.
   const bool use_sp2 = std::getenv("HFG_SCF_EIGH") == nullptr && nocc > 0 && nocc < m;
-  Dev dSx, dSb, dSt, dSerr;
-  const int SP2_MAX = 100;
-  if (use_sp2) {
-    dSx.alloc((size_t)m * m);
-    dSb.alloc(2);
-    dSt.alloc(2);
-    dSerr.alloc(SP2_MAX);
-  }
+  Sp2 sp2;
   auto sp2_density = [&](double* P) {  // F' in dTmp2 (m x m); false: not converged
-    const size_t mm = (size_t)m * m;
-    hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(256), 0, st, dTmp2.p, m, dSb.p);
-    hipLaunchKernelGGL(k_sp2_init, grid(mm), dim3(tb), 0, st, dTmp2.p, dSb.p, dSx.p, m);
-    int extra = -1;
-    for (int k = 0; k < SP2_MAX; ++k) {
-      bl.gemm(false, false, m, m, m, 1.0, dSx.p, m, dSx.p, m, 0.0, dTmp2.p, m);  // X^2
-      hipLaunchKernelGGL(k_sp2_traces, dim3(1), dim3(256), 0, st, dSx.p, dTmp2.p, m, dSt.p, dSerr.p, k);
-      hipLaunchKernelGGL(k_sp2_update, grid(mm), dim3(tb), 0, st, dSx.p, dTmp2.p, dSt.p, m, (double)nocc);
-      if (extra > 0 && --extra == 0) {
-        // P = 2 X D X^T
-        bl.gemm(false, false, n, m, m, 1.0, dX.p, m, dSx.p, m, 0.0, dTmp.p, m);
-        bl.gemm(false, true, n, n, m, 2.0, dTmp.p, m, dX.p, m, 0.0, P, n);
-        return true;
-      }
-      if (extra < 0 && k >= 11 && (k + 1) % 4 == 0) {
-        double e = 0.0;
-        download(&e, dSerr.p + k, 1, st);
-        if (std::fabs(e) < 1e-9) extra = 3;
-      }
-    }
-    return false;
+    if (!sp2.run(bl, st, dTmp2.p, m, nocc)) return false;
+    // P = 2 X D X^T
+    bl.gemm(false, false, n, m, m, 1.0, dX.p, m, sp2.x.p, m, 0.0, dTmp.p, m);
+    bl.gemm(false, true, n, n, m, 2.0, dTmp.p, m, dX.p, m, 0.0, P, n);
+    return true;
   };
   // diagonalise X^T F X, C = X C', P = 2 C_occ C_occ^T; eigenvalues into eps
   std::vector<double> eps(m);
