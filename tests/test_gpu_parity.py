@@ -108,6 +108,26 @@ def test_direct_jk_alkane_matches_oracle(hf, n_carbon, basis):
     assert ej <= 1e-10 and ek <= 1e-10
 
 
+def test_engine_rejects_what_it_does_not_support(hf):
+    """Error paths of the C ABI: f shells (cc-pVTZ) are refused with the
+    unsupported status, an odd electron count and a non-positive tolerance
+    with the invalid-input status (src/error.rs:5-15 semantics), and the
+    engine stays usable afterwards."""
+    mol = M.water_config1()
+    with pytest.raises(hf.HfError) as e:
+        hf.Context(hf.Basis.from_g94(M.basis_text("cc-pvtz"), mol.symbols, mol.xyz))
+    assert e.value.code == hf.HFG_UNSUPPORTED
+    b = hf.Basis.from_g94(M.basis_text("sto-3g"), mol.symbols, mol.xyz)
+    with pytest.raises(hf.HfError) as e:
+        hf.scf_rhf(b, mol.zxyz, charge=1)
+    assert e.value.code == hf.HFG_INVALID_INPUT
+    with pytest.raises(hf.HfError) as e:
+        hf.scf_rhf(b, mol.zxyz, tol=0.0)
+    assert e.value.code == hf.HFG_INVALID_INPUT
+    r = hf.scf_rhf(b, mol.zxyz, tol=1e-10, conv="cdiis")
+    assert abs(r["total_energy"] - (-74.96302777726257)) < 1e-8
+
+
 def test_build_fock_is_h_plus_j_minus_half_k(hf):
     mol = M.water_config1()
     ctx = hf.Context(hf.Basis.from_g94(M.basis_text("6-31g*"), mol.symbols, mol.xyz))
