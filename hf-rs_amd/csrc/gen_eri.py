@@ -304,10 +304,14 @@ def gc_caps(c):
 # far beyond the 512 registers of a lane, and the spills bound those classes
 # by memory latency at one wave per SIMD.  Classes with many primitive
 # quartets per contracted quartet and little spilling ((pp|pp), (dp|pp),
-# (dp|ds), (dd|ps)) were measured slower sliced and stay whole.
+# (dp|ds), (dd|ps)) were measured slower sliced and stay whole.  Among the
+# modes, the one that keeps the costly digestion sums whole wins: per c
+# component for (dp|dp) (its ket Coulomb sums, global atomics, stay one per
+# output; 0.78 -> 0.42 ms on (H2O)16/cc-pVDZ against per a), per a component
+# for (dd|pp) (6 slices instead of 18 (a, c) pairs; 0.43 -> 0.28 ms).
 SLICE_MODES = {
-    (2, 1, 2, 1): 1,
-    (2, 2, 1, 1): 3,
+    (2, 1, 2, 1): 2,
+    (2, 2, 1, 1): 1,
     (2, 2, 2, 0): 2,
     (2, 2, 2, 1): 3,
     (2, 2, 2, 2): 3,
