@@ -308,12 +308,13 @@ def gc_caps(c):
 # modes, the one that keeps the costly digestion sums whole wins: per c
 # component for (dp|dp) (its ket Coulomb sums, global atomics, stay one per
 # output; 0.78 -> 0.42 ms on (H2O)16/cc-pVDZ against per a), per a component
-# for (dd|pp) (6 slices instead of 18 (a, c) pairs; 0.43 -> 0.28 ms).
+# for (dd|pp) (6 slices instead of 18 (a, c) pairs; 0.43 -> 0.28 ms), per c
+# for (dd|dp) (6 instead of 36; 0.46 -> 0.28 ms), per a for (dd|ds).
 SLICE_MODES = {
     (2, 1, 2, 1): 2,
     (2, 2, 1, 1): 1,
-    (2, 2, 2, 0): 2,
-    (2, 2, 2, 1): 3,
+    (2, 2, 2, 0): 1,
+    (2, 2, 2, 1): 2,
     (2, 2, 2, 2): 3,
 }
 

@@ -54,8 +54,9 @@ def test_slice_code_declares_what_it_reads(c):
 
 
 def test_slices_cut_the_accumulators_of_the_high_classes():
-    """The sliced classes hold at most about 100 contracted accumulators per slice
-    (unsliced, (dd|dd) needs 961)."""
+    """The sliced classes hold at most 128 contracted accumulators per slice,
+    half of the 256 doubles a lane's registers hold at one wave per SIMD
+    (unsliced, (dd|dd) needs 961; the per-c slices of (dd|dp) hold 124)."""
     for c in JK:
         if not G.slice_mode(c):
             continue
@@ -64,4 +65,4 @@ def test_slices_cut_the_accumulators_of_the_high_classes():
         for S in range(G.slice_count(c)):
             used = set()
             G.hrr_lines(*c, {(e, f): "x" for e in E for f in F}, "", 0, G.slice_keep(c, S), used)
-            assert len(used) <= 100, (c, S, len(used))
+            assert len(used) <= 128, (c, S, len(used))
