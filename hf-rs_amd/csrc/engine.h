@@ -59,8 +59,8 @@ struct JKArgs {
 // Launch of class cls's J/K kernel over ntask tasks with at most max_blocks
 // workgroups; the kernels are split over three translation units.
 void launch_jk_class(int cls, const JKArgs& a, int ntask, int max_blocks, hipStream_t s);
-// Whether class cls runs on the side stream: the classes with a dp or dd bra,
-// (dp|ss) .. (dd|dd), unless HFG_SIDE_CLASSES lists class indices
+// Whether class cls runs on the side stream: (ds|ds) and the classes with a dp
+// or dd bra, (dp|ss) .. (dd|dd), unless HFG_SIDE_CLASSES lists class indices
 // (comma-separated; measured on (H2O)16/cc-pVDZ).
 bool side_class(int cls);
 

@@ -97,7 +97,7 @@ __global__ void k_symmetrize(const double* Jacc, const double* Kacc, double* J, 
 bool side_class(int cls) {
   static const unsigned mask = [] {
     unsigned m = 0;
-    for (int c = 10; c < HFG_NCLASS; ++c) m |= 1u << c;  // (dp|ss) .. (dd|dd)
+    for (int c = 9; c < HFG_NCLASS; ++c) m |= 1u << c;  // (ds|ds), (dp|ss) .. (dd|dd)
     if (const char* env = std::getenv("HFG_SIDE_CLASSES")) {
       m = 0;
       for (const char* p = env; *p;) {
