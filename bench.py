@@ -1,17 +1,25 @@
-"""Throughput of the Fock-build hot path on one MI355X.
+"""Throughput of the Fock-build hot path on one MI355X (one GPU).
 
-Workload: direct RHF/cc-pVDZ on a seeded (H2O)16 cluster (4x2x2 lattice, 2.9 A
-spacing, PCG64 seed 2, uniform rotations, +-0.1 A jitter), 400 Cartesian /
-384 spherical functions, Schwarz threshold 1e-12, FP64 throughout.  A step is
-one Fock build: J and K of one density over every surviving shell quartet,
-with the density and the results resident in HBM.  value = Fock builds/s.
-Also reported: a full SCF solve (SAD guess, ADIIS->CDIIS, spherical
-orthogonaliser, tol 1e-10, incremental Fock builds inside the iterations) as
-SCF solves/s with its iteration count and energy error against the oracle's
-in-core solve (a later solve in the process; the first, with one-time costs,
-and a solve with full builds only, beside it), the
-per-step latency percentiles, the roofline of the dominant shell-quartet class
-kernel and of the whole build, and the oracle's CPU time on a bounded sample.
+Headline (value): SCF solves/s of the batched form on the batched workload,
+512 RHF solves of def2-SVP molecules (half all-trans alkanes CnH2n+2,
+n ~ U{1..12}; half water clusters (H2O)m, m ~ U{1..8}; PCG64 seed 3;
+Cartesian, ADIIS->CDIIS, tol 1e-10, FP64 throughout, full Fock builds),
+solved together by the server (hfg_server_*: continuous batching over
+resident slots, every iteration's Fock builds fused into one launch per
+shell-quartet class, the DIIS and purification linear algebra batched),
+against the plain single-molecule GPU driver (hfg_scf_rhf) on a sample of
+the same molecules, with the energies compared.  A step (--steps) is one
+fused Fock build of the whole 512-molecule batch (J and K of every
+molecule's resident density), timed with its per-class launches for the
+roofline of the dominant kernel.
+
+Also reported: the fused against the molecule-by-molecule Fock builds of the
+batch; the served stream (Poisson arrivals of geometry trajectories, p50/p99
+latency against the plain path); the single-molecule direct Fock build and
+SCF solve on (H2O)16/cc-pVDZ (400 Cartesian functions, seed 2); density
+fitting on (H2O)32/cc-pVDZ; and the CPU baseline: the oracle's restatement
+of the reference algorithm (in-core packed ERI tensor, contract_jk, SAD,
+ADIIS/CDIIS) on a random sample of the batched workload's molecules.
 """
 import argparse
 import importlib.util
@@ -27,6 +35,11 @@ PKG = os.path.join(ROOT, "hf-rs_amd")
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
 
 
+def progress(msg):
+    """One line per finished phase on stderr (the JSON result is stdout's only line)."""
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
+
+
 def load_pkg():
     spec = importlib.util.spec_from_file_location("hf_rs_amd", os.path.join(PKG, "__init__.py"),
                                                   submodule_search_locations=[PKG])
@@ -36,21 +49,41 @@ def load_pkg():
     return mod
 
 
-def cpu_baseline(mol, text, stride, threads, density):
-    """Oracle direct J/K (the reference's McMurchie-Davidson algorithm, FP64,
-    OpenMP) on the bra pairs ij % stride == 0, scaled by algorithmic FLOPs."""
+def cpu_baseline(M, count, threads, budget_s, seed=11):
+    """The reference algorithm on the host: the oracle's in-core RHF
+    (screened packed ERI tensor built once, contract_jk every iteration, SAD,
+    ADIIS->CDIIS; src/scf.rs:410-627), FP64 with OpenMP, on the batched
+    workload's molecules in a seeded random order until budget_s seconds have
+    been spent (at least two solves): solves/s = solves / seconds."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    ob = O.Basis.build(text, mol.symbols, mol.xyz)
-    total, _, _ = ob.fock_flops()
-    sample, _, _ = ob.fock_flops(stride, 0)
-    _, _, sec = ob.direct_jk(density, stride, 0)
-    est = sec * total / sample
-    return {"value": 1.0 / est, "unit": "Fock builds/s", "cores": threads, "kind": "port",
-            "sample": "oracle direct J/K over bra shell pairs ij %% %d == 0 (%.1f%% of the algorithmic FLOPs), "
-                      "%.1f s on %d threads, scaled to the full build" % (stride, 100 * sample / total, sec,
-                                                                          threads)}
+    mols = M.batch_config3(count, seed=3)
+    text = M.basis_text("def2-svp")
+    order = np.random.Generator(np.random.PCG64(seed)).permutation(count)
+    done, spent, names = 0, 0.0, []
+    for i in order:
+        m = mols[int(i)]
+        b = O.Basis.build(text, m.symbols, m.xyz)
+        t0 = time.perf_counter()
+        O.scf_rhf(b, m.zxyz, tol=1e-10, conv="adiissoscf")
+        spent += time.perf_counter() - t0
+        done += 1
+        names.append("%s(n=%d)" % (m.name, b.n))
+        if spent > budget_s and done >= 2:
+            break
+    cpu = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": done / spent, "unit": "SCF solves/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu,
+            "sample": "oracle in-core RHF (reference algorithm) on %d of the %d batched-workload molecules in a "
+                      "seeded random order (%s), %.1f s" % (done, count, ", ".join(names), spent)}
 
 
 def df_report(hf, M, steps=5):
@@ -103,60 +136,343 @@ def df_report(hf, M, steps=5):
     return out
 
 
-def batch_report(hf, M, count, steps=5):
-    """The batched workload: `count` molecules (half all-trans alkanes
-    CnH2n+2, n ~ U{1..12}, half water clusters (H2O)m, m ~ U{1..8}; PCG64 seed
-    3) in def2-SVP; one step = J/K of every molecule in fused launches, next to
-    the same builds molecule after molecule."""
+PROFILE = os.path.join(ROOT, "profiles", "fused_jk_config3_def2svp.json")
+
+
+def fused_report(hf, M, count, steps, warmup):
+    """The hot path of the batched workload: one step = J and K of all
+    `count` molecules' resident densities in fused launches (one per
+    shell-quartet class over the whole batch).  Timed over `steps` builds with
+    per-class HIP events on the stream each class runs on; next to it the same
+    builds molecule after molecule (each molecule's own launches).  The
+    roofline is that of the class kernel with the largest measured time per
+    launch: its executed FP64 work per launch (rocprofv3 SQ instruction
+    counters, profiles/) over that time, against the FP64 peak."""
     mols = M.batch_config3(count, seed=3)
     text = M.basis_text("def2-svp")
     bases = [hf.Basis.from_g94(text, m.symbols, m.xyz) for m in mols]
     t0 = time.perf_counter()
     batch = hf.Batch(bases)
     setup = time.perf_counter() - t0
+    progress("batch of %d molecules set up in %.1f s" % (count, setup))
     info = batch.info()
     rng = np.random.default_rng(6)
     for i, b in enumerate(bases):
         c = rng.standard_normal((b.n, max(1, b.n // 4))) * 0.3
         batch.set_density(i, c @ c.T)
-    batch.jk()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(warmup):
         batch.jk()
-    fused = (time.perf_counter() - t0) / steps
-    batch.jk_each()
+    lat = []
+    batch.profile(True)
     t0 = time.perf_counter()
     for _ in range(steps):
+        s0 = time.perf_counter()
+        batch.jk()  # returns after the build is complete on the device
+        lat.append(time.perf_counter() - s0)
+    elapsed = time.perf_counter() - t0
+    batch.profile(False)
+    stats = batch.class_stats()
+    batch.jk_each()
+    t1 = time.perf_counter()
+    for _ in range(max(1, steps // 4)):
         batch.jk_each()
-    each = (time.perf_counter() - t0) / steps
-    return {"workload": "%d molecules def2-SVP (alkanes C1-C12, water clusters 1-8; seed 3), Cartesian n %d..%d"
-                        % (count, min(b.n for b in bases), max(b.n for b in bases)),
-            "fock_builds_per_s": count / fused, "ms_per_batch": 1e3 * fused,
-            "algorithmic_tflops": info["flops"] / fused / 1e12, "shell_quartets": info["quartets"],
-            "unbatched_fock_builds_per_s": count / each, "unbatched_ms_per_batch": 1e3 * each,
-            "speedup_vs_unbatched": each / fused, "setup_s": setup}
+    each = (time.perf_counter() - t1) / max(1, steps // 4)
+    fused = elapsed / steps
+    dom = max(stats, key=lambda x: x["ms"] / max(1, x["launches"]))
+    dom_ms = dom["ms"] / max(1, dom["launches"])
+    kname = "k_jk_rows<%d,%d,%d,%d>" % tuple(dom["l"])
+    roof = {"bound": "fp64-valu", "kernel": kname, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "ms_per_launch": dom_ms, "achieved": None, "frac": None, "traffic": None,
+            "md_equivalent": {"tflops": dom["flops"] / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else None,
+                              "flops_per_launch": dom["flops"],
+                              "model": "McMurchie-Davidson loop count per primitive quartet of every surviving "
+                                       "shell quartet (the reference's work; general contraction and HGP do less, "
+                                       "so this rate can exceed the peak)"}}
+    if os.path.exists(PROFILE):
+        t = json.load(open(PROFILE))["per_kernel"].get(kname)
+        if t and dom_ms > 0:
+            ef = t["executed"]["f64_flops"]
+            ach = ef / (dom_ms * 1e-3) / 1e12
+            roof.update(achieved=ach, frac=ach / PEAK_FP64_TFLOPS, flops_per_launch=ef,
+                        traffic=t["fetch_bytes"] + t["write_bytes"],
+                        source="profiles/fused_jk_config3_def2svp.json: executed FP64 FLOPs per launch from "
+                               "SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes (FMA = 2); traffic = FETCH_SIZE "
+                               "+ WRITE_SIZE, copy-calibrated",
+                        profile_avg_us=t.get("avg_us"))
+            assert roof["frac"] <= 1.0, roof
+    out = {"workload": "%d molecules def2-SVP (alkanes C1-C12, water clusters 1-8; seed 3), Cartesian n %d..%d"
+                       % (count, min(b.n for b in bases), max(b.n for b in bases)),
+           "fock_builds_per_s": count / fused, "ms_per_batch_build": 1e3 * fused,
+           "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+           "shell_quartets": info["quartets"], "md_algorithmic_tflops": info["flops"] / fused / 1e12,
+           "unbatched_fock_builds_per_s": count / each, "unbatched_ms_per_batch": 1e3 * each,
+           "speedup_vs_unbatched": each / fused, "setup_s": setup,
+           "classes": [{"l": x["l"], "ms": x["ms"] / max(1, x["launches"]), "quartets": x["quartets"]}
+                       for x in stats if x["quartets"]]}
+    batch.close()
+    return out, roof
+
+
+def fock_report(hf, M, steps, warmup, solve_repeats, scf):
+    """Direct RHF/cc-pVDZ on the seeded (H2O)16 cluster (4x2x2 lattice, 2.9 A,
+    seed 2; 400 Cartesian / 384 spherical functions): Fock builds of one
+    molecule (density and results in HBM), and full SCF solves (SAD,
+    ADIIS->CDIIS, spherical orthogonaliser, tol 1e-10) by the plain driver."""
+    mol = M.water_cluster((4, 2, 2), seed=2)
+    text = M.basis_text("cc-pvdz")
+    basis = hf.Basis.from_g94(text, mol.symbols, mol.xyz)
+    t0 = time.perf_counter()
+    ctx = hf.Context(basis)
+    setup_s = time.perf_counter() - t0
+    info = ctx.info()
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal((basis.n, 40)) * 0.2
+    ctx.set_density(c @ c.T)
+    for _ in range(warmup):
+        ctx.jk_resident()
+    ctx.profile(True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        ctx.jk_resident()
+        lat.append(time.perf_counter() - s0)
+    elapsed = time.perf_counter() - t0
+    ctx.profile(False)
+    stats = ctx.class_stats()
+    total_flops, prim_q = ctx.fock_flops()
+    out = {"workload": "direct RHF/cc-pVDZ (H2O)16, 4x2x2 lattice 2.9 A, seed 2, Schwarz 1e-12",
+           "n_cart": basis.n, "n_sph": 384, "shell_quartets": info["n_quartets"], "primitive_quartets": prim_q,
+           "fock_builds_per_s": steps / elapsed, "ms_per_build": 1e3 * elapsed / steps,
+           "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+           "md_algorithmic_tflops": total_flops / (elapsed / steps) / 1e12, "setup_s": setup_s,
+           "classes": [{"l": x["l"], "ms": x["ms"] / max(1, x["launches"]), "quartets": x["quartets"]}
+                       for x in stats if x["quartets"]]}
+    ctx.close()
+    if scf:
+        cold = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
+        full = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
+        sl = []
+        for _ in range(solve_repeats):
+            r = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True, incremental=True)
+            sl.append(r["seconds_total"])
+        out["scf"] = {"solves_per_s": 1.0 / float(np.median(sl)), "iterations": r["iterations"],
+                      "converged": r["converged"], "energy": r["total_energy"],
+                      "solve_latency_ms": {"p50": 1e3 * float(np.percentile(sl, 50)),
+                                           "p99": 1e3 * float(np.percentile(sl, 99)), "solves": len(sl)},
+                      "phases_s": {k: r["seconds_" + k] for k in ("setup", "one_electron", "guess", "fock",
+                                                                  "linalg")},
+                      "cold_seconds": cold["seconds_total"],
+                      "fock": "incremental (J/K of the density change inside the iterations; full first and final)",
+                      "full_build_solve": {"seconds": full["seconds_total"], "iterations": full["iterations"],
+                                           "energy": full["total_energy"]}}
+        ref = os.path.join(ROOT, "tests", "golden", "h2o16_ccpvdz_scf.json")
+        if os.path.exists(ref):
+            o = json.load(open(ref))
+            out["scf"]["dE_vs_oracle"] = r["total_energy"] - o["total_energy"]
+            out["scf"]["full_build_solve"]["dE_vs_oracle"] = full["total_energy"] - o["total_energy"]
+            out["scf"]["oracle_iterations"] = o["iterations"]
+            # the reference algorithm's in-core timing of this solve, from the fixture's own run
+            out["scf"]["oracle_incore_record"] = {k: o[k] for k in o if "second" in k or "threads" in k}
+    return out
+
+
+def scf_batch_report(hf, M, count=512, capacity=0, plain_every=16):
+    """Config 3 as RHF solves: `count` def2-SVP molecules (alkanes and water
+    clusters, seed 3, Cartesian) solved together by the server (continuous
+    batching over `capacity` slots, all by default), against the plain
+    single-molecule driver on a sample of the same molecules (indices
+    i % plain_every in {0, 1}: alkanes and water clusters alike).  Both run the
+    reference's defaults (ADIIS->CDIIS, tol 1e-10, FP64, full Fock builds)."""
+    mols = M.batch_config3(count, seed=3)
+    text = M.basis_text("def2-svp")
+    bases = [hf.Basis.from_g94(text, m.symbols, m.xyz) for m in mols]
+    kw = dict(tol=1e-10, conv="adiissoscf")
+    srv = hf.Server(capacity=capacity or count)
+    for b, m in zip(bases, mols):
+        srv.submit([b], [m.zxyz], **kw)
+    t0 = time.perf_counter()
+    srv.run()
+    dt = time.perf_counter() - t0
+    res = [srv.result(i) for i in range(count)]
+    st = srv.stats()
+    srv.close()
+    progress("batched RHF: %d solves in %.2f s" % (count, dt))
+    sample = [i for i in range(count) if i % plain_every in (0, 1)]
+    t0 = time.perf_counter()
+    plain = {i: hf.scf_rhf(bases[i], mols[i].zxyz, **kw) for i in sample}
+    tp = time.perf_counter() - t0
+    progress("plain RHF: %d solves in %.2f s" % (len(sample), tp))
+    de = [res[i]["steps_result"][0]["energy"] - plain[i]["total_energy"] for i in sample]
+    dit = [res[i]["steps_result"][0]["iterations"] - plain[i]["iterations"] for i in sample]
+    lat = [r["finished_s"] - r["arrival_s"] for r in res]
+    solves = count / dt
+    plain_solves = len(sample) / tp
+    return {"workload": "%d RHF solves def2-SVP (alkanes C1-C12, water clusters 1-8; seed 3), Cartesian n %d..%d, "
+                        "ADIIS->CDIIS tol 1e-10, FP64, full Fock builds" % (count, min(b.n for b in bases),
+                                                                         max(b.n for b in bases)),
+            "solves_per_s": solves, "seconds": dt, "slots": capacity or count,
+            "all_converged": all(r["converged"] for r in res),
+            "solve_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+            "fused_fock_builds": st["fused_builds"], "seconds_setup": st["seconds_setup"],
+            "plain_gpu": {"solves_per_s": plain_solves, "sample": len(sample),
+                          "sample_rule": "molecules with index %% %d in {0, 1}" % plain_every},
+            "speedup_vs_plain": solves / plain_solves,
+            "max_abs_dE_vs_plain": float(np.abs(de).max()), "max_abs_diter_vs_plain": int(np.abs(dit).max())}
+
+
+def config5_stream(hf, M, n_req, lam, steps, seed=5):
+    """The served workload: requests with Poisson arrivals (rate lam), each a
+    `steps`-step geometry trajectory (seeded Gaussian displacements, sigma
+    0.01 bohr per coordinate and step) of an alkane (n ~ U{1..20}) or a water
+    cluster (m ~ U{1..32}), half and half, in STO-3G, 6-31G* or cc-pVDZ."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out, t = [], 0.0
+    for i in range(n_req):
+        t += rng.exponential(1.0 / lam)
+        if rng.random() < 0.5:
+            mol = M.alkane(int(rng.integers(1, 21)))
+        else:
+            mol = M.water_cluster_m(int(rng.integers(1, 33)), seed=int(rng.integers(0, 2 ** 31)))
+        basis = ["sto-3g", "6-31g*", "cc-pvdz"][int(rng.integers(0, 3))]
+        geoms = [mol.xyz]
+        for _ in range(steps - 1):
+            geoms.append(geoms[-1] + rng.normal(0.0, 0.01, mol.xyz.shape))
+        out.append((t, mol, basis, geoms))
+    return out
+
+
+def serve_report(hf, M, n_req=16, lam=20.0, steps=10, capacity=16):
+    stream = config5_stream(hf, M, n_req, lam, steps)
+    kw = dict(tol=1e-10, conv="adiissoscf")
+    reqs = []
+    for t, mol, basis, geoms in stream:
+        text = M.basis_text(basis)
+        mols = [M.Molecule(mol.symbols, g) for g in geoms]
+        reqs.append((t, [hf.Basis.from_g94(text, m.symbols, m.xyz) for m in mols], [m.zxyz for m in mols], mol, basis))
+    srv = hf.Server(capacity=capacity)
+    for t, bases, zs, _, _ in reqs:
+        srv.submit(bases, zs, arrival=t, **kw)
+    t0 = time.perf_counter()
+    srv.run()
+    dt = time.perf_counter() - t0
+    res = [srv.result(i) for i in range(len(reqs))]
+    srv.close()
+    lat = [r["finished_s"] - r["arrival_s"] for r in res]
+    solves = sum(r["steps"] for r in res)
+    progress("served: %d requests (%d solves) in %.2f s" % (len(reqs), solves, dt))
+    # The plain path: one request after another, each step solved alone from
+    # the previous step's density (hfg_scf_rhf_guess); latency from the
+    # queue of a single worker fed by the same arrivals.
+    t_free, plat, e_err, ptime = 0.0, [], 0.0, 0.0
+    for (t, bases, zs, _, _), r in zip(reqs, res):
+        s0 = time.perf_counter()
+        guess = None
+        for k, (b, z) in enumerate(zip(bases, zs)):
+            o = hf.scf_rhf(b, z, guess=guess, want_matrices=True, **kw)
+            guess = o["density"]
+            e_err = max(e_err, abs(o["total_energy"] - r["steps_result"][k]["energy"]))
+        svc = time.perf_counter() - s0
+        ptime += svc
+        start = max(t, t_free)
+        t_free = start + svc
+        plat.append(t_free - t)
+    progress("plain served: %.2f s of solves" % ptime)
+    return {"workload": "%d requests x %d geometry steps, Poisson %.0f req/s (seed 5): alkanes C1-C20 / water clusters "
+                        "1-32 in STO-3G/6-31G*/cc-pVDZ, n %d..%d" % (len(reqs), steps, lam,
+                                                                   min(r["n"] for r in res), max(r["n"] for r in res)),
+            "slots": capacity, "solves_per_s": solves / dt, "seconds": dt,
+            "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+            "all_converged": all(r["converged"] for r in res),
+            "plain_gpu": {"solves_per_s": solves / ptime,
+                          "latency_ms": {"p50": 1e3 * float(np.percentile(plat, 50)),
+                                         "p99": 1e3 * float(np.percentile(plat, 99))},
+                          "rule": "requests in arrival order, one at a time, each step warm-started"},
+            "speedup_vs_plain": (solves / dt) / (solves / ptime), "max_abs_dE_vs_plain": e_err,
+            "gradient": "not computed (energy only)"}
+
+
+def summarize_profiles(trace_csv, pmc_dirs, copy_dirs, out_json, workload, commands):
+    """Turns rocprofv3 CSV output into the per-kernel summary bench.py reads:
+    average duration (kernel trace), HBM bytes per dispatch (FETCH_SIZE and
+    WRITE_SIZE in KiB, scaled by the same counters on a copy kernel of known
+    size) and executed FP64 FLOPs per dispatch (SQ instruction counters x 64
+    lanes, FMA = 2 FLOPs)."""
+    import csv
+    import glob
+
+    def rows(d, pat):
+        out = []
+        for f in glob.glob(os.path.join(d, "**", pat), recursive=True):
+            out += list(csv.DictReader(open(f)))
+        return out
+
+    def short(name):
+        return name.replace(" ", "").split("(")[0].replace("hfg::", "")
+    per = {}
+    for r in rows(trace_csv, "*kernel_stats.csv"):
+        per.setdefault(short(r["Name"]), {})["avg_us"] = float(r["AverageNs"]) / 1e3
+    cal = {}
+    for ctr, d in copy_dirs.items():
+        v = [float(r["Counter_Value"]) for r in rows(d, "*counter_collection.csv")
+             if "copy_calibration" in r["Kernel_Name"] and r["Counter_Name"] == ctr]
+        cal[ctr] = (2 ** 30 / 1024.0) / max(v) if v else 1.0  # a 1 GiB copy reads and writes 1 GiB
+    sums = {}
+    for d in pmc_dirs:
+        for r in rows(d, "*counter_collection.csv"):
+            k = short(r["Kernel_Name"])
+            sums.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    for k, ctrs in sums.items():
+        e = per.setdefault(k, {})
+        avg = {c: sum(v) / len(v) for c, v in ctrs.items()}
+        if "FETCH_SIZE" in avg:
+            e["fetch_bytes"] = avg["FETCH_SIZE"] * 1024 * cal.get("FETCH_SIZE", 1.0)
+        if "WRITE_SIZE" in avg:
+            e["write_bytes"] = avg["WRITE_SIZE"] * 1024 * cal.get("WRITE_SIZE", 1.0)
+        f64 = {c: avg[c] for c in avg if c.startswith("SQ_")}
+        if "SQ_INSTS_VALU_FMA_F64" in f64:
+            f64["f64_flops"] = 64.0 * (2 * f64["SQ_INSTS_VALU_FMA_F64"] + f64.get("SQ_INSTS_VALU_MUL_F64", 0) +
+                                       f64.get("SQ_INSTS_VALU_ADD_F64", 0) + f64.get("SQ_INSTS_VALU_TRANS_F64", 0))
+            e["executed"] = f64
+    json.dump({"workload": workload, "copy_calibration": cal, "commands": commands, "per_kernel": per},
+              open(out_json, "w"), indent=1)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-stride", type=int, default=24)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-scf", action="store_true")
-    ap.add_argument("--solve-repeats", type=int, default=8, help="warm SCF solves for the solve-latency percentiles")
-    ap.add_argument("--batch", action="store_true",
-                    help="also time the batched multi-molecule workload (off: see DESIGN.md, batches)")
+    ap.add_argument("--steps", type=int, default=5, help="timed fused Fock builds of the batch")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--only-fused", action="store_true", help="only the fused Fock builds (profiling runs)")
+    ap.add_argument("--no-solves", action="store_true", help="skip the batched SCF solves (the headline)")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--no-fock", action="store_true", help="skip the (H2O)16 single-molecule measurements")
+    ap.add_argument("--fock-steps", type=int, default=20)
+    ap.add_argument("--solve-repeats", type=int, default=5)
     ap.add_argument("--no-df", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="run only the counter-calibration copy kernel over BYTES bytes and exit")
+    ap.add_argument("--summarize-profiles", nargs=3, metavar=("TRACE_DIR", "PMC_DIRS", "COPY_DIRS"),
+                    help="write the per-kernel profile summary from rocprofv3 CSV directories "
+                         "(PMC_DIRS comma-separated; COPY_DIRS as FETCH_SIZE=dir,WRITE_SIZE=dir) and exit")
+    ap.add_argument("--profile-out", default=PROFILE)
     args = ap.parse_args()
+    if args.summarize_profiles:
+        tdir, pdirs, cdirs = args.summarize_profiles
+        copy = dict(x.split("=", 1) for x in cdirs.split(",") if x)
+        summarize_profiles(tdir, pdirs.split(","), copy, args.profile_out,
+                           "fused J/K of the %d-molecule def2-SVP batch (seed 3), per launch" % args.batch_count,
+                           {"trace": tdir, "pmc": pdirs, "copy": cdirs})
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this engine's workloads are single-GPU (see DESIGN.md, sharding); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this engine's workloads are single-GPU (DESIGN.md describes how the path "
+              "would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     hf = load_pkg()
     sys.path.insert(0, PKG)
@@ -169,120 +485,38 @@ def main():
     if hf.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    mol = M.water_cluster((4, 2, 2), seed=2)
-    text = M.basis_text("cc-pvdz")
-    basis = hf.Basis.from_g94(text, mol.symbols, mol.xyz)
-    t0 = time.perf_counter()
-    ctx = hf.Context(basis)
-    setup_s = time.perf_counter() - t0
-    info = ctx.info()
-    rng = np.random.default_rng(5)
-    c = rng.standard_normal((basis.n, 40)) * 0.2
-    dens = c @ c.T  # symmetric positive semidefinite; work does not depend on values
-    ctx.set_density(dens)
-    for _ in range(args.warmup):
-        ctx.jk_resident()
-    ctx.profile(True)
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        ctx.jk_resident()  # returns after the build completes on the device
-        lat.append(time.perf_counter() - s0)
-    elapsed = time.perf_counter() - t0
-    ctx.profile(False)  # keeps the accumulated statistics until the next enable
-    stats = ctx.class_stats()
-    total_flops, prim_q = ctx.fock_flops()
-    value = args.steps / elapsed
-    ms_classes = sum(s["ms"] for s in stats) / args.steps
-    # Dominant kernel: the class carrying the most algorithmic work (the
-    # high-L classes run on a second stream, where event times include the
-    # overlap with the main stream's classes).
-    dom = max(stats, key=lambda s: s["flops"])
-    dom_ms = dom["ms"] / max(1, dom["launches"])
-    dom_tflops = dom["flops"] / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
-    ms_step = 1e3 * elapsed / args.steps
-    whole_tflops = total_flops / (ms_step * 1e-3) / 1e12
-    # The J/K kernels keep K rows in LDS (k_jk_rows) unless HFG_JK_ROWS=0.
-    kname = ("k_jk_rows<%d, %d, %d, %d>" if os.environ.get("HFG_JK_ROWS", "1") != "0"
-             else "k_jk<%d, %d, %d, %d>") % tuple(dom["l"])
-    traffic, traffic_src, executed = None, None, None
-    tpath = os.path.join(ROOT, "profiles", "traffic_jk_h2o16_ccpvdz.json")
-    if os.path.exists(tpath):
-        t = json.load(open(tpath))["per_kernel"].get(kname.replace(" ", ""))
-        if t and "fetch_bytes" in t:
-            traffic = t["fetch_bytes"] + t["write_bytes"]
-            traffic_src = "profiles/traffic_jk_h2o16_ccpvdz.json (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
-        if t and "executed" in t and dom_ms > 0:
-            # FP64 work the kernel executes (SQ instruction counters x 64 lanes)
-            # over this run's launch time: the hardware utilisation, where the
-            # algorithmic figure above counts the reference's per-primitive
-            # work, which general contraction avoids in part.
-            ef = t["executed"]["f64_flops"]
-            executed = {"flops_per_launch": ef, "achieved": ef / (dom_ms * 1e-3) / 1e12,
-                        "frac": ef / (dom_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-                        "source": "profiles/traffic_jk_h2o16_ccpvdz.json (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64)"}
-    line = {
-        "metric": "Fock builds/s (direct FP64 J+K, |dE|<=1e-8 Eh vs FP64 oracle)",
-        "value": value, "unit": "Fock builds/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded water cluster geometry)",
-        "config": {"workload": "direct RHF/cc-pVDZ (H2O)16, 4x2x2 lattice 2.9 A, seed 2, Schwarz 1e-12",
-                   "n_cart": basis.n, "n_sph": 384, "shell_quartets": info["n_quartets"],
-                   "primitive_quartets": prim_q, "parallelism": "single GPU"},
-        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
-        "setup_s": setup_s,
-        "roofline": {"bound": "fp64-valu", "kernel": kname,
-                     "achieved": dom_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": dom_tflops / PEAK_FP64_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
-                     "flops_per_launch": dom["flops"], "ms_per_launch": dom_ms,
-                     "flop_model": "MD per primitive quartet of each contracted quartet (the reference's loops)",
-                     "executed": executed},
-        "fock_build_roofline": {"achieved": whole_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                                "frac": whole_tflops / PEAK_FP64_TFLOPS, "algorithmic_flops": total_flops,
-                                "wall_ms": ms_step, "class_kernel_ms_summed": ms_classes},
-        "classes": [{"l": s["l"], "ms": s["ms"] / max(1, s["launches"]), "gflop": s["flops"] / 1e9,
-                     "quartets": s["quartets"]} for s in stats if s["quartets"]],
-    }
-    if not args.no_scf:
-        # The first solve in a process also pays one-time costs (library
-        # initialisation, the per-element SAD cache); the second is what a
-        # serving process sees.  Both are reported, solves/s from the second.
-        cold = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
-        full = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True)
-        r = hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True, incremental=True)
-        # Solve latency over repeated warm solves (the same molecule, each a
-        # complete solve from the geometry: set-up, guess, iterations).
-        sl = [r["seconds_total"]]
-        for _ in range(args.solve_repeats - 1):
-            sl.append(hf.scf_rhf(basis, mol.zxyz, tol=1e-10, conv="adiissoscf", spherical=True,
-                                 incremental=True)["seconds_total"])
-        line["scf"] = {"solves_per_s": 1.0 / r["seconds_total"], "seconds": r["seconds_total"],
-                       "solve_latency_ms": {"p50": 1e3 * float(np.percentile(sl, 50)),
-                                            "p99": 1e3 * float(np.percentile(sl, 99)), "solves": len(sl)},
-                       "cold_seconds": cold["seconds_total"],
-                       "iterations": r["iterations"], "converged": r["converged"], "fock_builds": r["fock_builds"],
-                       "energy": r["total_energy"], "cold_energy": cold["total_energy"],
-                       "phases_s": {k: r["seconds_" + k] for k in ("setup", "one_electron", "guess", "fock",
-                                                                   "linalg")},
-                       "cold_phases_s": {k: cold["seconds_" + k] for k in ("setup", "one_electron", "guess",
-                                                                           "fock", "linalg")},
-                       "fock": "incremental (J/K of the density change inside the iterations; full first and "
-                               "final builds)",
-                       "full_build_solve": {"seconds": full["seconds_total"], "iterations": full["iterations"],
-                                            "energy": full["total_energy"]}}
-        ref = os.path.join(ROOT, "tests", "golden", "h2o16_ccpvdz_scf.json")
-        if os.path.exists(ref):
-            o = json.load(open(ref))
-            line["scf"]["dE_vs_oracle"] = r["total_energy"] - o["total_energy"]
-            line["scf"]["full_build_solve"]["dE_vs_oracle"] = full["total_energy"] - o["total_energy"]
-            line["scf"]["oracle_iterations"] = o["iterations"]
-    if args.batch:
-        line["batch"] = batch_report(hf, M, args.batch_count)
+    fused, roof = fused_report(hf, M, args.batch_count, args.steps, args.warmup)
+    progress("fused builds: %.1f ms per batch build" % fused["ms_per_batch_build"])
+    line = {"metric": "Fock builds/s & SCF solves/s at |dE|<=1e-8 Eh vs plain GPU; p50/p99 solve latency",
+            "value": None, "unit": "SCF solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": fused["ms_per_batch_build"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded geometries: all-trans alkanes, water clusters)",
+            "config": {"workload": "batched RHF: %d def2-SVP solves (alkanes C1-C12 / water clusters 1-8, seed 3), "
+                                   "FP64; a step = one fused Fock build of the whole batch" % args.batch_count,
+                       "molecules": args.batch_count, "basis": "def2-SVP (Cartesian)", "parallelism": "single GPU"},
+            "roofline": roof, "fused_fock_builds": fused}
+    if args.only_fused:
+        print(json.dumps(line), flush=True)
+        return
+    if not args.no_solves:
+        sb = scf_batch_report(hf, M, args.batch_count)
+        line["value"] = sb["solves_per_s"]
+        line["batched_solves"] = sb
+        progress("batched solves: %.2f solves/s (plain %.2f)" % (sb["solves_per_s"],
+                                                                 sb["plain_gpu"]["solves_per_s"]))
+    if not args.no_serve:
+        line["served"] = serve_report(hf, M, args.serve_requests, args.serve_rate)
+        progress("served stream done")
+    if not args.no_fock:
+        line["h2o16_ccpvdz"] = fock_report(hf, M, args.fock_steps, 3, args.solve_repeats, scf=True)
+        progress("(H2O)16 done")
     if not args.no_df:
         line["df"] = df_report(hf, M)
+        progress("DF done")
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(mol, text, args.cpu_stride, args.cpu_threads, dens)
+        line["cpu_baseline"] = cpu_baseline(M, args.batch_count, args.cpu_threads, args.cpu_seconds)
+        progress("CPU baseline done")
     print(json.dumps(line), flush=True)
 
 

@@ -56,7 +56,28 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_class_stats", "hfg_fock_flops", "hfg_md_flops", "hfg_persistent_blocks", "hfg_df_build",
             "hfg_df_destroy", "hfg_df_info", "hfg_df_jk", "hfg_df_rhf", "hfg_calibration_copy",
             "hfg_batch_create", "hfg_batch_destroy", "hfg_batch_info", "hfg_batch_set_density", "hfg_batch_jk",
-            "hfg_batch_get_jk", "hfg_batch_jk_each", "hfg_df_jk_ex"]
+            "hfg_batch_get_jk", "hfg_batch_jk_each", "hfg_df_jk_ex", "hfg_contract_jk_multi", "hfg_server_create",
+            "hfg_server_destroy", "hfg_server_submit", "hfg_server_run", "hfg_server_result", "hfg_server_stats",
+            "hfg_scf_rhf_guess", "hfg_eri_mo", "hfg_batch_profile", "hfg_batch_class_stats"]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("capacity", C.c_int32), ("hbm_budget_gb", C.c_double)]
+
+
+class StepResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("iterations", C.c_int32), ("converged", C.c_int32),
+                ("fock_builds", C.c_int32), ("n", C.c_int32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("admitted_s", C.c_double), ("finished_s", C.c_double),
+                ("steps", C.c_int32), ("converged", C.c_int32), ("status", C.c_int32), ("n", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("fused_builds", C.c_int64), ("iterations", C.c_int64), ("seconds", C.c_double),
+                ("seconds_setup", C.c_double), ("seconds_sync", C.c_double), ("hbm_budget_bytes", C.c_double)]
 
 
 class DfhfOptions(C.Structure):
@@ -85,11 +106,15 @@ def lib():
         L.hfg_schwarz.argtypes = [vp, vp]
         L.hfg_shell_quartet.argtypes = [vp, i32, i32, i32, i32, vp]
         L.hfg_eri_packed.argtypes = [vp, vp, C.c_size_t]
+        L.hfg_eri_mo.argtypes = [vp, vp, i32, vp, C.c_size_t]
         L.hfg_contract_jk.argtypes = [vp, vp, vp, vp]
         L.hfg_build_fock.argtypes = [vp, vp, vp, vp]
+        L.hfg_contract_jk_multi.argtypes = [vp, vp, i32, vp, vp, vp]
         L.hfg_contract_jk_device.argtypes = [vp, vp, vp, vp]
         L.hfg_scf_rhf.argtypes = [C.POINTER(BasisView), vp, i32, i32, C.POINTER(ScfOptions), C.POINTER(ScfResult),
                                   vp, vp, vp]
+        L.hfg_scf_rhf_guess.argtypes = [C.POINTER(BasisView), vp, i32, i32, C.POINTER(ScfOptions), vp,
+                                        C.POINTER(ScfResult), vp, vp, vp]
         L.hfg_basis_from_g94.argtypes = [C.c_char_p, C.c_char_p, vp, i32, C.POINTER(vp)]
         L.hfg_basis_free.argtypes = [vp]
         L.hfg_basis_get_view.argtypes = [vp, C.POINTER(BasisView)]
@@ -118,6 +143,16 @@ def lib():
         L.hfg_batch_jk.argtypes = [vp]
         L.hfg_batch_jk_each.argtypes = [vp]
         L.hfg_batch_get_jk.argtypes = [vp, i32, vp, vp]
+        L.hfg_batch_profile.argtypes = [vp, i32]
+        L.hfg_batch_class_stats.argtypes = [vp, i32, vp, C.POINTER(dbl), C.POINTER(C.c_int64), C.POINTER(dbl),
+                                            C.POINTER(C.c_int64)]
+        L.hfg_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(vp)]
+        L.hfg_server_destroy.argtypes = [vp]
+        L.hfg_server_submit.argtypes = [vp, dbl, i32, vp, vp, i32, i32, C.POINTER(ScfOptions), i32,
+                                        C.POINTER(C.c_int)]
+        L.hfg_server_run.argtypes = [vp]
+        L.hfg_server_result.argtypes = [vp, i32, C.POINTER(RequestResult), vp, vp]
+        L.hfg_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
         _LIB = L
     return _LIB
 
@@ -129,6 +164,35 @@ def _check(status):
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _square(a, n, name):
+    """a as a C-contiguous float64 (n, n) array; ValueError otherwise (the C
+    side reads exactly n*n doubles)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != (n, n):
+        raise ValueError("%s must have shape (%d, %d), got %s" % (name, n, n, a.shape))
+    return a
+
+
+def _columns(a, n, name):
+    """a as a C-contiguous float64 (n, k) array with k >= 1."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2 or a.shape[0] != n or a.shape[1] < 1:
+        raise ValueError("%s must have shape (%d, k) with k >= 1, got %s" % (name, n, a.shape))
+    return a
+
+
+def _atoms(zxyz):
+    """Nuclear charges and positions as a (natom, 4) float64 array."""
+    z = np.ascontiguousarray(np.asarray(zxyz, dtype=np.float64))
+    if z.ndim == 1:
+        if z.size % 4:
+            raise ValueError("zxyz must hold (Z, x, y, z) rows, got %d values" % z.size)
+        z = z.reshape(-1, 4)
+    if z.ndim != 2 or z.shape[1] != 4 or z.shape[0] < 1:
+        raise ValueError("zxyz must have shape (natom, 4), got %s" % (z.shape,))
+    return np.ascontiguousarray(z)
 
 
 def md_flops(la, lb, lc, ld):
@@ -211,7 +275,9 @@ class Batch:
         return dict(count=v[0].value, n_max=v[1].value, quartets=v[2].value, flops=f.value)
 
     def set_density(self, mol, p):
-        p = np.ascontiguousarray(p, dtype=np.float64)
+        if not 0 <= mol < len(self.bases):
+            raise ValueError("molecule index %d out of range" % mol)
+        p = _square(p, self.bases[mol].n, "density")
         _check(lib().hfg_batch_set_density(self.h, mol, _p(p)))
 
     def jk(self):
@@ -222,12 +288,106 @@ class Batch:
         """The same builds molecule after molecule (unbatched launches)."""
         _check(lib().hfg_batch_jk_each(self.h))
 
+    def profile(self, on=True):
+        _check(lib().hfg_batch_profile(self.h, int(on)))
+
+    def class_stats(self):
+        out = []
+        for c in range(21):
+            l4 = np.zeros(4, np.int32)
+            fl, ms = C.c_double(), C.c_double()
+            q, nl = C.c_int64(), C.c_int64()
+            _check(lib().hfg_batch_class_stats(self.h, c, _p(l4), C.byref(fl), C.byref(q), C.byref(ms),
+                                               C.byref(nl)))
+            out.append(dict(cls=c, l=tuple(int(x) for x in l4), flops=fl.value, quartets=q.value, ms=ms.value,
+                            launches=nl.value))
+        return out
+
     def get_jk(self, mol):
         n = self.bases[mol].n
         j = np.zeros((n, n))
         k = np.zeros((n, n))
         _check(lib().hfg_batch_get_jk(self.h, mol, _p(j), _p(k)))
         return j, k
+
+
+def scf_options(max_iter=100, tol=1e-10, residual=None, density=None, conv="adiissoscf", spherical=False,
+                verbose=False, incremental=False):
+    """hfg_scf_options with the derived tolerances of run_scf_in_space
+    (src/scf.rs:384-400)."""
+    if residual is None:
+        residual = max(math.sqrt(tol) * 1e-2, 1e-11)
+    if density is None:
+        density = min(max(math.sqrt(tol) * 0.1, 1e-12), 1e-10)
+    key = conv.replace("-", "").replace("_", "").lower()
+    if key not in CONV:
+        raise ValueError("unknown convergence algorithm %r" % conv)
+    return ScfOptions(max_iter, tol, residual, density, CONV[key], int(spherical), int(verbose), int(incremental))
+
+
+class Server:
+    """Served / batched RHF (hfg_server_*): requests of geometry trajectories,
+    admitted into `capacity` resident slots as they arrive and slots free up,
+    every iteration of all slots with one fused Fock-build launch per class."""
+
+    def __init__(self, capacity, hbm_budget_gb=0.0):
+        self.h = C.c_void_p()
+        self.opt = ServerOptions(int(capacity), float(hbm_budget_gb))
+        _check(lib().hfg_server_create(C.byref(self.opt), C.byref(self.h)))
+        self.requests = []
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value and _LIB is not None:
+            _LIB.hfg_server_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, bases, zxyzs, charge=0, arrival=0.0, keep_density=False, **scf_kw):
+        """Queues one request: a trajectory of len(bases) geometries of one
+        molecule (bases[k] built at zxyzs[k]); returns its id."""
+        bases = list(bases)
+        atoms = [_atoms(z) for z in zxyzs]
+        if not bases or len(bases) != len(atoms):
+            raise ValueError("one basis per geometry step is required")
+        natom = atoms[0].shape[0]
+        if any(a.shape[0] != natom for a in atoms):
+            raise ValueError("every step must have the same atoms")
+        views = (BasisView * len(bases))(*[b.view for b in bases])
+        zp = (C.c_void_p * len(atoms))(*[a.ctypes.data for a in atoms])
+        opt = scf_options(**scf_kw)
+        rid = C.c_int()
+        _check(lib().hfg_server_submit(self.h, float(arrival), len(bases), C.cast(views, C.c_void_p),
+                                       C.cast(zp, C.c_void_p), natom, int(charge), C.byref(opt), int(keep_density),
+                                       C.byref(rid)))
+        self.requests.append((rid.value, [b.n for b in bases], bases, atoms))
+        return rid.value
+
+    def run(self):
+        _check(lib().hfg_server_run(self.h))
+
+    def result(self, rid, density=False):
+        n_steps = len(self.requests[rid][1])
+        r = RequestResult()
+        steps = (StepResult * n_steps)()
+        n_last = self.requests[rid][1][-1]
+        p = np.zeros((n_last, n_last)) if density else None
+        st = lib().hfg_server_result(self.h, rid, C.byref(r), C.cast(steps, C.c_void_p),
+                                     _p(p) if p is not None else None)
+        out = {k: getattr(r, k) for k, _ in RequestResult._fields_}
+        out["steps_result"] = [{k: getattr(x, k) for k, _ in StepResult._fields_} for x in steps]
+        out["ok"] = st == HFG_OK
+        if st != HFG_OK:
+            out["error"] = lib().hfg_last_error().decode()
+        if density:
+            out["density"] = p
+        return out
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().hfg_server_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
 
 
 class Context:
@@ -268,22 +428,47 @@ class Context:
         _check(lib().hfg_eri_packed(self.h, _p(out), out.size))
         return out
 
+    def eri_mo(self, c):
+        """(pq|rs) in the MO basis of the columns of c (n x n_mo), packed
+        [rs][pq] with compound indices p >= q (transform_eri_ao_to_mo,
+        src/mo_integrals.rs:156-265)."""
+        c = _columns(c, self.n, "MO coefficients")
+        nmo = c.shape[1]
+        mp = nmo * (nmo + 1) // 2
+        out = np.zeros(mp * mp)
+        _check(lib().hfg_eri_mo(self.h, _p(c), nmo, _p(out), out.size))
+        return out
+
     def contract_jk(self, p):
-        p = np.ascontiguousarray(p, dtype=np.float64)
+        p = _square(p, self.n, "density")
         j = np.zeros((self.n, self.n))
         k = np.zeros((self.n, self.n))
         _check(lib().hfg_contract_jk(self.h, _p(p), _p(j), _p(k)))
         return j, k
 
+    def contract_jk_multi(self, total, spins):
+        """J of the total density and K of each spin density (contract_jk(total,
+        spins), src/scf.rs:337-364).  A spin density that is the total object
+        itself (RHF) shares its build."""
+        tot = _square(total, self.n, "total density")
+        arrs = [tot if s is total else _square(s, self.n, "spin density") for s in spins]
+        j = np.zeros((self.n, self.n))
+        ks = [np.zeros((self.n, self.n)) for _ in arrs]
+        pin = (C.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+        kout = (C.c_void_p * max(1, len(ks)))(*[k.ctypes.data for k in ks])
+        _check(lib().hfg_contract_jk_multi(self.h, _p(tot), len(arrs), C.cast(pin, C.c_void_p), _p(j),
+                                           C.cast(kout, C.c_void_p)))
+        return j, ks
+
     def build_fock(self, h, p):
-        h = np.ascontiguousarray(h, dtype=np.float64)
-        p = np.ascontiguousarray(p, dtype=np.float64)
+        h = _square(h, self.n, "core Hamiltonian")
+        p = _square(p, self.n, "density")
         f = np.zeros((self.n, self.n))
         _check(lib().hfg_build_fock(self.h, _p(h), _p(p), _p(f)))
         return f
 
     def set_density(self, p):
-        p = np.ascontiguousarray(p, dtype=np.float64)
+        p = _square(p, self.n, "density")
         _check(lib().hfg_set_density(self.h, _p(p)))
 
     def jk_resident(self):
@@ -340,7 +525,7 @@ class DfTensor:
         self.close()
 
     def coulomb_exchange_rhf(self, cocc):
-        cocc = np.ascontiguousarray(cocc, dtype=np.float64)
+        cocc = _columns(cocc, self.n, "occupied coefficients")
         j = np.zeros((self.n, self.n))
         k = np.zeros((self.n, self.n))
         _check(lib().hfg_df_jk(self.h, _p(cocc), cocc.shape[1], _p(j), _p(k)))
@@ -351,7 +536,9 @@ class DfTensor:
     def jk_precision(self, cocc, precision="fp64"):
         """J (FP64) and K with the exchange contraction in the given precision;
         also returns the device milliseconds of the contraction."""
-        cocc = np.ascontiguousarray(cocc, dtype=np.float64)
+        if precision not in self.PRECISION:
+            raise ValueError("precision must be one of %s" % sorted(self.PRECISION))
+        cocc = _columns(cocc, self.n, "occupied coefficients")
         j = np.zeros((self.n, self.n))
         k = np.zeros((self.n, self.n))
         ms = C.c_double()
@@ -375,7 +562,7 @@ def df_rhf(basis, aux, zxyz, charge=0, max_iter=100, e_tol=1e-12, r_tol=1e-10, d
         raise ValueError("k_precision must be one of %s" % sorted(DF_PRECISION))
     opt = DfhfOptions(max_iter, e_tol, r_tol, d_tol, metric_cutoff, DF_PRECISION[k_precision], switch_residual)
     res = DfhfResult()
-    zxyz = np.ascontiguousarray(zxyz, dtype=np.float64)
+    zxyz = _atoms(zxyz)
     n = basis.n
     p, f, j, k = (np.zeros((n, n)) for _ in range(4))
     _check(lib().hfg_df_rhf(C.byref(basis.view), C.byref(aux.view), _p(zxyz), len(zxyz), charge, C.byref(opt),
@@ -389,25 +576,25 @@ CONV = {"adiissoscf": 0, "default": 0, "cdiis": 1, "diis": 1, "pulay": 1, "adiis
 
 
 def scf_rhf(basis, zxyz, charge=0, max_iter=100, tol=1e-10, residual=None, density=None, conv="adiissoscf",
-            spherical=False, verbose=False, want_matrices=False, incremental=False):
+            spherical=False, verbose=False, want_matrices=False, incremental=False, guess=None):
     """RHF with the derived tolerances of run_scf_in_space (src/scf.rs:384-400).
     incremental: J/K of the density change inside the iterations (full first
-    build and final rebuild)."""
-    if residual is None:
-        residual = max(math.sqrt(tol) * 1e-2, 1e-11)
-    if density is None:
-        density = min(max(math.sqrt(tol) * 0.1, 1e-12), 1e-10)
-    key = conv.replace("-", "").replace("_", "").lower()
-    opt = ScfOptions(max_iter, tol, residual, density, CONV[key], int(spherical), int(verbose), int(incremental))
+    build and final rebuild).  guess: initial density instead of SAD
+    (run_scf_with_initial_density, src/scf.rs:630-639)."""
+    opt = scf_options(max_iter, tol, residual, density, conv, spherical, verbose, incremental)
     res = ScfResult()
-    zxyz = np.ascontiguousarray(zxyz, dtype=np.float64)
+    zxyz = _atoms(zxyz)
     n = basis.n
+    g = None if guess is None else _square(guess, n, "initial density")
     p = np.zeros((n, n)) if want_matrices else None
     f = np.zeros((n, n)) if want_matrices else None
     eps = np.zeros(n) if want_matrices else None
-    st = lib().hfg_scf_rhf(C.byref(basis.view), _p(zxyz), len(zxyz), charge, C.byref(opt), C.byref(res),
-                           _p(p) if p is not None else None, _p(f) if f is not None else None,
-                           _p(eps) if eps is not None else None)
+    outs = (_p(p) if p is not None else None, _p(f) if f is not None else None, _p(eps) if eps is not None else None)
+    if g is None:
+        st = lib().hfg_scf_rhf(C.byref(basis.view), _p(zxyz), len(zxyz), charge, C.byref(opt), C.byref(res), *outs)
+    else:
+        st = lib().hfg_scf_rhf_guess(C.byref(basis.view), _p(zxyz), len(zxyz), charge, C.byref(opt), _p(g),
+                                     C.byref(res), *outs)
     if st not in (HFG_OK, HFG_NOT_CONVERGED):
         _check(st)
     out = {k: getattr(res, k) for k, _ in ScfResult._fields_}

@@ -1,252 +1,194 @@
-// batch.hip -- Fock builds of many molecules in fused launches.
-//
-// Each molecule keeps its own engine (shell pairs, Schwarz bounds, J/K plan,
-// density and accumulators in HBM).  The batch concatenates the molecules'
-// plans into one view table, one ket-group table and, per quartet class, one
-// task and job list sorted by cost across molecules; every view carries its
-// molecule's index, through which the J/K kernels reach that molecule's
-// density, accumulators, AO factors and size (MolDev).  One build of the whole
-// batch is 21 class launches (two streams), one clearing and one
-// symmetrisation launch, whatever the number of molecules: quartets of uneven
-// classes from many small molecules fill the chip together, where molecule by
-// molecule each launch would be nearly empty.  The reference builds each
-// molecule's Fock matrix on its own (src/scf.rs:366-369, via
-// src/shell_integrals.rs:156-240 and src/scf.rs:337-364); the results per
-// molecule are the same.
+// batch.hip -- the resident molecule set (fleet.h) and the hfg_batch_* entry
+// points: Fock builds of many molecules in fused launches.
 #include <algorithm>
 #include <cstring>
 #include <memory>
 
 #include "../../include/hfg.h"
-#include "engine.h"
+#include "fleet.h"
 
 namespace hfg {
 
-__global__ void k_batch_clear(const MolDev* __restrict__ mols, int count) {
+__global__ void k_zero_int(int* p, int n);
+
+// Clears the per-XCC accumulator copies of the active slots (grid (x, count)).
+__global__ void k_fleet_clear(const MolTab* __restrict__ tabs, const int* __restrict__ slots, int count) {
   const int m = blockIdx.y;
   if (m >= count) return;
-  const MolDev md = mols[m];
-  const size_t len = md.nn * NXCC_SLOTS;
+  const MolTab& t = tabs[slots[m]];
+  double* J = t.Jacc;
+  double* K = t.Kacc;
+  const size_t len = t.nn * NXCC_SLOTS;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
-    md.Jacc[i] = 0.0;
-    md.Kacc[i] = 0.0;
+    J[i] = 0.0;
+    K[i] = 0.0;
   }
 }
 
-// J = sum over the XCC copies of Jacc + Jacc^T, K likewise, per molecule.
-__global__ void k_batch_symmetrize(const MolDev* __restrict__ mols, double* const* __restrict__ Jout,
-                                   double* const* __restrict__ Kout, int count) {
+__global__ void k_fleet_jk_out(const MolTab* __restrict__ tabs, const int* __restrict__ slots,
+                               double* const* __restrict__ jout, double* const* __restrict__ kout, int count) {
   const int m = blockIdx.y;
   if (m >= count) return;
-  const MolDev md = mols[m];
-  const int n = md.n;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < md.nn;
-       idx += (size_t)gridDim.x * blockDim.x) {
+  const MolTab& t = tabs[slots[m]];
+  const int n = t.n;
+  const size_t nn = t.nn;
+  double* J = jout ? jout[m] : nullptr;
+  double* K = kout ? kout[m] : nullptr;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
     const int r = (int)(idx / n), c = (int)(idx % n);
     const size_t tr = (size_t)c * n + r;
     double j = 0.0, k = 0.0;
     for (int x = 0; x < NXCC_SLOTS; ++x) {
-      j += md.Jacc[x * md.nn + idx] + md.Jacc[x * md.nn + tr];
-      k += md.Kacc[x * md.nn + idx] + md.Kacc[x * md.nn + tr];
+      j += t.Jacc[x * nn + idx] + t.Jacc[x * nn + tr];
+      k += t.Kacc[x * nn + idx] + t.Kacc[x * nn + tr];
     }
-    Jout[m][idx] = j;
-    Kout[m][idx] = k;
+    if (J) J[idx] = j;
+    if (K) K[idx] = k;
   }
 }
 
-struct Batch {
-  std::vector<std::unique_ptr<Engine>> eng;
-  hipStream_t stream = nullptr, side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  DevBuf d_views, d_combos, d_groups, d_vb, d_kt, d_kcoef, d_vq, d_tasks, d_jobs, d_counters, d_mols, d_jout, d_kout,
-      d_boys;
-  std::vector<size_t> task_begin, job_begin;
-  size_t vb_size = 0, kt_size = 0;
-  int nmax = 0;
-  int side_blocks = 512;
-  double flops = 0;
-  int64_t quartets = 0;
-
-  explicit Batch(const hfg_basis_view* bases, int count);
-  ~Batch();
-  void jk();
-};
-
-Batch::Batch(const hfg_basis_view* bases, int count) {
-  if (count <= 0) throw HfgError{HFG_INVALID_INPUT, "empty batch"};
-  for (int m = 0; m < count; ++m) {
-    eng.emplace_back(new Engine(bases[m]));
-    nmax = std::max(nmax, eng.back()->n);
-    for (double f : eng.back()->class_flops) flops += f;
-    quartets += eng.back()->n_quartets;
-  }
+Fleet::Fleet(int cap) : capacity(cap) {
+  if (cap <= 0) throw HfgError{HFG_INVALID_INPUT, "a fleet needs at least one slot"};
   HIP_CHECK(hipStreamCreate(&stream));  // default priority
   HIP_CHECK(hipStreamCreate(&side));    // default priority
   HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
   HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   if (const char* env = std::getenv("HFG_SIDE_BLOCKS")) side_blocks = std::max(0, std::atoi(env));
-
-  // concatenated tables
-  std::vector<ViewDev> views;
-  std::vector<ComboDev> combos;
-  std::vector<KetGroup> groups;
-  std::vector<double> brec, kcoef, vq;
-  std::vector<std::vector<double>> kcols(9);
-  std::vector<int> vo(count), go(count);
-  for (int m = 0; m < count; ++m) {
-    const JKPlan& P = eng[m]->plan;
-    vo[m] = (int)views.size();
-    go[m] = (int)groups.size();
-    const int co = (int)combos.size(), bo = (int)(brec.size() / HFG_BRA_REC), ko = (int)kcols[0].size(),
-              cfo = (int)kcoef.size();
-    for (size_t i = 0; i < P.views.size(); ++i) {
-      ViewDev v = P.views[i];
-      v.p0 += bo;
-      v.p1 += bo;
-      v.c0 += co;
-      v.mol = m;
-      views.push_back(v);
-      vq.push_back(P.view_q[i]);
-    }
-    combos.insert(combos.end(), P.combos.begin(), P.combos.end());
-    for (KetGroup g : P.groups) {
-      g.start += vo[m];
-      g.base += ko;
-      g.cbase += cfo;
-      groups.push_back(g);
-    }
-    brec.insert(brec.end(), P.vb.begin(), P.vb.begin() + P.vb_size * HFG_BRA_REC);
-    for (int c = 0; c < 9; ++c)
-      kcols[c].insert(kcols[c].end(), P.kt.begin() + c * P.kt_size, P.kt.begin() + (c + 1) * P.kt_size);
-    kcoef.insert(kcoef.end(), P.kcoef.begin(), P.kcoef.end());
-  }
-  vb_size = brec.size() / HFG_BRA_REC;
-  kt_size = kcols[0].size();
-  std::vector<double> kt(9 * std::max<size_t>(1, kt_size), 0.0);
-  for (int c = 0; c < 9; ++c) std::copy(kcols[c].begin(), kcols[c].end(), kt.begin() + c * kt_size);
-  if (brec.empty()) brec.assign(HFG_BRA_REC, 0.0);
-  if (kcoef.empty()) kcoef.push_back(0.0);
-
-  // per class: the molecules' jobs merged by cost, their tasks renumbered
-  std::vector<JKTask> tasks;
-  std::vector<int> jobs;
-  task_begin.assign(HFG_NCLASS + 1, 0);
-  job_begin.assign(HFG_NCLASS + 1, 0);
-  for (int c = 0; c < HFG_NCLASS; ++c) {
-    task_begin[c] = tasks.size();
-    job_begin[c] = jobs.size();
-    struct J {
-      double cost;
-      int m, j;
-    };
-    std::vector<J> js;
-    for (int m = 0; m < count; ++m) {
-      const JKPlan& P = eng[m]->plan;
-      for (size_t j = P.job_begin[c]; j < P.job_begin[c + 1]; ++j) js.push_back({P.job_cost[j], m, (int)j});
-    }
-    std::stable_sort(js.begin(), js.end(), [](const J& x, const J& y) { return x.cost > y.cost; });
-    for (const J& x : js) {
-      const JKPlan& P = eng[x.m]->plan;
-      jobs.push_back((int)tasks.size());
-      for (int t = P.jobs[x.j]; t < P.jobs[x.j + 1]; ++t) {
-        JKTask k = P.tasks[t];
-        k.bra += vo[x.m];
-        k.ket0 += vo[x.m];
-        k.grp += go[x.m];
-        tasks.push_back(k);
-      }
-    }
-  }
-  task_begin[HFG_NCLASS] = tasks.size();
-  job_begin[HFG_NCLASS] = jobs.size();
-  jobs.push_back((int)tasks.size());
-
-  std::vector<MolDev> mols(count);
-  std::vector<double*> jout(count), kout(count);
-  for (int m = 0; m < count; ++m) {
-    Engine& e = *eng[m];
-    mols[m] = MolDev{e.d_D.as<double>(), e.d_Jacc.as<double>(), e.d_Kacc.as<double>(), e.d_scale.as<double>(),
-                     (size_t)e.n * e.n, e.n, 0};
-    jout[m] = e.d_J.as<double>();
-    kout[m] = e.d_K.as<double>();
-  }
-  auto up = [&](DevBuf& d, const void* src, size_t bytes) {
-    d.alloc(std::max<size_t>(bytes, 16));
-    if (bytes) HIP_CHECK(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream));
-  };
-  up(d_views, views.data(), views.size() * sizeof(ViewDev));
-  up(d_combos, combos.data(), combos.size() * sizeof(ComboDev));
-  up(d_groups, groups.data(), groups.size() * sizeof(KetGroup));
-  up(d_vb, brec.data(), brec.size() * sizeof(double));
-  up(d_kt, kt.data(), kt.size() * sizeof(double));
-  up(d_kcoef, kcoef.data(), kcoef.size() * sizeof(double));
-  up(d_vq, vq.data(), vq.size() * sizeof(double));
-  up(d_tasks, tasks.data(), tasks.size() * sizeof(JKTask));
-  up(d_jobs, jobs.data(), jobs.size() * sizeof(int));
-  up(d_mols, mols.data(), mols.size() * sizeof(MolDev));
-  up(d_jout, jout.data(), jout.size() * sizeof(double*));
-  up(d_kout, kout.data(), kout.size() * sizeof(double*));
+  if (const char* env = std::getenv("HFG_JK_ROWS")) use_rows = std::atoi(env) != 0;
+  tabs.assign(cap, MolTab{});
+  occupied.assign(cap, 0);
+  d_tabs.alloc(cap * sizeof(MolTab));
+  d_active.alloc(cap * sizeof(int));
   d_counters.alloc(HFG_NCLASS * sizeof(int));
-  HIP_CHECK(hipStreamSynchronize(stream));
+  auto tab = boys_table();
+  d_boys.alloc(tab.size() * sizeof(double));
+  HIP_CHECK(hipMemcpyAsync(d_boys.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  sync();
 }
 
-Batch::~Batch() {
+Fleet::~Fleet() {
+  for (auto e : events) (void)hipEventDestroy(e);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-__global__ void k_zero_int(int* p, int n);
-
-void Batch::jk() {
-  const int count = (int)eng.size();
-  // every molecule's density is staged by its engine's stream; order it first
-  for (auto& e : eng) HIP_CHECK(hipStreamSynchronize(e->stream));
-  hipLaunchKernelGGL(k_batch_clear, dim3(64, count), dim3(256), 0, stream, d_mols.as<MolDev>(), count);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, d_counters.as<int>(), HFG_NCLASS);
-  JKArgs a;
-  std::memset(&a, 0, sizeof a);
-  a.n = nmax;  // sizes the LDS rows; everything else per molecule
-  a.views = d_views.as<ViewDev>();
-  a.combos = d_combos.as<ComboDev>();
-  a.groups = d_groups.as<KetGroup>();
-  a.brec = d_vb.as<double>();
-  const double* kb = d_kt.as<double>();
-  const size_t m = kt_size;
-  a.kt = PrimPairs{kb, kb + m, kb + 2 * m, kb + 3 * m, kb + 4 * m, kb + 5 * m, kb + 6 * m, kb + 7 * m, kb + 8 * m};
-  a.kcoef = d_kcoef.as<double>();
-  a.screen = a.screen0 = eng[0]->screen;
-  a.vq = d_vq.as<double>();
-  a.boys = eng[0]->d_boys.as<double>();
-  a.mols = d_mols.as<MolDev>();
-  a.tasks = d_tasks.as<JKTask>();
-  auto on_side = [&](int c) { return side_blocks > 0 && side_class(c); };
-  const bool two = side_blocks > 0;
-  if (two) {
-    HIP_CHECK(hipEventRecord(ev_fork, stream));
-    HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
-  }
-  for (int pass = 0; pass < 2; ++pass)
-    for (int c = 0; c < HFG_NCLASS; ++c) {
-      const size_t nt = task_begin[c + 1] - task_begin[c];
-      if (!nt || on_side(c) != (pass == 0)) continue;
-      a.task0 = (int)task_begin[c];
-      a.jobs = d_jobs.as<int>() + job_begin[c];
-      a.njob = (int)(job_begin[c + 1] - job_begin[c]);
-      a.job_counter = d_counters.as<int>() + c;
-      launch_jk_class(c, a, (int)nt, on_side(c) ? side_blocks : 4 * 256, on_side(c) ? side : stream);
-      HIP_CHECK(hipGetLastError());
-    }
-  if (two) {
-    HIP_CHECK(hipEventRecord(ev_join, side));
-    HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
-  }
-  hipLaunchKernelGGL(k_batch_symmetrize, dim3(64, count), dim3(256), 0, stream, d_mols.as<MolDev>(),
-                     d_jout.as<double*>(), d_kout.as<double*>(), count);
-  HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(stream));
+std::unique_ptr<Engine> Fleet::make_engine(const hfg_basis_view& b) {
+  return std::unique_ptr<Engine>(new Engine(b, stream));
 }
+
+void Fleet::attach(int s, const Engine& e) {
+  if (s < 0 || s >= capacity) throw HfgError{HFG_INVALID_INPUT, "fleet slot out of range"};
+  tabs[s] = e.tab_host;
+  occupied[s] = 1;
+  HIP_CHECK(hipMemcpyAsync(d_tabs.as<MolTab>() + s, &tabs[s], sizeof(MolTab), hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));  // the host copy may change before the next build
+}
+
+void Fleet::detach(int s) {
+  if (s >= 0 && s < capacity) occupied[s] = 0;
+}
+
+void Fleet::set_active(const std::vector<int>& slots) {
+  std::vector<const MolTab*> th;
+  for (int s : slots) {
+    if (s < 0 || s >= capacity || !occupied[s]) throw HfgError{HFG_INVALID_INPUT, "inactive fleet slot"};
+    th.push_back(&tabs[s]);
+  }
+  active = slots;
+  dir = make_directory(slots, th);
+  if (d_dir.bytes < dir.host.size() * sizeof(int)) d_dir.alloc(std::max<size_t>(dir.host.size(), 1024) * sizeof(int));
+  HIP_CHECK(hipMemcpyAsync(d_dir.p, dir.host.data(), dir.host.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+  if (!slots.empty())
+    HIP_CHECK(hipMemcpyAsync(d_active.p, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));  // host vectors are reused by the next call
+}
+
+void Fleet::build() {
+  const int count = (int)active.size();
+  if (!count) return;
+  hipLaunchKernelGGL(k_fleet_clear, dim3(64, count), dim3(256), 0, stream, d_tabs.as<MolTab>(), d_active.as<int>(),
+                     count);
+  JKLaunch L{d_counters.as<int>(), ev_fork, ev_join, d_boys.as<double>(), screen0, use_rows,
+             profile ? events.data() : nullptr};
+  launch_jk_set(L, d_tabs.as<MolTab>(), dir, d_dir.as<int>(), stream, side, side_blocks);
+  if (profile) {
+    sync();
+    for (int c = 0; c < HFG_NCLASS; ++c) {
+      if (!dir.count[c]) continue;
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, events[2 * c], events[2 * c + 1]));
+      class_ms[c] += ms;
+      class_launches[c] += 1;
+    }
+  }
+}
+
+void Fleet::set_profile(bool on) {
+  if (on && events.empty()) {
+    events.resize(2 * HFG_NCLASS);
+    for (auto& e : events) HIP_CHECK(hipEventCreate(&e));
+  }
+  if (on) {
+    class_ms.assign(HFG_NCLASS, 0.0);
+    class_launches.assign(HFG_NCLASS, 0);
+  }
+  profile = on;
+}
+
+void Fleet::build_jk(double* const* d_jout, double* const* d_kout) {
+  build();
+  const int count = (int)active.size();
+  if (!count) return;
+  hipLaunchKernelGGL(k_fleet_jk_out, dim3(64, count), dim3(256), 0, stream, d_tabs.as<MolTab>(), d_active.as<int>(),
+                     d_jout, d_kout, count);
+  HIP_CHECK(hipGetLastError());
+}
+
+void Fleet::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+
+// The hfg_batch_* entry points: a fleet holding every molecule, all active.
+struct Batch {
+  std::unique_ptr<Fleet> fleet;
+  std::vector<std::unique_ptr<Engine>> eng;
+  DevBuf d_jout, d_kout;
+  int nmax = 0;
+  double flops = 0;
+  int64_t quartets = 0;
+
+  Batch(const hfg_basis_view* bases, int count) {
+    if (count <= 0) throw HfgError{HFG_INVALID_INPUT, "empty batch"};
+    fleet.reset(new Fleet(count));
+    std::vector<int> slots;
+    std::vector<double*> jout, kout;
+    for (int m = 0; m < count; ++m) {
+      eng.push_back(fleet->make_engine(bases[m]));
+      Engine& e = *eng.back();
+      fleet->attach(m, e);
+      slots.push_back(m);
+      jout.push_back(e.d_J.as<double>());
+      kout.push_back(e.d_K.as<double>());
+      nmax = std::max(nmax, e.n);
+      for (double f : e.class_flops) flops += f;
+      quartets += e.n_quartets;
+    }
+    fleet->set_active(slots);
+    d_jout.alloc(count * sizeof(double*));
+    d_kout.alloc(count * sizeof(double*));
+    HIP_CHECK(hipMemcpyAsync(d_jout.p, jout.data(), count * sizeof(double*), hipMemcpyHostToDevice, fleet->stream));
+    HIP_CHECK(hipMemcpyAsync(d_kout.p, kout.data(), count * sizeof(double*), hipMemcpyHostToDevice, fleet->stream));
+    fleet->sync();
+  }
+  ~Batch() {
+    eng.clear();  // engines use the fleet's stream: release them first
+    fleet.reset();
+  }
+  void jk() {
+    fleet->build_jk(d_jout.as<double*>(), d_kout.as<double*>());
+    fleet->sync();
+  }
+};
 
 }  // namespace hfg
 
@@ -293,12 +235,43 @@ int hfg_batch_info(const hfg_batch* b, int64_t* count, int64_t* n_max, int64_t* 
   return HFG_OK;
 }
 
+int hfg_batch_profile(hfg_batch* b, int enable) {
+  if (!b) return set_error(HFG_INVALID_INPUT, "null batch");
+  BGUARD({
+    b->b->fleet->set_profile(enable != 0);
+    return HFG_OK;
+  })
+}
+
+int hfg_batch_class_stats(hfg_batch* b, int cls, int32_t* l4, double* flops, int64_t* quartets, double* ms,
+                          int64_t* launches) {
+  if (!b || cls < 0 || cls >= HFG_NCLASS) return set_error(HFG_INVALID_INPUT, "class index out of range");
+  Batch& B = *b->b;
+  int l[4];
+  class_angular(cls, l);
+  if (l4)
+    for (int k = 0; k < 4; ++k) l4[k] = l[k];
+  double f = 0;
+  int64_t q = 0;
+  for (auto& e : B.eng) {
+    f += e->class_flops[cls];
+    q += e->class_quartets[cls];
+  }
+  if (flops) *flops = f;
+  if (quartets) *quartets = q;
+  const Fleet& F = *B.fleet;
+  if (ms) *ms = F.class_ms.empty() ? 0.0 : F.class_ms[cls];
+  if (launches) *launches = F.class_launches.empty() ? 0 : F.class_launches[cls];
+  return HFG_OK;
+}
+
 int hfg_batch_set_density(hfg_batch* b, int mol, const double* p) {
   if (!b || !p) return set_error(HFG_INVALID_INPUT, "null argument");
   if (mol < 0 || mol >= (int)b->b->eng.size()) return set_error(HFG_INVALID_INPUT, "molecule index out of range");
   BGUARD({
     Engine& e = *b->b->eng[mol];
     HIP_CHECK(hipMemcpyAsync(e.d_D.p, p, (size_t)e.n * e.n * 8, hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipStreamSynchronize(e.stream));
     return HFG_OK;
   })
 }
@@ -315,7 +288,7 @@ int hfg_batch_jk_each(hfg_batch* b) {
   if (!b) return set_error(HFG_INVALID_INPUT, "null batch");
   BGUARD({
     for (auto& e : b->b->eng) e->jk_device(e->d_D.as<double>(), e->d_J.as<double>(), e->d_K.as<double>());
-    for (auto& e : b->b->eng) e->sync();
+    b->b->fleet->sync();
     return HFG_OK;
   })
 }
@@ -326,9 +299,9 @@ int hfg_batch_get_jk(hfg_batch* b, int mol, double* j, double* k) {
   BGUARD({
     Engine& e = *b->b->eng[mol];
     const size_t bytes = (size_t)e.n * e.n * 8;
-    if (j) HIP_CHECK(hipMemcpyAsync(j, e.d_J.p, bytes, hipMemcpyDeviceToHost, b->b->stream));
-    if (k) HIP_CHECK(hipMemcpyAsync(k, e.d_K.p, bytes, hipMemcpyDeviceToHost, b->b->stream));
-    HIP_CHECK(hipStreamSynchronize(b->b->stream));
+    if (j) HIP_CHECK(hipMemcpyAsync(j, e.d_J.p, bytes, hipMemcpyDeviceToHost, e.stream));
+    if (k) HIP_CHECK(hipMemcpyAsync(k, e.d_K.p, bytes, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipStreamSynchronize(e.stream));
     return HFG_OK;
   })
 }

@@ -43,9 +43,84 @@ static int nc_of(const Engine& e, int pair) {
 }
 
 namespace hfg {
+void eri_mo(Engine& e, const double* C, int nmo, double* out);
+void jk_device_noalloc(Engine& e, const double* dP, double* dJ, double* dK);
 int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
-            hfg_scf_result* res, double* p_out, double* f_out, double* eps_out);
+            hfg_scf_result* res, double* p_out, double* f_out, double* eps_out, const double* p_guess);
 }
+
+// The packed screened tensor of compute_shell_eri_tensor
+// (src/shell_integrals.rs:156-240): surviving quartets ij >= kl in batches,
+// blocks from the GPU block kernels, scattered by eri_index (src/scf.rs:221-235).
+void hfg::eri_packed_host(Engine& e, double* out) {
+  {
+    const size_t n = e.n, np = n * (n + 1) / 2;
+    const size_t len = np * (np + 1) / 2;
+    std::memset(out, 0, len * sizeof(double));
+    const size_t npair = e.pairs.size();
+    auto cidx = [](size_t i, size_t j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
+    // batches of surviving quartets ij >= kl (src/shell_integrals.rs:225-235)
+    size_t ij = 0, kl = 0;
+    const size_t budget = 1 << 24;  // doubles per batch
+    while (ij < npair) {
+      std::vector<BlockTask> tasks;
+      std::vector<int> cls;
+      std::vector<std::pair<int, int>> which;
+      long long off = 0;
+      while (ij < npair && (size_t)off < budget) {
+        if (e.Q[ij] * e.Q[kl] > e.screen) {
+          const PairInfo& pb = e.pairs[ij];
+          const PairInfo& pk = e.pairs[kl];
+          int pcb = pair_class(pb.la, pb.lb), pck = pair_class(pk.la, pk.lb);
+          int b = (int)ij, k = (int)kl;
+          if (pck > pcb) {
+            std::swap(b, k);
+            std::swap(pcb, pck);
+          }
+          tasks.push_back({b, k, off});
+          cls.push_back(quartet_class(pcb, pck));
+          which.push_back({b, k});
+          off += (long long)nc_of(e, b) * nc_of(e, k);
+        }
+        if (++kl > ij) {
+          ++ij;
+          kl = 0;
+        }
+      }
+      // group by class for the launches
+      std::vector<size_t> ord(tasks.size());
+      for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+      std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return cls[x] < cls[y]; });
+      std::vector<BlockTask> t2;
+      std::vector<int> c2;
+      for (size_t i : ord) {
+        t2.push_back(tasks[i]);
+        c2.push_back(cls[i]);
+      }
+      std::vector<double> blk(off);
+      e.blocks(t2, c2, blk);
+      for (const BlockTask& t : t2) {
+        const PairInfo& pb = e.pairs[t.bra];
+        const PairInfo& pk = e.pairs[t.ket];
+        const int na = (int)e.shells[pb.sh_a].scale.size(), nb = (int)e.shells[pb.sh_b].scale.size();
+        const int nc_ = (int)e.shells[pk.sh_a].scale.size(), nd = (int)e.shells[pk.sh_b].scale.size();
+        const double* v = blk.data() + t.off;
+        for (int i = 0; i < na; ++i)
+          for (int j = 0; j < nb; ++j) {
+            const size_t mu = pb.ao_a + i, nu = pb.ao_b + j;
+            const double fab = e.ao_scale[mu] * e.ao_scale[nu];
+            for (int k = 0; k < nc_; ++k)
+              for (int l = 0; l < nd; ++l) {
+                const size_t la = pk.ao_a + k, si = pk.ao_b + l;
+                out[cidx(cidx(mu, nu), cidx(la, si))] =
+                    fab * e.ao_scale[la] * e.ao_scale[si] * v[((i * nb + j) * nc_ + k) * nd + l];
+              }
+          }
+      }
+    }
+  }
+}
+
 
 extern "C" {
 
@@ -123,71 +198,19 @@ int hfg_shell_quartet(hfg_ctx* ctx, int sa, int sb, int sc, int sd, double* out)
 int hfg_eri_packed(hfg_ctx* ctx, double* out, size_t len) {
   if (!ctx || !out) return set_err(HFG_INVALID_INPUT, "null argument");
   GUARD({
-    Engine& e = *ctx->e;
-    const size_t n = e.n, np = n * (n + 1) / 2;
+    const size_t n = ctx->e->n, np = n * (n + 1) / 2;
     if (len != np * (np + 1) / 2) return set_err(HFG_INVALID_INPUT, "packed length must be np(np+1)/2");
-    std::memset(out, 0, len * sizeof(double));
-    const size_t npair = e.pairs.size();
-    auto cidx = [](size_t i, size_t j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
-    // batches of surviving quartets ij >= kl (src/shell_integrals.rs:225-235)
-    size_t ij = 0, kl = 0;
-    const size_t budget = 1 << 24;  // doubles per batch
-    while (ij < npair) {
-      std::vector<BlockTask> tasks;
-      std::vector<int> cls;
-      std::vector<std::pair<int, int>> which;
-      long long off = 0;
-      while (ij < npair && (size_t)off < budget) {
-        if (e.Q[ij] * e.Q[kl] > e.screen) {
-          const PairInfo& pb = e.pairs[ij];
-          const PairInfo& pk = e.pairs[kl];
-          int pcb = pair_class(pb.la, pb.lb), pck = pair_class(pk.la, pk.lb);
-          int b = (int)ij, k = (int)kl;
-          if (pck > pcb) {
-            std::swap(b, k);
-            std::swap(pcb, pck);
-          }
-          tasks.push_back({b, k, off});
-          cls.push_back(quartet_class(pcb, pck));
-          which.push_back({b, k});
-          off += (long long)nc_of(e, b) * nc_of(e, k);
-        }
-        if (++kl > ij) {
-          ++ij;
-          kl = 0;
-        }
-      }
-      // group by class for the launches
-      std::vector<size_t> ord(tasks.size());
-      for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
-      std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return cls[x] < cls[y]; });
-      std::vector<BlockTask> t2;
-      std::vector<int> c2;
-      for (size_t i : ord) {
-        t2.push_back(tasks[i]);
-        c2.push_back(cls[i]);
-      }
-      std::vector<double> blk(off);
-      e.blocks(t2, c2, blk);
-      for (const BlockTask& t : t2) {
-        const PairInfo& pb = e.pairs[t.bra];
-        const PairInfo& pk = e.pairs[t.ket];
-        const int na = (int)e.shells[pb.sh_a].scale.size(), nb = (int)e.shells[pb.sh_b].scale.size();
-        const int nc_ = (int)e.shells[pk.sh_a].scale.size(), nd = (int)e.shells[pk.sh_b].scale.size();
-        const double* v = blk.data() + t.off;
-        for (int i = 0; i < na; ++i)
-          for (int j = 0; j < nb; ++j) {
-            const size_t mu = pb.ao_a + i, nu = pb.ao_b + j;
-            const double fab = e.ao_scale[mu] * e.ao_scale[nu];
-            for (int k = 0; k < nc_; ++k)
-              for (int l = 0; l < nd; ++l) {
-                const size_t la = pk.ao_a + k, si = pk.ao_b + l;
-                out[cidx(cidx(mu, nu), cidx(la, si))] =
-                    fab * e.ao_scale[la] * e.ao_scale[si] * v[((i * nb + j) * nc_ + k) * nd + l];
-              }
-          }
-      }
-    }
+    eri_packed_host(*ctx->e, out);
+    return HFG_OK;
+  })
+}
+
+int hfg_eri_mo(hfg_ctx* ctx, const double* c, int n_mo, double* out, size_t len) {
+  if (!ctx || !c || !out) return set_err(HFG_INVALID_INPUT, "null argument");
+  const size_t mp = (size_t)n_mo * (n_mo + 1) / 2;
+  if (n_mo <= 0 || len != mp * mp) return set_err(HFG_INVALID_INPUT, "output length must be (n_mo(n_mo+1)/2)^2");
+  GUARD({
+    eri_mo(*ctx->e, c, n_mo, out);
     return HFG_OK;
   })
 }
@@ -213,6 +236,43 @@ int hfg_contract_jk(hfg_ctx* ctx, const double* p, double* j, double* k) {
     HIP_CHECK(hipMemcpyAsync(j, e.d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
     HIP_CHECK(hipMemcpyAsync(k, e.d_K.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
     e.sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_contract_jk_multi(hfg_ctx* ctx, const double* p_total, int nspin, const double* const* p_spin, double* j,
+                          double* const* k_spin) {
+  if (!ctx || !p_total || !j || nspin < 0 || (nspin > 0 && (!p_spin || !k_spin)))
+    return set_err(HFG_INVALID_INPUT, "null argument or negative spin count");
+  for (int s = 0; s < nspin; ++s)
+    if (!p_spin[s] || !k_spin[s]) return set_err(HFG_INVALID_INPUT, "null spin density or exchange output");
+  GUARD({
+    Engine& e = *ctx->e;
+    const size_t nn = (size_t)e.n * e.n;
+    auto finite = [&](const double* m) {
+      for (size_t x = 0; x < nn; ++x)
+        if (!std::isfinite(m[x])) return false;
+      return true;
+    };
+    if (!finite(p_total)) return set_err(HFG_INVALID_INPUT, "non-finite density");
+    bool total_k = false;  // a spin density given as the total itself (RHF) shares the total's build
+    for (int s = 0; s < nspin; ++s) {
+      if (p_spin[s] == p_total) total_k = true;
+      else if (!finite(p_spin[s])) return set_err(HFG_INVALID_INPUT, "non-finite spin density");
+    }
+    HIP_CHECK(hipMemcpyAsync(e.d_D.p, p_total, nn * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    e.jk_device(e.d_D.as<double>(), e.d_J.as<double>(), total_k ? e.d_K.as<double>() : nullptr);
+    HIP_CHECK(hipMemcpyAsync(j, e.d_J.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    for (int s = 0; s < nspin; ++s)
+      if (p_spin[s] == p_total) HIP_CHECK(hipMemcpyAsync(k_spin[s], e.d_K.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+    for (int s = 0; s < nspin; ++s) {
+      if (p_spin[s] == p_total) continue;
+      HIP_CHECK(hipMemcpyAsync(e.d_D.p, p_spin[s], nn * sizeof(double), hipMemcpyHostToDevice, e.stream));
+      e.jk_device(e.d_D.as<double>(), nullptr, e.d_K.as<double>());
+      HIP_CHECK(hipMemcpyAsync(k_spin[s], e.d_K.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+      e.sync();
+    }
     return HFG_OK;
   })
 }
@@ -310,7 +370,17 @@ int hfg_calibration_copy(size_t bytes, double* ms) {
 int hfg_scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
                 hfg_scf_result* res, double* p_out, double* f_out, double* eps_out) {
   if (!basis || !zxyz || !opt || !res) return set_err(HFG_INVALID_INPUT, "null argument");
-  GUARD({ return scf_rhf(basis, zxyz, natom, charge, opt, res, p_out, f_out, eps_out); })
+  GUARD({ return scf_rhf(basis, zxyz, natom, charge, opt, res, p_out, f_out, eps_out, nullptr); })
+}
+
+int hfg_scf_rhf_guess(const hfg_basis_view* basis, const double* zxyz, int natom, int charge,
+                      const hfg_scf_options* opt, const double* p_guess, hfg_scf_result* res, double* p_out,
+                      double* f_out, double* eps_out) {
+  if (!basis || !zxyz || !opt || !res || !p_guess) return set_err(HFG_INVALID_INPUT, "null argument");
+  const size_t nn = (size_t)basis->n_functions * basis->n_functions;
+  for (size_t x = 0; x < nn; ++x)
+    if (!std::isfinite(p_guess[x])) return set_err(HFG_INVALID_INPUT, "non-finite initial density");
+  GUARD({ return scf_rhf(basis, zxyz, natom, charge, opt, res, p_out, f_out, eps_out, p_guess); })
 }
 
 }  // extern "C"

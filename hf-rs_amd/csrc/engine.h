@@ -16,49 +16,67 @@ namespace hfg {
 // Number of per-XCC copies of the J/K accumulators (see jk_kernel.h).
 constexpr int NXCC_SLOTS = 16;
 
-// One molecule of a batch as the J/K kernels address it: its density, the
-// first of its per-XCC accumulator copies, its AO component factors.
-struct MolDev {
-  const double* D;
-  double* Jacc;
-  double* Kacc;
-  const double* scale;
-  size_t nn;
-  int n, pad;
-};
-
-// Arguments of the J/K kernels (jk_kernel.h).
-struct JKArgs {
-  const JKTask* tasks;   // all classes' tasks; this class's start at task0
-  int task0, ntask, n;
+// One molecule as the J/K kernels address it: its plan tables (jk_plan.cpp),
+// the density of the current build, its per-XCC accumulator copies, AO
+// factors and size, and the Schwarz threshold of the current build.  It lives
+// in device memory and a launch reaches it through a directory (JKArgs), so
+// one launch per class serves any set of resident molecules; a single
+// molecule is a directory of one entry.
+struct MolTab {
+  const JKTask* tasks;
   const ViewDev* views;
   const ComboDev* combos;
   const KetGroup* groups;
-  const double* brec;    // bra primitive records, contiguous per view
-  PrimPairs kt;          // ket side, transposed per group
-  const double* kcoef;   // ket coefficients, transposed per group
-  size_t nn;             // n*n, the stride between per-XCC accumulator copies
-  double screen;         // Schwarz threshold of a combination pair in this build
-  double screen0;        // the threshold the task lists were built with
-  const double* vq;      // per view: largest Schwarz bound of its combinations
-  const double* scale;
-  const double* boys;
-  const double* D;
-  double* Jacc;
+  const double* brec;   // bra primitive records, contiguous per view
+  const double* kt;     // ket primitive pairs, 9 columns of kt_size, transposed per group
+  const double* kcoef;  // ket coefficients, transposed per group
+  const double* vq;     // per view: largest Schwarz bound of its combinations
+  const double* scale;  // per AO component factor
+  const double* D;      // density of the build
+  double* Jacc;         // NXCC_SLOTS copies of n*n
   double* Kacc;
-  // LDS-row path (k_jk_rows): the class's jobs (absolute task offsets, njob + 1
-  // entries) and its job counter, zero at launch; njob = 0 selects k_jk.
-  const int* jobs;
-  int njob;
-  int* job_counter;
-  // Batch: per-molecule addressing (ViewDev::mol); null for one molecule,
-  // which then uses n, nn, D, Jacc, Kacc and scale above (n is the largest
-  // molecule's for a batch: it sizes the LDS rows).
-  const MolDev* mols;
+  const int* jobs;      // absolute task offsets of the molecule's jobs, plus a sentinel
+  size_t kt_size;
+  size_t nn;            // n*n, the stride between per-XCC accumulator copies
+  double screen;        // Schwarz threshold of a combination pair in this build
+  int n;
+  int pad;
+  int task_begin[HFG_NCLASS + 1];
+  int job_begin[HFG_NCLASS + 1];
 };
-// Launch of class cls's J/K kernel over ntask tasks with at most max_blocks
-// workgroups; the kernels are split over three translation units.
-void launch_jk_class(int cls, const JKArgs& a, int ntask, int max_blocks, hipStream_t s);
+
+// Arguments of one class launch over a directory: entry e is molecule
+// tabs[dir_mol[e]]; the class's jobs of the entries are numbered
+// consecutively, entry e owning [dir_job[e], dir_job[e + 1]) (its tasks
+// likewise [dir_task[e], dir_task[e + 1])).
+struct JKArgs {
+  const MolTab* tabs;
+  const int* dir_mol;
+  const int* dir_job;
+  const int* dir_task;
+  int ndir, njob, ntask;
+  int nmax;            // largest n of the entries: sizes the LDS rows
+  double screen0;      // the threshold the task lists were built with
+  const double* boys;
+  int* job_counter;    // zero at launch (k_jk_rows)
+  int use_rows;        // 0: the global-memory kernel k_jk
+};
+// Per-class directory arrays of one launch set, in one int buffer: class c's
+// entries start at off[c] (dir_mol, then dir_job and dir_task with one extra
+// entry each).
+struct JKDirectory {
+  std::vector<int> host;
+  std::vector<int> off;     // HFG_NCLASS entries into host
+  std::vector<int> count;   // entries per class
+  std::vector<int> njob, ntask;
+  int nmax = 0;
+};
+// Directory of the molecules (tab indices) for every class, skipping entries
+// without work in a class; tabs_host holds the molecules' tables.
+JKDirectory make_directory(const std::vector<int>& mol, const std::vector<const MolTab*>& tabs_host);
+// Launch of class cls's J/K kernel with at most max_blocks workgroups; the
+// kernels are split over five translation units.
+void launch_jk_class(int cls, const JKArgs& a, int max_blocks, hipStream_t s);
 // Whether class cls runs on the side stream: (ds|ds) and the classes with a dp
 // or dd bra, (dp|ss) .. (dd|dd), unless HFG_SIDE_CLASSES lists class indices
 // (comma-separated; measured on (H2O)16/cc-pVDZ).
@@ -105,9 +123,31 @@ double md_flops(int la, int lb, int lc, int ld);
 // Records the message returned by hfg_last_error(); returns code.
 int set_error(int code, const std::string& msg);
 
+// What one set of class launches needs besides the directory: job counters
+// (HFG_NCLASS ints), fork/join events of the side stream, the Boys table,
+// the threshold the task lists were built with, the LDS-row choice and
+// optional per-class event pairs.
+struct JKLaunch {
+  int* counters;
+  hipEvent_t fork, join;
+  const double* boys;
+  double screen0;
+  bool use_rows;
+  hipEvent_t* events;
+};
+// The J/K class launches of one build over directory dir (device copy d_dir)
+// of the tables at d_tabs; accumulators must be cleared beforehand.  Classes
+// side_class() names run on side_stream (at most side_cap workgroups each)
+// when side_cap > 0; the streams are joined at the end.
+void launch_jk_set(const JKLaunch& L, const MolTab* d_tabs, const JKDirectory& dir, const int* d_dir, hipStream_t main,
+                   hipStream_t side_stream, int side_cap);
+
 struct Engine {
   int device = 0;
+  // The engine's stream: its own, or the one a batch shares among its members
+  // (then the engine creates no stream and runs single-stream builds).
   hipStream_t stream = nullptr;
+  bool own_streams = true;
   // Second stream (default priority) for the latency-bound high-L classes,
   // which then overlap the large low-L classes; joined before symmetrisation.
   hipStream_t side = nullptr;
@@ -135,7 +175,9 @@ struct Engine {
   std::vector<int64_t> class_launches;
   std::vector<hipEvent_t> events;  // 2 per class
 
-  explicit Engine(const hfg_basis_view& b);
+  // shared: a stream owned by the caller (batch members); null creates the
+  // engine's own two streams.
+  explicit Engine(const hfg_basis_view& b, hipStream_t shared = nullptr);
   ~Engine();
   PrimPairs pp_view() const;
   PrimPairs kt_view() const;
@@ -161,11 +203,22 @@ struct Engine {
   // change) raises the Schwarz threshold to screen / dmax for this build.
   void jk_device(const double* dP, double* dJ, double* dK, const double* dH = nullptr, double* dF = nullptr,
                  double dmax = 1.0);
+  // The molecule's table (host copy: D = d_D, screen = the plan's threshold)
+  // and its one-entry directory in HBM.
+  MolTab tab_host;
+  DevBuf d_tab, d_dir;
+  JKDirectory dir;
+  // The engine's launch resources (job counters, fork/join events, Boys table,
+  // threshold, profiling events) for launch_jk_set.
+  JKLaunch launch_ctx();
   void sync();
   void set_profile(bool on);
 };
 
 void class_angular(int c, int* l4);
+// The packed 8-fold screened ERI tensor of the engine's molecule into host
+// out (np(np+1)/2 doubles, np = n(n+1)/2).
+void eri_packed_host(Engine& e, double* out);
 int persistent_blocks();
 double calibration_copy(size_t bytes);
 

@@ -91,8 +91,46 @@ __global__ void k_symmetrize(const double* Jacc, const double* Kacc, double* J, 
   if (F) F[idx] = H[idx] + j - 0.5 * k;
 }
 
+// The density pointer and Schwarz threshold of the next build in a
+// molecule's table (a kernel, so that it orders with the launches around it).
+__global__ void k_tab_set(MolTab* tab, const double* D, double screen) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    tab->D = D;
+    tab->screen = screen;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side.
+
+JKDirectory make_directory(const std::vector<int>& mol, const std::vector<const MolTab*>& tabs_host) {
+  JKDirectory d;
+  d.off.assign(HFG_NCLASS, 0);
+  d.count.assign(HFG_NCLASS, 0);
+  d.njob.assign(HFG_NCLASS, 0);
+  d.ntask.assign(HFG_NCLASS, 0);
+  for (size_t i = 0; i < mol.size(); ++i) d.nmax = std::max(d.nmax, tabs_host[i]->n);
+  for (int c = 0; c < HFG_NCLASS; ++c) {
+    std::vector<int> ids, jp{0}, tp{0};
+    for (size_t i = 0; i < mol.size(); ++i) {
+      const MolTab& t = *tabs_host[i];
+      const int nj = t.job_begin[c + 1] - t.job_begin[c], nt = t.task_begin[c + 1] - t.task_begin[c];
+      if (!nt) continue;
+      ids.push_back(mol[i]);
+      jp.push_back(jp.back() + nj);
+      tp.push_back(tp.back() + nt);
+    }
+    d.off[c] = (int)d.host.size();
+    d.count[c] = (int)ids.size();
+    d.njob[c] = jp.back();
+    d.ntask[c] = tp.back();
+    d.host.insert(d.host.end(), ids.begin(), ids.end());
+    d.host.insert(d.host.end(), jp.begin(), jp.end());
+    d.host.insert(d.host.end(), tp.begin(), tp.end());
+  }
+  if (d.host.empty()) d.host.push_back(0);
+  return d;
+}
 
 bool side_class(int cls) {
   static const unsigned mask = [] {
@@ -140,7 +178,7 @@ void launch_block_class(int cls, const BlockArgs& a, hipStream_t s) {
   HIP_CHECK(hipGetLastError());
 }
 
-Engine::Engine(const hfg_basis_view& b) {
+Engine::Engine(const hfg_basis_view& b, hipStream_t shared) {
   shells = group_shells(b);
   n = b.n_functions;
   for (auto& s : shells)
@@ -149,11 +187,20 @@ Engine::Engine(const hfg_basis_view& b) {
   HIP_CHECK(hipGetDeviceCount(&ndev));
   if (ndev <= 0) throw HfgError{HFG_DEVICE, "no GPU visible"};
   HIP_CHECK(hipGetDevice(&device));
-  HIP_CHECK(hipStreamCreate(&stream));  // default priority
-  HIP_CHECK(hipStreamCreate(&side));    // default priority
-  HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-  HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-  if (const char* env = std::getenv("HFG_SIDE_BLOCKS")) side_blocks = std::max(0, std::atoi(env));
+  if (shared) {
+    // A batch member: no streams of its own (every stream of a process maps
+    // onto one of a few hardware queues, each of which keeps its own scratch
+    // reservation), single-stream builds on the batch's stream.
+    stream = shared;
+    own_streams = false;
+    side_blocks = 0;
+  } else {
+    HIP_CHECK(hipStreamCreate(&stream));  // default priority
+    HIP_CHECK(hipStreamCreate(&side));    // default priority
+    HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    if (const char* env = std::getenv("HFG_SIDE_BLOCKS")) side_blocks = std::max(0, std::atoi(env));
+  }
   {
     int cus = 0;
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -221,8 +268,10 @@ Engine::~Engine() {
   for (auto e : events) (void)hipEventDestroy(e);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
-  if (side) (void)hipStreamDestroy(side);
-  if (stream) (void)hipStreamDestroy(stream);
+  if (own_streams) {
+    if (side) (void)hipStreamDestroy(side);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 }
 
 // McMurchie-Davidson loop-count model of one primitive quartet (see hfg.h).
@@ -367,6 +416,33 @@ void Engine::build_tasks() {
   prim_pairs_kept = plan.prim_pairs_kept;
   n_gen_shells = plan.n_gen_shells;
   n_views = (int)plan.views.size();
+  std::memset(&tab_host, 0, sizeof tab_host);
+  tab_host.tasks = d_tasks.as<JKTask>();
+  tab_host.views = d_views.as<ViewDev>();
+  tab_host.combos = d_combos.as<ComboDev>();
+  tab_host.groups = d_groups.as<KetGroup>();
+  tab_host.brec = d_vb.as<double>();
+  tab_host.kt = d_kt.as<double>();
+  tab_host.kcoef = d_kcoef.as<double>();
+  tab_host.vq = d_vq.as<double>();
+  tab_host.scale = d_scale.as<double>();
+  tab_host.D = d_D.as<double>();
+  tab_host.Jacc = d_Jacc.as<double>();
+  tab_host.Kacc = d_Kacc.as<double>();
+  tab_host.jobs = d_jobs.as<int>();
+  tab_host.kt_size = kt_size;
+  tab_host.nn = (size_t)n * n;
+  tab_host.screen = screen;
+  tab_host.n = n;
+  for (int c = 0; c <= HFG_NCLASS; ++c) {
+    tab_host.task_begin[c] = (int)task_begin[c];
+    tab_host.job_begin[c] = (int)job_begin[c];
+  }
+  d_tab.alloc(sizeof(MolTab));
+  HIP_CHECK(hipMemcpyAsync(d_tab.p, &tab_host, sizeof(MolTab), hipMemcpyHostToDevice, stream));
+  dir = make_directory({0}, {&tab_host});
+  d_dir.alloc(dir.host.size() * sizeof(int));
+  HIP_CHECK(hipMemcpyAsync(d_dir.p, dir.host.data(), dir.host.size() * sizeof(int), hipMemcpyHostToDevice, stream));
   sync();
   count_work();
 }
@@ -418,58 +494,61 @@ void Engine::count_work() {
   }
 }
 
+void launch_jk_set(const JKLaunch& L, const MolTab* d_tabs, const JKDirectory& dr, const int* d_dr, hipStream_t main,
+                   hipStream_t side_stream, int side_cap) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, main, L.counters, HFG_NCLASS);
+  // The high-L classes, few and long quartets, run on the side stream with a
+  // capped grid and fill the room the low-L classes leave.
+  auto on_side = [&](int c) { return side_cap > 0 && side_stream && side_class(c); };
+  const bool two = side_cap > 0 && side_stream;
+  if (two) {
+    HIP_CHECK(hipEventRecord(L.fork, main));
+    HIP_CHECK(hipStreamWaitEvent(side_stream, L.fork, 0));
+  }
+  JKArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.tabs = d_tabs;
+  a.nmax = dr.nmax;
+  a.screen0 = L.screen0;
+  a.boys = L.boys;
+  a.use_rows = L.use_rows ? 1 : 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int c = 0; c < HFG_NCLASS; ++c) {
+      if (!dr.count[c] || on_side(c) != (pass == 0)) continue;
+      hipStream_t s = on_side(c) ? side_stream : main;
+      const int* base = d_dr + dr.off[c];
+      a.ndir = dr.count[c];
+      a.dir_mol = base;
+      a.dir_job = base + dr.count[c];
+      a.dir_task = base + 2 * dr.count[c] + 1;
+      a.njob = dr.njob[c];
+      a.ntask = dr.ntask[c];
+      a.job_counter = L.counters + c;
+      if (L.events) HIP_CHECK(hipEventRecord(L.events[2 * c], s));
+      launch_jk_class(c, a, on_side(c) ? side_cap : g_max_blocks, s);
+      HIP_CHECK(hipGetLastError());
+      if (L.events) HIP_CHECK(hipEventRecord(L.events[2 * c + 1], s));
+    }
+  if (two) {
+    HIP_CHECK(hipEventRecord(L.join, side_stream));
+    HIP_CHECK(hipStreamWaitEvent(main, L.join, 0));
+  }
+}
+
+JKLaunch Engine::launch_ctx() {
+  return JKLaunch{d_counters.as<int>(), ev_fork, ev_join, d_boys.as<double>(), screen, use_rows,
+                  profile ? events.data() : nullptr};
+}
+
 void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* dH, double* dF, double dmax) {
   const size_t nn = (size_t)n * n;
   const int tb = 256;
   const size_t nacc = nn * NXCC_SLOTS;
   hipLaunchKernelGGL(k_fill, dim3((nacc + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(), nacc, 0.0);
   hipLaunchKernelGGL(k_fill, dim3((nacc + tb - 1) / tb), dim3(tb), 0, stream, d_Kacc.as<double>(), nacc, 0.0);
-  JKArgs a;
-  a.n = n;
-  a.nn = nn;
-  a.views = d_views.as<ViewDev>();
-  a.combos = d_combos.as<ComboDev>();
-  a.groups = d_groups.as<KetGroup>();
-  a.brec = d_vb.as<double>();
-  a.kt = kt_view();
-  a.kcoef = d_kcoef.as<double>();
-  a.screen0 = screen;
-  a.screen = dmax > 0.0 && dmax < 1.0 ? screen / dmax : screen;
-  a.vq = d_vq.as<double>();
-  a.scale = d_scale.as<double>();
-  a.boys = d_boys.as<double>();
-  a.D = dP;
-  a.Jacc = d_Jacc.as<double>();
-  a.Kacc = d_Kacc.as<double>();
-  a.mols = nullptr;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, d_counters.as<int>(), HFG_NCLASS);
-  a.tasks = d_tasks.as<JKTask>();
-  // The high-L classes, few and long quartets, run on the side stream with a
-  // capped grid and fill the room the low-L classes leave.
-  auto on_side = [&](int c) { return side_blocks > 0 && side_class(c); };
-  const bool two = side_blocks > 0;
-  if (two) {
-    HIP_CHECK(hipEventRecord(ev_fork, stream));
-    HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
-  }
-  for (int pass = 0; pass < 2; ++pass)
-    for (int c = 0; c < HFG_NCLASS; ++c) {
-      const size_t nt = task_begin[c + 1] - task_begin[c];
-      if (!nt || on_side(c) != (pass == 0)) continue;
-      hipStream_t s = on_side(c) ? side : stream;
-      a.task0 = (int)task_begin[c];
-      a.jobs = d_jobs.as<int>() + job_begin[c];
-      a.njob = use_rows ? (int)(job_begin[c + 1] - job_begin[c]) : 0;
-      a.job_counter = d_counters.as<int>() + c;
-      if (profile) HIP_CHECK(hipEventRecord(events[2 * c], s));
-      launch_jk_class(c, a, (int)nt, on_side(c) ? side_blocks : g_max_blocks, s);
-      HIP_CHECK(hipGetLastError());
-      if (profile) HIP_CHECK(hipEventRecord(events[2 * c + 1], s));
-    }
-  if (two) {
-    HIP_CHECK(hipEventRecord(ev_join, side));
-    HIP_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
-  }
+  hipLaunchKernelGGL(k_tab_set, dim3(1), dim3(64), 0, stream, d_tab.as<MolTab>(), dP,
+                     dmax > 0.0 && dmax < 1.0 ? screen / dmax : screen);
+  launch_jk_set(launch_ctx(), d_tab.as<MolTab>(), dir, d_dir.as<int>(), stream, side, side_blocks);
   hipLaunchKernelGGL(k_symmetrize, dim3((nn + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(),
                      d_Kacc.as<double>(), dJ, dK, n, dH, dF);
   HIP_CHECK(hipGetLastError());

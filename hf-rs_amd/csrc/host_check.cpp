@@ -174,3 +174,23 @@ extern "C" int hfgc_shell_quartet(const hfg_basis_view* b, int sa, int sb, int s
     return e.code;
   }
 }
+
+#include "scf_host.h"
+
+// The RHF drivers' ADIIS minimiser (diis_host.cpp) and simplex projection, for
+// the CPU tests that check them against brute force.
+extern "C" void hfgc_simplex_minimise(const double* q, const double* b, int m, double* c) {
+  const auto r = hfg::simplex_minimise(std::vector<double>(q, q + (size_t)m * m), std::vector<double>(b, b + m));
+  std::copy(r.begin(), r.end(), c);
+}
+extern "C" void hfgc_simplex_project(double* v, int m) {
+  std::vector<double> x(v, v + m);
+  hfg::simplex_project(x);
+  std::copy(x.begin(), x.end(), v);
+}
+extern "C" int hfgc_solve_small(const double* a, const double* b, int n, double* x) {
+  std::vector<double> xx;
+  if (!hfg::solve_small(std::vector<double>(a, a + (size_t)n * n), std::vector<double>(b, b + n), n, xx)) return 0;
+  std::copy(xx.begin(), xx.end(), x);
+  return 1;
+}

@@ -108,33 +108,76 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Per-molecule addresses of a task or job: the molecule's density, this
-// XCC's accumulator copies, its AO factors and size.
-struct MolCtx {
+// One molecule's addresses for a job or task, read from its table once into
+// registers (wave-uniform): the plan tables, the density, this XCC's
+// accumulator copies, AO factors, size and the build's Schwarz threshold.
+// Held by value, the loads are not repeated after the kernel's atomics.
+struct MolView {
+  const JKTask* tasks;
+  const ViewDev* views;
+  const ComboDev* combos;
+  const KetGroup* groups;
+  const double* brec;
+  PrimPairs kt;
+  const double* kcoef;
+  const double* vq;
+  const double* scale;
   const double* D;
   double* Jx;
   double* Kx;
-  const double* scale;
+  const int* jobs;
+  double screen;
   int n;
 };
-__device__ __forceinline__ MolCtx mol_ctx(const JKArgs& a, int mol) {
-  const double* D = a.D;
-  double* J = a.Jacc;
-  double* K = a.Kacc;
-  const double* sc = a.scale;
-  size_t nn = a.nn;
-  int n = a.n;
-  if (a.mols) {
-    const MolDev& m = a.mols[mol];
-    D = m.D;
-    J = m.Jacc;
-    K = m.Kacc;
-    sc = m.scale;
-    nn = m.nn;
-    n = __builtin_amdgcn_readfirstlane(m.n);
+// The table is read through the constant address space: its address is
+// wave-uniform and it does not change during a launch, so every field comes
+// in by scalar loads and stays in scalar registers.  (Through the global
+// address space the kernel's atomics would keep the compiler from proving
+// that, and the bra records and factors that follow from these pointers
+// would be read by vector loads.)
+typedef const MolTab __attribute__((address_space(4)))* ctab;
+__device__ __forceinline__ MolView mol_view(const MolTab* T_) {
+  const ctab T = (ctab)T_;
+  MolView m;
+  m.tasks = T->tasks;
+  m.views = T->views;
+  m.combos = T->combos;
+  m.groups = T->groups;
+  m.brec = T->brec;
+  const double* kb = T->kt;
+  const size_t ks = T->kt_size;
+  m.kt = PrimPairs{kb, kb + ks, kb + 2 * ks, kb + 3 * ks, kb + 4 * ks, kb + 5 * ks, kb + 6 * ks, kb + 7 * ks, kb + 8 * ks};
+  m.kcoef = T->kcoef;
+  m.vq = T->vq;
+  m.scale = T->scale;
+  m.D = T->D;
+  const size_t slot = (size_t)xcc_id() * T->nn;
+  m.Jx = T->Jacc + slot;
+  m.Kx = T->Kacc + slot;
+  m.jobs = T->jobs;
+  m.screen = T->screen;
+  m.n = T->n;
+  return m;
+}
+// A field of a molecule's table (constant address space, as above).
+__device__ __forceinline__ int tab_task_begin(const MolTab* T, int c) { return ((ctab)T)->task_begin[c]; }
+__device__ __forceinline__ int tab_job_begin(const MolTab* T, int c) { return ((ctab)T)->job_begin[c]; }
+// Directory arrays and job offsets do not change during a launch either.
+typedef const int __attribute__((address_space(4)))* ciptr;
+
+// Entry of a directory prefix array (nd + 1 ascending offsets) that holds
+// item x: the last e with pre[e] <= x.  Wave-uniform.
+__device__ __forceinline__ int dir_entry(const int* __restrict__ pre_, int nd, int x) {
+  const ciptr pre = (ciptr)pre_;
+  int lo = 0, hi = nd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
   }
-  const size_t slot = (size_t)xcc_id() * nn;
-  return MolCtx{D, J + slot, K + slot, sc, n};
+  return __builtin_amdgcn_readfirstlane(lo);
 }
 
 // One task: the bra view (wave-uniform) against up to 64 ket views.  The
@@ -146,11 +189,12 @@ __device__ __forceinline__ MolCtx mol_ctx(const JKArgs& a, int mol) {
 // over a and c, J[c_k, d_l] over a and b, and J[a_i, b_j] over c, d and the
 // 64 lanes (one wave reduction).  One atomic per output element and member
 // tuple remains.
-// rows: the workgroup's LDS accumulator of the bra's K rows and bra Coulomb
-// block (see k_jk_rows), or null to update J and K in global memory only.
-template <int LA, int LB, int LC, int LD, int S>
-__device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restrict__ boys, int task, int lane,
-                                        double* __restrict__ rows) {
+// ROWS: rows is the workgroup's LDS accumulator of the bra's K rows and bra
+// Coulomb block (see k_jk_rows); otherwise J and K are updated in global
+// memory only.
+template <int LA, int LB, int LC, int LD, bool ROWS, int S>
+__device__ __forceinline__ void jk_task(const MolView& a, double screen0, const double* __restrict__ boys, int task,
+                                        int lane, double* __restrict__ rows) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NBLK = NAB * NCD;
   constexpr int CLS = class_index<LA, LB, LC, LD>();
@@ -162,7 +206,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
   const int ket0 = __builtin_amdgcn_readfirstlane(t.ket0);
   int nket = __builtin_amdgcn_readfirstlane(t.nket);
   const int grp = __builtin_amdgcn_readfirstlane(t.grp);
-  if (a.screen > a.screen0) {
+  if (a.screen > screen0) {
     // A raised threshold (a density change): the task's kets, sorted by
     // bound, keep the prefix with qmax_bra * qmax_ket > screen.
     const cdptr vq = (cdptr)a.vq;
@@ -205,8 +249,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
   for (int z = 0; z < RC; ++z) Co[z] = cket[z * RD].ao_a;
 #pragma unroll
   for (int u = 0; u < RD; ++u) Do[u] = cket[u].ao_b;
-  const MolCtx mc = mol_ctx(a, __builtin_amdgcn_readfirstlane(vb.mol));
-  const cdptr sc_ = (cdptr)mc.scale;  // read-only in the kernel: loads survive the atomics
+  const cdptr sc_ = (cdptr)a.scale;  // read-only in the kernel: loads survive the atomics
   // scaling
 #pragma unroll
   for (int xy = 0; xy < CA; ++xy) {
@@ -232,10 +275,10 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
     }
   }
 
-  const cdptr Dm = (cdptr)mc.D;  // read-only in the kernel: loads survive the atomics
-  const int n = mc.n;
-  double* __restrict__ Jx = mc.Jx;
-  double* __restrict__ Kx = mc.Kx;
+  const cdptr Dm = (cdptr)a.D;  // read-only in the kernel: loads survive the atomics
+  const int n = a.n;
+  double* __restrict__ Jx = a.Jx;
+  double* __restrict__ Kx = a.Kx;
   // Each term is its own pass over the blocks, one output element at a time,
   // so that only the blocks, one density block and one sum are live.
   // Coulomb, bra side: J[a_i, b_j] over c, d and the wave (all lanes share the bra).
@@ -264,7 +307,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
             }
           s = wave_sum(s);
           if (lane == 0) {
-            if (rows)
+            if constexpr (ROWS)
               lds_add(&rows[NR * n + (x * RB + y) * NAB + i * NB + jj], 2.0 * s);
             else
               add_local(&Jx[(size_t)(Ao[x] + i) * n + Bo[y] + jj], 2.0 * s);
@@ -322,7 +365,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                 for (int l = 0; l < ND; ++l)
                   s += Dm[(size_t)(Bo[y] + jj) * n + Do[u] + l] * blk[((i * NB + jj) * NC + k) * ND + l];
             }
-          if (rows)
+          if constexpr (ROWS)
             lds_add(&rows[(size_t)(x * NA + i) * n + Co[z] + k], s);
           else
             add_local(&Kx[(size_t)(Ao[x] + i) * n + Co[z] + k], s);
@@ -353,7 +396,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                   s += Dm[(size_t)(Ao[x] + i) * n + Do[u] + l] * blk[((i * NB + jj) * NC + k) * ND + l];
               }
             }
-          if (rows)
+          if constexpr (ROWS)
             lds_add(&rows[(size_t)(RA * NA + y * NB + jj) * n + Co[z] + k], s);
           else
             add_local(&Kx[(size_t)(Bo[y] + jj) * n + Co[z] + k], s);
@@ -384,7 +427,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                   s += Dm[(size_t)(Bo[y] + jj) * n + Co[z] + k] * blk[((i * NB + jj) * NC + k) * ND + l];
                 }
             }
-          if (rows)
+          if constexpr (ROWS)
             lds_add(&rows[(size_t)(x * NA + i) * n + Do[u] + l], s);
           else
             add_local(&Kx[(size_t)(Ao[x] + i) * n + Do[u] + l], s);
@@ -414,7 +457,7 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
                   s += Dm[(size_t)(Ao[x] + i) * n + Co[z] + k] * blk[((i * NB + jj) * NC + k) * ND + l];
                 }
             }
-          if (rows)
+          if constexpr (ROWS)
             lds_add(&rows[(size_t)(RA * NA + y * NB + jj) * n + Do[u] + l], s);
           else
             add_local(&Kx[(size_t)(Bo[y] + jj) * n + Do[u] + l], s);
@@ -423,41 +466,49 @@ __device__ __forceinline__ void jk_task(const JKArgs& a, const double* __restric
 }
 
 // Slice s (wave-uniform) of a task.
-template <int LA, int LB, int LC, int LD, int S = 0>
-__device__ __forceinline__ void jk_task_slice(const JKArgs& a, const double* __restrict__ boys, int task, int s,
-                                              int lane, double* __restrict__ rows) {
+template <int LA, int LB, int LC, int LD, bool ROWS, int S = 0>
+__device__ __forceinline__ void jk_task_slice(const MolView& a, double screen0, const double* __restrict__ boys,
+                                              int task, int s, int lane, double* __restrict__ rows) {
   if constexpr (S + 1 < slice_count<LA, LB, LC, LD>()) {
     if (s != S) {
-      jk_task_slice<LA, LB, LC, LD, S + 1>(a, boys, task, s, lane, rows);
+      jk_task_slice<LA, LB, LC, LD, ROWS, S + 1>(a, screen0, boys, task, s, lane, rows);
       return;
     }
   }
-  jk_task<LA, LB, LC, LD, S>(a, boys, task, lane, rows);
+  jk_task<LA, LB, LC, LD, ROWS, S>(a, screen0, boys, task, lane, rows);
 }
 
-// Persistent waves: every wave walks the class's task list with a stride of
-// the total wave count (tasks are ordered by job, heaviest first), so the LDS
-// copy of the Boys slice is made once per workgroup.  J and K are updated in
-// global memory (per-XCC copies).
+// Persistent waves: every wave walks the launch's tasks (over all directory
+// entries, in order; each molecule's tasks ordered by job, heaviest first)
+// with a stride of the total wave count, so the LDS copy of the Boys slice is
+// made once per workgroup.  J and K are updated in global memory (per-XCC
+// copies).
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
   __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
   load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
   constexpr int NS = slice_count<LA, LB, LC, LD>();
+  constexpr int CLS = class_index<LA, LB, LC, LD>();
   const int lane = threadIdx.x & 63;
   const int waves = (int)(gridDim.x * (blockDim.x >> 6));
   for (int v = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); v < a.ntask * NS; v += waves) {
-    const int task = __builtin_amdgcn_readfirstlane(v / NS);
-    jk_task_slice<LA, LB, LC, LD>(a, boys, task, __builtin_amdgcn_readfirstlane(v - task * NS), lane, nullptr);
+    const int gt = __builtin_amdgcn_readfirstlane(v / NS);
+    const int e = dir_entry(a.dir_task, a.ndir, gt);
+    const MolTab* T = a.tabs + ((ciptr)a.dir_mol)[e];
+    const MolView m = mol_view(T);
+    const int task = tab_task_begin(T, CLS) + gt - ((ciptr)a.dir_task)[e];
+    jk_task_slice<LA, LB, LC, LD, false>(m, a.screen0, boys, task, __builtin_amdgcn_readfirstlane(v - gt * NS),
+                                         lane, nullptr);
   }
 }
 
-// Persistent workgroups over jobs (tasks of one bra view), taken from a
-// counter in order (heaviest first).  The four waves share the job's tasks
-// and accumulate every K update -- all of them have a bra index as their row
-// -- and the bra Coulomb block in LDS, NR x n rows plus CA x NAB; after the
-// job the workgroup adds the nonzero entries to the global copies and clears
-// them.  Only the ket Coulomb updates go to global memory directly.
+// Persistent workgroups over jobs (tasks of one bra view of one molecule),
+// taken from a counter in order (each molecule's heaviest first).  The four
+// waves share the job's tasks and accumulate every K update -- all of them
+// have a bra index as their row -- and the bra Coulomb block in LDS, NR x n
+// rows plus CA x NAB; after the job the workgroup adds the nonzero entries to
+// the molecule's global copies and clears them.  Only the ket Coulomb updates
+// go to global memory directly.
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves<LA, LB, LC, LD>()))) void k_jk_rows(JKArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB;
@@ -468,29 +519,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
   __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
   __shared__ int job_sh;
   extern __shared__ double rows[];
-  // a.n is the largest molecule's size: it bounds every job's rows
-  for (int e = threadIdx.x; e < NR * a.n + CA * NAB; e += blockDim.x) rows[e] = 0.0;
+  // a.nmax is the largest molecule's size: it bounds every job's rows
+  for (int e = threadIdx.x; e < NR * a.nmax + CA * NAB; e += blockDim.x) rows[e] = 0.0;
   load_boys_slice<LA + LB + LC + LD>(a.boys, boys);  // ends with a barrier
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) job_sh = atomicAdd(a.job_counter, 1);
     __syncthreads();
-    const int job = job_sh;
+    const int job = __builtin_amdgcn_readfirstlane(job_sh);
     __syncthreads();
     if (job >= a.njob) break;
-    const int t0 = a.jobs[job], t1 = a.jobs[job + 1];
+    const int de = dir_entry(a.dir_job, a.ndir, job);
+    const MolTab* T = a.tabs + ((ciptr)a.dir_mol)[de];
+    const MolView m = mol_view(T);
+    const int lj = tab_job_begin(T, CLS) + job - ((ciptr)a.dir_job)[de];
+    const int t0 = ((ciptr)m.jobs)[lj], t1 = ((ciptr)m.jobs)[lj + 1];
     for (int v = t0 * NS + wave; v < t1 * NS; v += nwave) {
       const int task = __builtin_amdgcn_readfirstlane(v / NS);
-      jk_task_slice<LA, LB, LC, LD>(a, boys, task, __builtin_amdgcn_readfirstlane(v - task * NS), lane, rows);
+      jk_task_slice<LA, LB, LC, LD, true>(m, a.screen0, boys, task, __builtin_amdgcn_readfirstlane(v - task * NS),
+                                          lane, rows);
     }
     __syncthreads();
-    const ViewDev& vb = a.views[a.tasks[t0].bra];
-    const ComboDev* __restrict__ cbra = a.combos + vb.c0;
-    const MolCtx mc = mol_ctx(a, __builtin_amdgcn_readfirstlane(vb.mol));
-    const int n = mc.n;
+    const ViewDev& vb = m.views[m.tasks[t0].bra];
+    const ComboDev* __restrict__ cbra = m.combos + vb.c0;
+    const int n = m.n;
     const int nrow = NR * n + CA * NAB;
-    double* __restrict__ Jx = mc.Jx;
-    double* __restrict__ Kx = mc.Kx;
+    double* __restrict__ Jx = m.Jx;
+    double* __restrict__ Kx = m.Kx;
     for (int e = threadIdx.x; e < nrow; e += blockDim.x) {
       const double v = rows[e];
       if (v == 0.0) continue;
@@ -518,9 +573,7 @@ constexpr size_t jk_rows_bytes(int n) {
 }
 
 template <int LA, int LB, int LC, int LD>
-void launch_jk(const JKArgs& a, int ntask, int max_blocks, hipStream_t s) {
-  JKArgs b = a;
-  b.ntask = ntask;
+void launch_jk(const JKArgs& a, int max_blocks, hipStream_t s) {
   const int waves_per_block = 4;
   // Device limits and the resident-workgroup count of this kernel for the
   // current LDS size, queried once per (device, size): the queries cost more
@@ -529,14 +582,14 @@ void launch_jk(const JKArgs& a, int ntask, int max_blocks, hipStream_t s) {
   static thread_local size_t cached_lds = 0;
   int dev = 0;
   HIP_CHECK(hipGetDevice(&dev));
-  const size_t lds = jk_rows_bytes<LA, LB, LC, LD>(a.n);
+  const size_t lds = jk_rows_bytes<LA, LB, LC, LD>(a.nmax);
   if (dev != cached_dev) {
     HIP_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     cached_dev = dev;
     cached_lds = 0;
   }
-  if (a.njob > 0 && lds + sizeof(double) * BOYS_ROWS * BOYS_SLICE + 64 <= (size_t)lds_limit) {
+  if (a.use_rows && a.njob > 0 && lds + sizeof(double) * BOYS_ROWS * BOYS_SLICE + 64 <= (size_t)lds_limit) {
     if (lds != cached_lds) {
       int per_cu = 0;
       HIP_CHECK(hipFuncSetAttribute((const void*)k_jk_rows<LA, LB, LC, LD>,
@@ -547,12 +600,12 @@ void launch_jk(const JKArgs& a, int ntask, int max_blocks, hipStream_t s) {
       cached_lds = lds;
     }
     const int blocks = std::max(1, std::min(std::min(a.njob, max_blocks), resident));
-    hipLaunchKernelGGL((k_jk_rows<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), lds, s, b);
+    hipLaunchKernelGGL((k_jk_rows<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), lds, s, a);
     return;
   }
-  b.tasks = a.tasks + a.task0;
-  const int blocks = std::min(max_blocks, (ntask + waves_per_block - 1) / waves_per_block);
-  hipLaunchKernelGGL((k_jk<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), 0, s, b);
+  if (a.ntask <= 0) return;
+  const int blocks = std::min(max_blocks, (a.ntask + waves_per_block - 1) / waves_per_block);
+  hipLaunchKernelGGL((k_jk<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), 0, s, a);
 }
 
 }  // namespace hfg

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "scf_host.h"
 
 namespace hfg {
 
@@ -236,31 +237,5 @@ struct Eigh {
   }
 };
 
-
-// Gaussian elimination with partial pivoting (src/diis.rs:532-589); false if singular.
-inline bool solve_small(std::vector<double> a, std::vector<double> b, int n, std::vector<double>& x) {
-  for (int i = 0; i < n; ++i) {
-    int piv = i;
-    for (int k = i + 1; k < n; ++k)
-      if (std::fabs(a[(size_t)k * n + i]) > std::fabs(a[(size_t)piv * n + i])) piv = k;
-    if (std::fabs(a[(size_t)piv * n + i]) < 1e-28) return false;
-    if (piv != i) {
-      std::swap(b[i], b[piv]);
-      for (int c = 0; c < n; ++c) std::swap(a[(size_t)i * n + c], a[(size_t)piv * n + c]);
-    }
-    for (int j = i + 1; j < n; ++j) {
-      const double f = a[(size_t)j * n + i] / a[(size_t)i * n + i];
-      for (int k = i; k < n; ++k) a[(size_t)j * n + k] -= f * a[(size_t)i * n + k];
-      b[j] -= f * b[i];
-    }
-  }
-  x.assign(n, 0.0);
-  for (int i = n - 1; i >= 0; --i) {
-    double s = b[i];
-    for (int k = i + 1; k < n; ++k) s -= a[(size_t)i * n + k] * x[k];
-    x[i] = s / a[(size_t)i * n + i];
-  }
-  return true;
-}
 
 }  // namespace hfg

@@ -27,7 +27,7 @@
 
 #include "engine.h"
 #include "la_device.h"
-#include "scf_host.h"
+#include "rhf.h"
 
 namespace hfg {
 
@@ -104,46 +104,6 @@ static bool host_orthogonalizer(int n, const HMat& s, double cut, HMat& x, int& 
     for (int r = 0; r < n; ++r) x[(size_t)r * m + c] = v[(size_t)r * n + keep[c]] / std::sqrt(w[keep[c]]);
   return true;
 }
-// Projected-gradient minimisation of 1/2 c^T Q c + b^T c on the probability
-// simplex, fixed step 1/L with L the Gershgorin bound of sym(Q)
-// (src/diis.rs:195-266); projection by the sort-and-threshold rule.
-static void simplex_project(std::vector<double>& v) {
-  std::vector<double> u = v;
-  std::sort(u.begin(), u.end(), std::greater<double>());
-  double css = 0, theta = 0;
-  for (size_t k = 0; k < u.size(); ++k) {
-    css += u[k];
-    const double t = (css - 1.0) / (double)(k + 1);
-    if (u[k] - t > 0) theta = t;
-  }
-  for (double& x : v) x = std::max(x - theta, 0.0);
-}
-static std::vector<double> simplex_minimise(const HMat& q, const std::vector<double>& b) {
-  const int m = (int)b.size();
-  double lip = 1e-12;
-  for (int i = 0; i < m; ++i) {
-    double s = 0;
-    for (int j = 0; j < m; ++j) s += std::fabs(0.5 * (q[(size_t)i * m + j] + q[(size_t)j * m + i]));
-    lip = std::max(lip, s);
-  }
-  std::vector<double> c(m, 0.0), nx(m);
-  c[m - 1] = 1.0;
-  simplex_project(c);
-  for (int it = 0; it < 5000; ++it) {
-    for (int i = 0; i < m; ++i) {
-      double g = b[i];
-      for (int j = 0; j < m; ++j) g += 0.5 * (q[(size_t)i * m + j] + q[(size_t)j * m + i]) * c[j];
-      nx[i] = c[i] - g / lip;
-    }
-    simplex_project(nx);
-    double d = 0;
-    for (int i = 0; i < m; ++i) d = std::max(d, std::fabs(nx[i] - c[i]));
-    c = nx;
-    if (d < 1e-14) break;
-  }
-  return c;
-}
-
 // Fractional occupations with equal shares inside degenerate groups (|de| < 1e-6).
 static std::vector<double> frac_occ(const std::vector<double>& e, double nel) {
   if (nel < 0 || nel > 2.0 * e.size() + 1e-9) throw HfgError{HFG_INVALID_INPUT, "atomic electron count exceeds basis capacity"};
@@ -314,8 +274,8 @@ static std::string shell_key(const std::vector<Shell>& sh, double z) {
   return os.str();
 }
 
-static HMat sad_guess(const std::vector<Shell>& sh, int n, const std::vector<std::array<double, 4>>& atoms, int charge,
-                      const HMat& S) {
+HMat sad_guess(const std::vector<Shell>& sh, int n, const std::vector<std::array<double, 4>>& atoms, int charge,
+               const HMat& S) {
   static std::mutex mu;
   static std::map<std::string, AtomicDensity> cache;  // src/sad.rs:136-214
   HMat P((size_t)n * n, 0.0);
@@ -352,8 +312,71 @@ static HMat sad_guess(const std::vector<Shell>& sh, int n, const std::vector<std
   return P;
 }
 
+RhfInputs rhf_inputs(const hfg_basis_view& basis, const double* zxyz, int natom, int charge) {
+  RhfInputs in;
+  if (natom <= 0 || !zxyz) throw HfgError{HFG_INVALID_INPUT, "no atoms"};
+  in.atoms.resize(natom);
+  double zsum = 0;
+  for (int a = 0; a < natom; ++a) {
+    for (int k = 0; k < 4; ++k) in.atoms[a][k] = zxyz[4 * a + k];
+    zsum += in.atoms[a][0];
+  }
+  in.nel = (int)std::lround(zsum) - charge;
+  if (in.nel <= 0) throw HfgError{HFG_INVALID_INPUT, "invalid electron count"};
+  if (in.nel % 2) throw HfgError{HFG_INVALID_INPUT, "RHF requires an even number of electrons; use UHF for open shells"};
+  in.nocc = in.nel / 2;
+  if (in.nocc > basis.n_functions) throw HfgError{HFG_INVALID_INPUT, "more occupied orbitals than basis functions"};
+  for (int i = 0; i < natom; ++i)
+    for (int j = i + 1; j < natom; ++j) {
+      const double dx = in.atoms[i][1] - in.atoms[j][1], dy = in.atoms[i][2] - in.atoms[j][2],
+                   dz = in.atoms[i][3] - in.atoms[j][3];
+      in.enuc += in.atoms[i][0] * in.atoms[j][0] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  return in;
+}
+
+// Canonical orthogonaliser X = Tr U_kept s^{-1/2} of S' = Tr^T S Tr (src/ao.rs:58-67,
+// Matrix::canonical_orthogonalizer src/matrix.rs:329-346, cut 1e-8), the
+// eigensolver on the device; X (n x m) returned on the host.
+int orthogonalizer(Blas& bl, hipStream_t st, const std::vector<Shell>& shells, int n, const double* dS, bool spherical,
+                   HMat& X) {
+  HMat Tr;
+  int m0 = n;
+  if (spherical) spherical_transform(shells, n, Tr, m0);
+  Dev dSp, dTr, dTmp;
+  dSp.alloc((size_t)m0 * m0);
+  if (spherical) {
+    dTr.alloc((size_t)n * m0);
+    dTmp.alloc((size_t)n * m0);
+    upload(dTr.p, Tr.data(), (size_t)n * m0, st);
+    bl.gemm(false, false, n, m0, n, 1.0, dS, n, dTr.p, m0, 0.0, dTmp.p, m0);      // S Tr
+    bl.gemm(true, false, m0, m0, n, 1.0, dTr.p, m0, dTmp.p, m0, 0.0, dSp.p, m0);  // Tr^T S Tr
+  } else {
+    HIP_CHECK(hipMemcpyAsync(dSp.p, dS, (size_t)n * n * 8, hipMemcpyDeviceToDevice, st));
+  }
+  Dev dw;
+  dw.alloc(m0);
+  Eigh eig;
+  eig.run(bl, dSp.p, dw.p, m0);
+  std::vector<double> w(m0);
+  download(w.data(), dw.p, m0, st);
+  if (w[0] < -1e-8) throw HfgError{HFG_INTEGRAL, "overlap has negative eigenvalue"};
+  int first = 0;
+  while (first < m0 && w[first] < 1e-8) ++first;
+  const int m = m0 - first;
+  if (m <= 0) throw HfgError{HFG_INTEGRAL, "overlap has zero retained rank"};
+  // U (row-major m0 x m0, columns = vectors) is the transpose of dSp's row-major view.
+  HMat U((size_t)m0 * m0);
+  download(U.data(), dSp.p, (size_t)m0 * m0, st);
+  HMat Xp((size_t)m0 * m);
+  for (int c = 0; c < m; ++c)
+    for (int r = 0; r < m0; ++r) Xp[(size_t)r * m + c] = U[(size_t)(first + c) * m0 + r] / std::sqrt(w[first + c]);
+  X = spherical ? hmm(Tr, Xp, n, m0, m) : Xp;
+  return m;
+}
+
 int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
-            hfg_scf_result* res, double* p_out, double* f_out, double* eps_out) {
+            hfg_scf_result* res, double* p_out, double* f_out, double* eps_out, const double* p_guess) {
   using clk = std::chrono::steady_clock;
   const auto t_start = clk::now();
   std::memset(res, 0, sizeof *res);
@@ -361,18 +384,10 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   if (opt->max_iter <= 0 || basis->n_functions <= 0 || !(tol > 0) || !(opt->residual_tol > 0) ||
       !(opt->density_tol > 0))
     throw HfgError{HFG_INVALID_INPUT, "SCF requires a nonempty basis, positive tolerance and iteration limit"};
-  std::vector<std::array<double, 4>> atoms(natom);
-  double zsum = 0;
-  for (int a = 0; a < natom; ++a) {
-    for (int k = 0; k < 4; ++k) atoms[a][k] = zxyz[4 * a + k];
-    zsum += atoms[a][0];
-  }
-  const int nel = (int)std::lround(zsum) - charge;
-  if (nel <= 0) throw HfgError{HFG_INVALID_INPUT, "invalid electron count"};
-  if (nel % 2) throw HfgError{HFG_INVALID_INPUT, "RHF requires an even number of electrons; use UHF for open shells"};
-  const int nocc = nel / 2;
+  const RhfInputs in = rhf_inputs(*basis, zxyz, natom, charge);
+  const std::vector<std::array<double, 4>>& atoms = in.atoms;
+  const int nocc = in.nocc;
   const int n = basis->n_functions;
-  if (nocc > n) throw HfgError{HFG_INVALID_INPUT, "more occupied orbitals than basis functions"};
 
   auto t_phase = clk::now();
   auto lap = [&]() {
@@ -398,21 +413,12 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   hipStream_t st = eng.stream;
   Blas bl(st);
   const size_t nn = (size_t)n * n;
-  double enuc = 0;
-  for (int i = 0; i < natom; ++i)
-    for (int j = i + 1; j < natom; ++j) {
-      const double dx = atoms[i][1] - atoms[j][1], dy = atoms[i][2] - atoms[j][2], dz = atoms[i][3] - atoms[j][3];
-      enuc += atoms[i][0] * atoms[j][0] / std::sqrt(dx * dx + dy * dy + dz * dz);
-    }
+  const double enuc = in.enuc;
   one_e.get();  // the remaining wait, if any, is the one-electron phase
   HMat H(nn);
   for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
   res->seconds_one_electron = lap();
 
-  // Orthogonaliser X = Tr * U_kept s^{-1/2} of S' = Tr^T S Tr (src/ao.rs:58-67).
-  HMat Tr;
-  int m0 = n;
-  if (opt->spherical) spherical_transform(eng.shells, n, Tr, m0);
   Dev dS, dH, dX, dTmp, dTmp2, dF, dP, dPprev, dErr, dW, dC, dFu;
   dS.alloc(nn);
   dH.alloc(nn);
@@ -430,35 +436,9 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   Eigh eig;
   int m = 0;
   {
-    // S' (m0 x m0) on the device
-    Dev dTr, dSp;
-    dSp.alloc((size_t)m0 * m0);
-    if (opt->spherical) {
-      dTr.alloc((size_t)n * m0);
-      upload(dTr.p, Tr.data(), (size_t)n * m0, st);
-      bl.gemm(false, false, n, m0, n, 1.0, dS.p, n, dTr.p, m0, 0.0, dTmp.p, m0);        // S Tr
-      bl.gemm(true, false, m0, m0, n, 1.0, dTr.p, m0, dTmp.p, m0, 0.0, dSp.p, m0);      // Tr^T S Tr
-    } else {
-      HIP_CHECK(hipMemcpyAsync(dSp.p, dS.p, nn * 8, hipMemcpyDeviceToDevice, st));
-    }
-    Dev dw;
-    dw.alloc(m0);
-    eig.run(bl, dSp.p, dw.p, m0);
-    std::vector<double> w(m0);
-    download(w.data(), dw.p, m0, st);
-    if (w[0] < -1e-8) throw HfgError{HFG_INTEGRAL, "overlap has negative eigenvalue"};
-    int first = 0;
-    while (first < m0 && w[first] < 1e-8) ++first;
-    m = m0 - first;
-    if (m <= 0) throw HfgError{HFG_INTEGRAL, "overlap has zero retained rank"};
+    HMat X;
+    m = orthogonalizer(bl, st, eng.shells, n, dS.p, opt->spherical != 0, X);
     if (nocc > m) throw HfgError{HFG_INVALID_INPUT, "more occupied orbitals than retained overlap rank"};
-    // U (row-major m0 x m0, columns = vectors) is the transpose of dSp's row-major view.
-    HMat U((size_t)m0 * m0);
-    download(U.data(), dSp.p, (size_t)m0 * m0, st);
-    HMat Xp((size_t)m0 * m);
-    for (int c = 0; c < m; ++c)
-      for (int r = 0; r < m0; ++r) Xp[(size_t)r * m + c] = U[(size_t)(first + c) * m0 + r] / std::sqrt(w[first + c]);
-    HMat X = opt->spherical ? hmm(Tr, Xp, n, m0, m) : Xp;
     dX.alloc((size_t)n * m);
     upload(dX.p, X.data(), (size_t)n * m, st);
   }
@@ -548,12 +528,18 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   // Initial density: SAD, else core Hamiltonian (src/scf.rs:452-471).  The
   // reference also diagonalises the SAD Fock matrix for the orbital-rotation
   // accelerators only; the Fock-extrapolation family used here never reads it.
+  // A given density replaces the guess (run_scf_with_initial_density,
+  // src/scf.rs:630-639).
   bool have_sad = true;
   HMat P0;
-  try {
-    P0 = sad_guess(eng.shells, n, atoms, charge, S);
-  } catch (HfgError&) {
-    have_sad = false;
+  if (p_guess) {
+    P0.assign(p_guess, p_guess + nn);
+  } else {
+    try {
+      P0 = sad_guess(eng.shells, n, atoms, charge, S);
+    } catch (HfgError&) {
+      have_sad = false;
+    }
   }
   if (have_sad)
     upload(dP.p, P0.data(), nn, st);

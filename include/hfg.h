@@ -69,10 +69,24 @@ int hfg_shell_quartet(hfg_ctx* ctx, int sa, int sb, int sc, int sd, double* out)
  * (src/scf.rs:275-278) for callers that keep HFResult.eri. */
 int hfg_eri_packed(hfg_ctx* ctx, double* out, size_t len);
 
+/* Two-electron integrals in the MO basis of the n_mo columns of C (n x n_mo,
+ * row-major): (pq|rs) at out[rs * np + pq], np = n_mo(n_mo+1)/2 and pq, rs
+ * compound indices p >= q (len = np * np).  The AO tensor is the screened
+ * packed one (hfg_eri_packed); both half transformations run on the device.
+ * n <= 200.  Replaces transform_eri_ao_to_mo (src/mo_integrals.rs:156-265),
+ * the reader of HFResult.eri (src/mo_integrals.rs:505). */
+int hfg_eri_mo(hfg_ctx* ctx, const double* c, int n_mo, double* out, size_t len);
+
 /* Direct J and K for a symmetric total density P (n x n), over the unique
  * shell quartets with Q_ab*Q_cd > 1e-12.  Replaces contract_jk
  * (src/scf.rs:337-364) with the packed tensor never formed. */
 int hfg_contract_jk(hfg_ctx* ctx, const double* p, double* j, double* k);
+/* J of the total density and K of each spin density, as
+ * contract_jk(total, spins, eri, n) -> (J, Vec<K>) (src/scf.rs:337-364):
+ * RHF passes spins = {P} (the same pointer as p_total: one build), UHF
+ * {P_alpha, P_beta} (one build per distinct density).  All n x n. */
+int hfg_contract_jk_multi(hfg_ctx* ctx, const double* p_total, int nspin, const double* const* p_spin, double* j,
+                          double* const* k_spin);
 /* F = H + J - K/2.  Replaces build_fock_matrix (src/scf.rs:366-369). */
 int hfg_build_fock(hfg_ctx* ctx, const double* h, const double* p, double* f);
 /* Same as hfg_contract_jk on device pointers (HBM-resident P, J, K), on the
@@ -144,6 +158,12 @@ int hfg_scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int 
                 const hfg_scf_options* opt, hfg_scf_result* res, double* p_out, double* f_out,
                 double* eps_out);
 
+/* The same from a given initial density P (n x n) instead of SAD.  Replaces
+ * run_scf_with_initial_density (src/scf.rs:630-639). */
+int hfg_scf_rhf_guess(const hfg_basis_view* basis, const double* zxyz, int natom, int charge,
+                      const hfg_scf_options* opt, const double* p_guess, hfg_scf_result* res, double* p_out,
+                      double* f_out, double* eps_out);
+
 /* Density fitting with a Coulomb metric.  hfg_df_build computes the metric
  * (P|Q), its canonical orthogonaliser X (eigenvalues below metric_cutoff
  * dropped), the 3-index integrals (P|mu nu) with the unit function of
@@ -211,6 +231,58 @@ int hfg_batch_jk(hfg_batch* batch);
 int hfg_batch_jk_each(hfg_batch* batch);
 /* Copies molecule mol's J and K (either may be NULL) to the host. */
 int hfg_batch_get_jk(hfg_batch* batch, int mol, double* j, double* k);
+/* Instrumentation of the fused builds, as hfg_profile / hfg_class_stats for
+ * one molecule: per class, the algorithmic FLOPs and quartets of one build of
+ * the whole batch, and the measured milliseconds over the profiled builds. */
+int hfg_batch_profile(hfg_batch* batch, int enable);
+int hfg_batch_class_stats(hfg_batch* batch, int cls, int32_t* l4, double* flops_per_launch, int64_t* quartets,
+                          double* ms_total, int64_t* launches);
+
+/* Served and batched RHF (no reference counterpart: the reference solves one
+ * molecule per call of run_scf_in_space_with_convergence, src/scf.rs:410-627,
+ * or run_scf_with_initial_density, :630-639).  A server holds `capacity`
+ * slots; each admitted request occupies one and every server iteration
+ * advances all of them by one RHF iteration with the Fock builds of all slots
+ * fused into one launch per shell-quartet class and the DIIS/purification
+ * linear algebra batched over the slots.  A finished slot is refilled from the
+ * queue of requests whose arrival time has passed, while the resident HBM
+ * footprint stays within the budget (continuous batching).  A request is a
+ * trajectory of geometry steps of one molecule: each step is an RHF solve
+ * with the given options, started from the previous step's density (SAD for
+ * the first); the per-molecule arithmetic is that of hfg_scf_rhf. */
+typedef struct hfg_server hfg_server;
+typedef struct {
+  int32_t capacity;     /* slots: molecules resident at once */
+  double hbm_budget_gb; /* bound on the resident footprint; <= 0: 80% of free HBM at creation */
+} hfg_server_options;
+typedef struct {
+  double energy; /* total RHF energy of the step (Eh) */
+  int32_t iterations, converged, fock_builds, n;
+} hfg_step_result;
+typedef struct {
+  double arrival_s, admitted_s, finished_s; /* seconds from the start of hfg_server_run */
+  int32_t steps, converged, status, n;      /* status: HFG_* of the request */
+} hfg_request_result;
+typedef struct {
+  int64_t fused_builds, iterations;
+  double seconds, seconds_setup, seconds_sync, hbm_budget_bytes;
+} hfg_server_stats_t;
+int hfg_server_create(const hfg_server_options* opt, hfg_server** out);
+void hfg_server_destroy(hfg_server* server);
+/* Queues a request: nsteps geometries of one molecule (bases[k] and zxyz[k],
+ * natom atoms each, copied), arriving arrival_s seconds after the start of
+ * the run.  keep_density: keep the last step's density for hfg_server_result. */
+int hfg_server_submit(hfg_server* server, double arrival_s, int nsteps, const hfg_basis_view* bases,
+                      const double* const* zxyz, int natom, int charge, const hfg_scf_options* opt, int keep_density,
+                      int* request_id);
+/* Serves every queued request (arrival times against the wall clock from the
+ * start of this call) and returns when all have finished. */
+int hfg_server_run(hfg_server* server);
+/* A request's timing and per-step results (steps: nsteps entries, may be
+ * NULL), and the last step's density (n x n, may be NULL; needs keep_density). */
+int hfg_server_result(hfg_server* server, int request_id, hfg_request_result* result, hfg_step_result* steps,
+                      double* p_out);
+int hfg_server_stats(const hfg_server* server, hfg_server_stats_t* stats);
 
 /* Host-side basis construction from Gaussian-94 text, mirroring build_basis
  * (src/basis.rs:296-496).  The arrays are owned by the returned handle. */

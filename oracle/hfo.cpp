@@ -1220,7 +1220,9 @@ struct Cdiis {
     f.push_back(F);
     e.push_back(E);
   }
-  bool extrapolate(Mat& out) const {
+  // Pulay coefficients: the bordered system [B -1; -1 0] [c; l] = [0; -1]
+  // with B_ij = <e_i, e_j> (src/diis.rs:72-137).
+  bool coefficients(std::vector<double>& c) const {
     int m = (int)e.size();
     if (m < 2) return false;
     int dim = m + 1;
@@ -1233,9 +1235,16 @@ struct Cdiis {
       b[(size_t)i * dim + m] = -1.0;
       b[(size_t)m * dim + i] = -1.0;
     }
-    std::vector<double> rhs(dim, 0.0), c;
+    std::vector<double> rhs(dim, 0.0);
     rhs[m] = -1.0;
     if (!solve_linear(b, rhs, dim, c)) return false;
+    c.resize(m);
+    return true;
+  }
+  bool extrapolate(Mat& out) const {
+    std::vector<double> c;
+    if (!coefficients(c)) return false;
+    const int m = (int)c.size();
     out.assign(f[0].size(), 0.0);
     for (int i = 0; i < m; ++i)
       for (size_t k = 0; k < out.size(); ++k) out[k] += c[i] * f[i][k];
@@ -1295,9 +1304,10 @@ struct Adiis {
     p.push_back(P);
     en.push_back(E);
   }
-  bool extrapolate(Mat& out) const {
+  // The quadratic model f(c) = E_n + b^T c + 1/2 c^T Q c of the energy of
+  // sum_i c_i P_i about the newest entry n (src/diis.rs:325-333).
+  void model(std::vector<std::vector<double>>& q, std::vector<double>& b) const {
     int m = (int)f.size();
-    if (m < 2) return false;
     int nl = m - 1;
     std::vector<Mat> dp(m), df(m);
     for (int i = 0; i < m; ++i) {
@@ -1308,12 +1318,19 @@ struct Adiis {
         df[i][k] -= f[nl][k];
       }
     }
-    std::vector<double> b(m);
-    std::vector<std::vector<double>> q(m, std::vector<double>(m));
+    b.assign(m, 0.0);
+    q.assign(m, std::vector<double>(m));
     for (int i = 0; i < m; ++i) {
       b[i] = frob(dp[i], f[nl]);
       for (int j = 0; j < m; ++j) q[i][j] = frob(dp[i], df[j]);
     }
+  }
+  bool extrapolate(Mat& out) const {
+    int m = (int)f.size();
+    if (m < 2) return false;
+    std::vector<double> b;
+    std::vector<std::vector<double>> q;
+    model(q, b);
     std::vector<double> st(m, 0.0);
     st[m - 1] = 1.0;
     auto c = min_quad_simplex(q, b, st);
@@ -2142,4 +2159,47 @@ int hfo_fock_flops(const hfo_basis* b, double* flops, double* quartets, double* 
   return hfo_fock_flops_sample(b, 1, 0, flops, quartets, prim_quartets);
 }
 
+
+// DIIS pieces on their own, for the tests that pin them.
+void hfo_project_simplex(double* v, int m) {
+  std::vector<double> x(v, v + m);
+  project_simplex(x);
+  std::copy(x.begin(), x.end(), v);
+}
+void hfo_min_quad_simplex(const double* q, const double* b, int m, double* c) {
+  std::vector<std::vector<double>> qq(m, std::vector<double>(m));
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) qq[i][j] = q[(size_t)i * m + j];
+  std::vector<double> st(m, 0.0);
+  st[m - 1] = 1.0;
+  auto r = min_quad_simplex(qq, std::vector<double>(b, b + m), st);
+  std::copy(r.begin(), r.end(), c);
+}
+// errors: m entries of len doubles each; c: m coefficients.
+int hfo_cdiis_coefficients(const double* errors, int m, int len, double* c) {
+  Cdiis d;
+  d.maxs = (size_t)std::max(m, 1);
+  for (int i = 0; i < m; ++i) d.push(Mat(len, 0.0), Mat(errors + (size_t)i * len, errors + (size_t)(i + 1) * len));
+  std::vector<double> cc;
+  if (!d.coefficients(cc)) return fail("CDIIS system is singular or has fewer than two entries");
+  std::copy(cc.begin(), cc.end(), c);
+  return 0;
+}
+// dens, focks: m entries of len doubles each (oldest first); q (m x m), b (m).
+int hfo_adiis_model(const double* dens, const double* focks, int m, int len, double* q, double* b) {
+  if (m < 1) return fail("empty ADIIS history");
+  Adiis a;
+  a.maxs = (size_t)m;
+  for (int i = 0; i < m; ++i)
+    a.push(Mat(focks + (size_t)i * len, focks + (size_t)(i + 1) * len),
+           Mat(dens + (size_t)i * len, dens + (size_t)(i + 1) * len), 0.0);
+  std::vector<std::vector<double>> qq;
+  std::vector<double> bb;
+  a.model(qq, bb);
+  for (int i = 0; i < m; ++i) {
+    b[i] = bb[i];
+    for (int j = 0; j < m; ++j) q[(size_t)i * m + j] = qq[i][j];
+  }
+  return 0;
+}
 }  // extern "C"

@@ -271,9 +271,6 @@ def test_rhf_densities_by_purification_match_the_eigensolver(hf, monkeypatch):
     assert abs(r["iterations"] - e["iterations"]) <= 1
 
 
-@pytest.mark.skip(reason="a single-molecule J/K run in the same process after this batch ended with an HSA "
-                         "queue abort (HSA_STATUS_ERROR_OUT_OF_RESOURCES) on the (dd|dd) kernel, which uses 13.9 KB "
-                         "of scratch per lane; cause not established, case kept off the GPU")
 def test_batch_jk_matches_single_molecule_builds(hf):
     """Fused multi-molecule J/K (mixed sizes, bases and shell classes) equals
     each molecule's own build and the oracle's J/K."""
@@ -304,3 +301,47 @@ def test_batch_jk_matches_single_molecule_builds(hf):
         j, k = batch.get_jk(i)
         j1, k1 = hf.Context(bases[i]).contract_jk(dens[i])
         assert np.abs(j - j1).max() < 1e-11 * max(1.0, np.abs(j1).max())
+
+
+@pytest.mark.timeout(900)
+def test_batch_config3_subset_matches_oracle(hf):
+    """The first 16 molecules of the batched def2-SVP workload (alkanes and
+    water clusters, seed 3; n = 25..285): one fused J/K build of all of them,
+    each molecule's J and K against the oracle's direct J/K at 1e-10."""
+    mols = M.batch_config3(512, seed=3)[:16]
+    text = M.basis_text("def2-svp")
+    bases = [hf.Basis.from_g94(text, m.symbols, m.xyz) for m in mols]
+    batch = hf.Batch(bases)
+    rng = np.random.default_rng(16)
+    dens = []
+    for i, b in enumerate(bases):
+        c = rng.standard_normal((b.n, max(1, b.n // 5))) * 0.3
+        dens.append(c @ c.T)
+        batch.set_density(i, dens[-1])
+    batch.jk()
+    worst = 0.0
+    for i, m in enumerate(mols):
+        j, k = batch.get_jk(i)
+        jo, ko, _ = O.Basis.build(text, m.symbols, m.xyz).direct_jk(dens[i])
+        ej, ek = np.abs(j - jo).max(), np.abs(k - ko).max()
+        print("%s n=%d  J %.2e  K %.2e" % (m.name, bases[i].n, ej, ek), flush=True)
+        worst = max(worst, ej, ek)
+        assert ej <= 1e-10 and ek <= 1e-10, (i, m.name)
+    print("config-3 subset: worst |dJ|,|dK| vs oracle %.2e" % worst)
+
+
+def test_global_atomic_single_stream_path_matches_oracle(hf, monkeypatch):
+    """The J/K kernels without LDS rows (k_jk, global per-XCC atomics; the
+    path classes take when the rows do not fit the LDS) on one stream
+    (HFG_JK_ROWS=0, HFG_SIDE_BLOCKS=0), against the oracle at 1e-10."""
+    monkeypatch.setenv("HFG_JK_ROWS", "0")
+    monkeypatch.setenv("HFG_SIDE_BLOCKS", "0")
+    for mol, basis in ((M.water_cluster((2, 1, 1), seed=2), "cc-pvdz"), (M.alkane(3), "def2-svp")):
+        text = M.basis_text(basis)
+        ctx = hf.Context(hf.Basis.from_g94(text, mol.symbols, mol.xyz))
+        p = sym_density(ctx.n, 44)
+        j, k = ctx.contract_jk(p)
+        jr, kr, _ = O.Basis.build(text, mol.symbols, mol.xyz).direct_jk(p)
+        ej, ek = np.abs(j - jr).max(), np.abs(k - kr).max()
+        print("global-atomic %s %s n=%d  J %.2e  K %.2e" % (mol.name, basis, ctx.n, ej, ek))
+        assert ej <= 1e-10 and ek <= 1e-10
