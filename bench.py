@@ -139,7 +139,7 @@ def df_report(hf, M, steps=5):
 PROFILE = os.path.join(ROOT, "profiles", "fused_jk_config3_def2svp.json")
 
 
-def fused_report(hf, M, count, steps, warmup):
+def fused_report(hf, M, count, steps, warmup, unbatched=True):
     """The hot path of the batched workload: one step = J and K of all
     `count` molecules' resident densities in fused launches (one per
     shell-quartet class over the whole batch).  Timed over `steps` builds with
@@ -172,11 +172,13 @@ def fused_report(hf, M, count, steps, warmup):
     elapsed = time.perf_counter() - t0
     batch.profile(False)
     stats = batch.class_stats()
-    batch.jk_each()
-    t1 = time.perf_counter()
-    for _ in range(max(1, steps // 4)):
+    each = float("nan")
+    if unbatched:
         batch.jk_each()
-    each = (time.perf_counter() - t1) / max(1, steps // 4)
+        t1 = time.perf_counter()
+        for _ in range(max(1, steps // 4)):
+            batch.jk_each()
+        each = (time.perf_counter() - t1) / max(1, steps // 4)
     fused = elapsed / steps
     dom = max(stats, key=lambda x: x["ms"] / max(1, x["launches"]))
     dom_ms = dom["ms"] / max(1, dom["launches"])
@@ -205,8 +207,9 @@ def fused_report(hf, M, count, steps, warmup):
            "fock_builds_per_s": count / fused, "ms_per_batch_build": 1e3 * fused,
            "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
            "shell_quartets": info["quartets"], "md_algorithmic_tflops": info["flops"] / fused / 1e12,
-           "unbatched_fock_builds_per_s": count / each, "unbatched_ms_per_batch": 1e3 * each,
-           "speedup_vs_unbatched": each / fused, "setup_s": setup,
+           "unbatched_fock_builds_per_s": count / each if unbatched else None,
+           "unbatched_ms_per_batch": 1e3 * each if unbatched else None,
+           "speedup_vs_unbatched": each / fused if unbatched else None, "setup_s": setup,
            "classes": [{"l": x["l"], "ms": x["ms"] / max(1, x["launches"]), "quartets": x["quartets"]}
                        for x in stats if x["quartets"]]}
     batch.close()
@@ -315,6 +318,7 @@ def scf_batch_report(hf, M, count=512, capacity=0, plain_every=16):
             "all_converged": all(r["converged"] for r in res),
             "solve_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
             "fused_fock_builds": st["fused_builds"], "seconds_setup": st["seconds_setup"],
+            "jk_config": "%d of %d (autotuned)" % (st["jk_config"], st["jk_candidates"]),
             "plain_gpu": {"solves_per_s": plain_solves, "sample": len(sample),
                           "sample_rule": "molecules with index %% %d in {0, 1}" % plain_every},
             "speedup_vs_plain": solves / plain_solves,
@@ -408,7 +412,8 @@ def summarize_profiles(trace_csv, pmc_dirs, copy_dirs, out_json, workload, comma
         return out
 
     def short(name):
-        return name.replace(" ", "").split("(")[0].replace("hfg::", "")
+        k = name.replace(" ", "").split("(")[0].replace("hfg::", "")
+        return k[4:] if k.startswith("void") else k
     per = {}
     for r in rows(trace_csv, "*kernel_stats.csv"):
         per.setdefault(short(r["Name"]), {})["avg_us"] = float(r["AverageNs"]) / 1e3
@@ -442,7 +447,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5, help="timed fused Fock builds of the batch")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=6,
+                    help="untimed fused builds (the first five also tune the class-launch configuration)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--only-fused", action="store_true", help="only the fused Fock builds (profiling runs)")
     ap.add_argument("--no-solves", action="store_true", help="skip the batched SCF solves (the headline)")
@@ -485,7 +491,7 @@ def main():
     if hf.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    fused, roof = fused_report(hf, M, args.batch_count, args.steps, args.warmup)
+    fused, roof = fused_report(hf, M, args.batch_count, args.steps, args.warmup, unbatched=not args.only_fused)
     progress("fused builds: %.1f ms per batch build" % fused["ms_per_batch_build"])
     line = {"metric": "Fock builds/s & SCF solves/s at |dE|<=1e-8 Eh vs plain GPU; p50/p99 solve latency",
             "value": None, "unit": "SCF solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

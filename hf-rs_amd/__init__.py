@@ -77,7 +77,8 @@ class RequestResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("fused_builds", C.c_int64), ("iterations", C.c_int64), ("seconds", C.c_double),
-                ("seconds_setup", C.c_double), ("seconds_sync", C.c_double), ("hbm_budget_bytes", C.c_double)]
+                ("seconds_setup", C.c_double), ("seconds_sync", C.c_double), ("hbm_budget_bytes", C.c_double),
+                ("jk_config", C.c_int32), ("jk_candidates", C.c_int32)]
 
 
 class DfhfOptions(C.Structure):

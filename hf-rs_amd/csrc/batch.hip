@@ -2,6 +2,7 @@
 // points: Fock builds of many molecules in fused launches.
 #include <algorithm>
 #include <cstring>
+#include <cstdio>
 #include <memory>
 
 #include "../../include/hfg.h"
@@ -63,11 +64,30 @@ Fleet::Fleet(int cap) : capacity(cap) {
   auto tab = boys_table();
   d_boys.alloc(tab.size() * sizeof(double));
   HIP_CHECK(hipMemcpyAsync(d_boys.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  slot_quartets.assign(cap, 0.0);
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // Candidates: the default split (the one measured best on (H2O)16/cc-pVDZ),
+  // one stream, only the dp/dd-bra classes aside, the default split with a
+  // larger side grid, only the dd-bra classes aside.
+  const unsigned def = default_side_mask();
+  unsigned dpdd = 0, dd = 0;
+  for (int c = 10; c < HFG_NCLASS; ++c) dpdd |= 1u << c;
+  for (int c = 15; c < HFG_NCLASS; ++c) dd |= 1u << c;
+  candidates = {{def, side_blocks, 4}, {def, 0, 4}, {dpdd, side_blocks, 4}, {def, 2 * side_blocks, 4}, {dd, side_blocks / 2, 4}};
+  score.assign(candidates.size(), -1.0);
+  const char* at = std::getenv("HFG_AUTOTUNE");
+  tuned = side_blocks == 0 || (at && std::atoi(at) == 0);
+  HIP_CHECK(hipEventCreate(&tb));
+  HIP_CHECK(hipEventCreate(&te));
   sync();
 }
 
 Fleet::~Fleet() {
   for (auto e : events) (void)hipEventDestroy(e);
+  if (tb) (void)hipEventDestroy(tb);
+  if (te) (void)hipEventDestroy(te);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (side) (void)hipStreamDestroy(side);
@@ -78,12 +98,14 @@ std::unique_ptr<Engine> Fleet::make_engine(const hfg_basis_view& b) {
   return std::unique_ptr<Engine>(new Engine(b, stream));
 }
 
-void Fleet::attach(int s, const Engine& e) {
+void Fleet::attach(int s, const Engine& e, hipStream_t st) {
   if (s < 0 || s >= capacity) throw HfgError{HFG_INVALID_INPUT, "fleet slot out of range"};
+  if (!st) st = stream;
   tabs[s] = e.tab_host;
   occupied[s] = 1;
-  HIP_CHECK(hipMemcpyAsync(d_tabs.as<MolTab>() + s, &tabs[s], sizeof(MolTab), hipMemcpyHostToDevice, stream));
-  HIP_CHECK(hipStreamSynchronize(stream));  // the host copy may change before the next build
+  slot_quartets[s] = (double)e.n_quartets;
+  HIP_CHECK(hipMemcpyAsync(d_tabs.as<MolTab>() + s, &tabs[s], sizeof(MolTab), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipStreamSynchronize(st));  // the host copy may change before the next build
 }
 
 void Fleet::detach(int s) {
@@ -105,14 +127,47 @@ void Fleet::set_active(const std::vector<int>& slots) {
   HIP_CHECK(hipStreamSynchronize(stream));  // host vectors are reused by the next call
 }
 
-void Fleet::build() {
+void Fleet::tune_step(bool full_build) {
+  if (pending >= 0) {
+    // the previous build's time (its events are complete once the stream
+    // has run past them)
+    HIP_CHECK(hipEventSynchronize(te));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, tb, te));
+    if (pending_work > 0) score[pending] = ms / pending_work;
+    pending = -1;
+    size_t next = 0;
+    while (next < score.size() && score[next] >= 0) ++next;
+    if (next == score.size()) {
+      current = (int)(std::min_element(score.begin(), score.end()) - score.begin());
+      tuned = true;
+      if (std::getenv("HFG_TRACE"))
+        for (size_t i = 0; i < score.size(); ++i)
+          fprintf(stderr, "hfg autotune: candidate %zu (side mask %#x, cap %d): %.3e ms/quartet%s\n", i,
+                  candidates[i].side_mask, candidates[i].side_cap, score[i], (int)i == current ? "  <- chosen" : "");
+    } else {
+      current = (int)next;
+    }
+  }
+  if (!tuned && full_build) {
+    pending = current;
+    pending_work = 0;
+    for (int s : active) pending_work += slot_quartets[s];
+  }
+}
+
+void Fleet::build(bool full_build) {
   const int count = (int)active.size();
   if (!count) return;
+  if (!tuned) tune_step(full_build);
   hipLaunchKernelGGL(k_fleet_clear, dim3(64, count), dim3(256), 0, stream, d_tabs.as<MolTab>(), d_active.as<int>(),
                      count);
+  const JKConfig& cfg = candidates[pending >= 0 ? pending : current];
   JKLaunch L{d_counters.as<int>(), ev_fork, ev_join, d_boys.as<double>(), screen0, use_rows,
-             profile ? events.data() : nullptr};
-  launch_jk_set(L, d_tabs.as<MolTab>(), dir, d_dir.as<int>(), stream, side, side_blocks);
+             profile ? events.data() : nullptr, cfg.side_mask, cfg.blocks_per_cu * cus};
+  if (pending >= 0) HIP_CHECK(hipEventRecord(tb, stream));
+  launch_jk_set(L, d_tabs.as<MolTab>(), dir, d_dir.as<int>(), stream, side, cfg.side_cap);
+  if (pending >= 0) HIP_CHECK(hipEventRecord(te, stream));
   if (profile) {
     sync();
     for (int c = 0; c < HFG_NCLASS; ++c) {

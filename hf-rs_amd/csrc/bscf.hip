@@ -32,15 +32,18 @@
 #include <memory>
 #include <numeric>
 #include <thread>
+#include <atomic>
+#include <array>
 
 #include "../../include/hfg.h"
+#include "bscf_layout.h"
 #include "fleet.h"
 #include "la_batch.h"
 #include "rhf.h"
 
 namespace hfg {
 
-constexpr int BMAXS = 8;  // DIIS subspace (src/diis.rs:53-70)
+constexpr int BMAXS = SLOT_DIIS;  // DIIS subspace (src/diis.rs:53-70)
 
 // One slot's matrices as the batched kernels address them (row-major).
 struct BMol {
@@ -412,7 +415,9 @@ struct Request {
   std::string error;
 };
 
-enum Phase { FREE = 0, ITER, PRE, FINAL };
+// FREE: empty; ITER: iterating; PRE: one more build after max_iter; FINAL:
+// the final build; NEXT: finished a geometry step, the next one's set-up is due.
+enum Phase { FREE = 0, ITER, PRE, FINAL, NEXT };
 
 struct Slot {
   Phase phase = FREE;
@@ -432,45 +437,47 @@ struct Slot {
   double G[BMAXS][BMAXS];
   bool converged = false;
   double e_final = 0;
+  std::vector<double> guess;  // density handed to the next geometry step (NEXT)
 };
 
-static size_t slot_doubles(int n, int m) {
-  const size_t nn = (size_t)n * n, mm = (size_t)m * m;
-  // S H X F P Pprev G Pb T1 T2 E Fu (n x n), Fo Xs X2 (m x m), cF aF aP (8 n^2), cE (8 elen)
-  return 12 * nn + 3 * mm + 3 * BMAXS * nn + BMAXS * std::max(nn, mm) + 64;
-}
+// A set-up worker: its own stream and BLAS handle, so that several slots are
+// set up at once (engine, one-electron matrices, orthogonaliser, guess).
+struct Worker {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Blas> blas;
+};
 
+static size_t slot_doubles(int n, int m) { return slot_layout(n, m, nullptr); }
+
+// Carves the slot's buffer by slot_layout (every sub-buffer 256-byte aligned).
 static void layout(Slot& s) {
+  size_t off[SM_COUNT];
+  const size_t total = slot_layout(s.n, s.m, off);
+  if (total > s.cap) throw HfgError{HFG_DEVICE, "slot buffer smaller than its layout"};
   double* p = s.mem.as<double>();
-  const size_t nn = (size_t)s.n * s.n, mm = (size_t)s.m * s.m;
-  auto take = [&](size_t cnt) {
-    double* r = p;
-    p += (cnt + 31) & ~(size_t)31;  // 256-byte alignment
-    return r;
-  };
   BMol& M = s.mol;
   M.n = s.n;
   M.m = s.m;
   M.elen = s.elen;
-  M.S = take(nn);
-  M.H = take(nn);
-  M.X = take((size_t)s.n * s.m);
-  M.F = take(nn);
-  M.P = take(nn);
-  M.Pprev = take(nn);
-  M.G = take(nn);
-  M.Pb = take(nn);
-  M.T1 = take(nn);
-  M.T2 = take(nn);
-  M.E = take(nn);
-  M.Fu = take(nn);
-  M.Fo = take(mm);
-  M.Xs = take(mm);
-  M.X2 = take(mm);
-  M.cF = take(BMAXS * nn);
-  M.aF = take(BMAXS * nn);
-  M.aP = take(BMAXS * nn);
-  M.cE = take((size_t)BMAXS * s.elen);
+  M.S = p + off[SM_S];
+  M.H = p + off[SM_H];
+  M.X = p + off[SM_X];
+  M.F = p + off[SM_F];
+  M.P = p + off[SM_P];
+  M.Pprev = p + off[SM_PPREV];
+  M.G = p + off[SM_G];
+  M.Pb = p + off[SM_PB];
+  M.T1 = p + off[SM_T1];
+  M.T2 = p + off[SM_T2];
+  M.E = p + off[SM_E];
+  M.Fu = p + off[SM_FU];
+  M.Fo = p + off[SM_FO];
+  M.Xs = p + off[SM_XS];
+  M.X2 = p + off[SM_X2];
+  M.cF = p + off[SM_CF];
+  M.aF = p + off[SM_AF];
+  M.aP = p + off[SM_AP];
+  M.cE = p + off[SM_CE];
   M.err = s.m < s.n ? M.T2 : M.E;
   M.Dd = s.eng->d_D.as<double>();
 }
@@ -489,6 +496,7 @@ struct Server {
   std::vector<int> h_lists;  // staging of the slot lists (4 x capacity)
   bool dirty = true;         // slot contents changed: the J/K directory must be rebuilt
   std::unique_ptr<Blas> blas;
+  std::vector<Worker> workers;
   size_t budget = 0, resident = 0;
   // statistics
   int64_t fused_builds = 0, iterations = 0;
@@ -496,9 +504,13 @@ struct Server {
   std::chrono::steady_clock::time_point t0;
 
   explicit Server(const hfg_server_options& o);
+  ~Server();
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
   void run();
-  bool admit(int slot, int req, int step, const double* guess);
+  // 0: failed (recorded in the request), 1: ready, 2: ready but the density
+  // must come from the core Hamiltonian (no SAD guess)
+  int admit(int slot, int req, int step, const double* guess, Worker& w);
+  void admit_all(const std::vector<std::array<int, 3>>& todo);
   void finish_step(int slot);
   void density(const std::vector<int>& L);
   void upload_lists(const std::vector<int>& L, int which);
@@ -521,9 +533,25 @@ Server::Server(const hfg_server_options& o) : opt(o) {
   h_want.assign(cap, 0);
   h_lists.assign(4 * (size_t)cap, 0);
   blas.reset(new Blas(fleet->stream));
+  int nw = std::max(1, std::min(8, (int)std::thread::hardware_concurrency() / 2));
+  if (const char* env = std::getenv("HFG_SERVER_WORKERS")) nw = std::max(1, std::atoi(env));
+  nw = std::min(nw, opt.capacity);
+  workers.resize(nw);
+  for (Worker& w : workers) {
+    HIP_CHECK(hipStreamCreate(&w.stream));  // default priority
+    w.blas.reset(new Blas(w.stream));
+  }
   size_t fr = 0, tot = 0;
   HIP_CHECK(hipMemGetInfo(&fr, &tot));
   budget = opt.hbm_budget_gb > 0 ? (size_t)(opt.hbm_budget_gb * 1e9) : (size_t)(0.8 * fr);
+}
+
+Server::~Server() {
+  for (Slot& sl : slots) sl.eng.reset();  // engines were set up on the workers' streams
+  for (Worker& w : workers) {
+    w.blas.reset();
+    if (w.stream) (void)hipStreamDestroy(w.stream);
+  }
 }
 
 void Server::upload_lists(const std::vector<int>& L, int which) {
@@ -543,13 +571,12 @@ static size_t footprint(int n) {
 
 // Set-up of one geometry step in a slot: engine (J/K plan), one-electron
 // matrices, orthogonaliser, guess (SAD, or the given density), as scf.cpp.
-bool Server::admit(int si, int ri, int step, const double* guess) {
+int Server::admit(int si, int ri, int step, const double* guess, Worker& w) {
   Request& R = reqs[ri];
   Slot& s = slots[si];
   const Request::Step& st = R.steps[step];
   const hfg_basis_view bv = st.view();
   const double ts = now();
-  dirty = true;
   try {
     const RhfInputs in = rhf_inputs(bv, st.zxyz.data(), R.natom, R.charge);
     const int n = bv.n_functions;
@@ -558,14 +585,14 @@ bool Server::admit(int si, int ri, int step, const double* guess) {
     auto one_e = std::async(std::launch::async, [&] { one_electron(sh1, n, in.atoms, S, T, V); });
     if (s.eng) fleet->detach(si);
     s.eng.reset();
+    hipStream_t stq = w.stream;
     try {
-      s.eng = fleet->make_engine(bv);
+      s.eng.reset(new Engine(bv, stq));
     } catch (...) {
       one_e.wait();
       throw;
     }
     one_e.get();
-    hipStream_t stq = fleet->stream;
     s.n = n;
     s.nocc = in.nocc;
     s.enuc = in.enuc;
@@ -575,7 +602,7 @@ bool Server::admit(int si, int ri, int step, const double* guess) {
     dS.alloc(nn);
     upload(dS.p, S.data(), nn, stq);
     HMat X;
-    s.m = orthogonalizer(*blas, stq, s.eng->shells, n, dS.p, R.opt.spherical != 0, X);
+    s.m = orthogonalizer(*w.blas, stq, s.eng->shells, n, dS.p, R.opt.spherical != 0, X);
     if (s.nocc > s.m) throw HfgError{HFG_INVALID_INPUT, "more occupied orbitals than retained overlap rank"};
     s.elen = s.m < n ? s.m * s.m : (int)nn;
     const size_t need = slot_doubles(n, s.m);
@@ -590,8 +617,8 @@ bool Server::admit(int si, int ri, int step, const double* guess) {
     upload(s.mol.H, H.data(), nn, stq);
     upload(s.mol.X, X.data(), (size_t)n * s.m, stq);
     // DIIS histories start at zero (scf.cpp clears them the same way)
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)((3 * BMAXS * nn + 255) / 256)), dim3(256), 0, stq, s.mol.cF,
-                       3 * BMAXS * nn);
+    for (double* h : {s.mol.cF, s.mol.aF, s.mol.aP})
+      hipLaunchKernelGGL(k_zero, dim3((unsigned)((BMAXS * nn + 255) / 256)), dim3(256), 0, stq, h, BMAXS * nn);
     hipLaunchKernelGGL(k_zero, dim3((unsigned)(((size_t)BMAXS * s.elen + 255) / 256)), dim3(256), 0, stq, s.mol.cE,
                        (size_t)BMAXS * s.elen);
     bool core_guess = false;
@@ -606,7 +633,7 @@ bool Server::admit(int si, int ri, int step, const double* guess) {
         core_guess = true;  // src/scf.rs:452-471: core Hamiltonian when SAD is unavailable
       }
     }
-    fleet->attach(si, *s.eng);
+    fleet->attach(si, *s.eng, stq);
     s.mol.tab = fleet->d_tabs.as<MolTab>() + si;
     h_mols[si] = s.mol;
     HIP_CHECK(hipMemcpyAsync(d_mols.as<BMol>() + si, &h_mols[si], sizeof(BMol), hipMemcpyHostToDevice, stq));
@@ -623,21 +650,11 @@ bool Server::admit(int si, int ri, int step, const double* guess) {
     std::memset(s.G, 0, sizeof s.G);
     s.converged = false;
     s.footprint = footprint(n);
-    if (core_guess) {
-      // density from H before the first build
-      std::vector<int> L{si};
-      h_ctrl[si] = BCtrl{};
-      h_ctrl[si].src = 3;
-      h_ctrl[si].nocc = s.nocc;
-      HIP_CHECK(hipMemcpyAsync(d_ctrl.as<BCtrl>() + si, &h_ctrl[si], sizeof(BCtrl), hipMemcpyHostToDevice, stq));
-      density(L);
-    }
     HIP_CHECK(hipMemcpyAsync(s.mol.Pprev, s.mol.P, nn * 8, hipMemcpyDeviceToDevice, stq));
     HIP_CHECK(hipStreamSynchronize(stq));
     if (R.admitted < 0) R.admitted = ts;
     R.n = n;
-    seconds_setup += now() - ts;
-    return true;
+    return core_guess ? 2 : 1;
   } catch (HfgError& e) {
     R.status = e.code;
     R.error = e.msg;
@@ -650,8 +667,58 @@ bool Server::admit(int si, int ri, int step, const double* guess) {
   s.eng.reset();
   s.phase = FREE;
   s.req = -1;
+  return 0;
+}
+
+// Sets up the listed (slot, request, step) admissions on the workers in
+// parallel; then, on the fleet's stream, the core-Hamiltonian densities of
+// those without a SAD guess.
+void Server::admit_all(const std::vector<std::array<int, 3>>& todo) {
+  if (todo.empty()) return;
+  const double ts = now();
+  dirty = true;
+  std::vector<int> res(todo.size(), 0);
+  std::atomic<size_t> next{0};
+  auto work = [&](Worker& w) {
+    for (;;) {
+      const size_t i = next++;
+      if (i >= todo.size()) break;
+      const int si = todo[i][0], ri = todo[i][1], step = todo[i][2];
+      Slot& s = slots[si];
+      const bool warm = !s.guess.empty() && s.guess.size() == reqs[ri].steps[step].atom.size() * reqs[ri].steps[step].atom.size();
+      res[i] = admit(si, ri, step, warm ? s.guess.data() : nullptr, w);
+    }
+  };
+  const size_t nt = std::min(workers.size(), todo.size());
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; ++t) th.emplace_back(work, std::ref(workers[t]));
+  work(workers[0]);
+  for (auto& t : th) t.join();
+  std::vector<int> core;
+  for (size_t i = 0; i < todo.size(); ++i) {
+    Slot& s = slots[todo[i][0]];
+    s.guess.clear();
+    s.guess.shrink_to_fit();
+    if (res[i] == 0) resident -= std::min(resident, footprint((int)reqs[todo[i][1]].steps[todo[i][2]].atom.size()));
+    if (res[i] == 2) core.push_back(todo[i][0]);
+  }
+  if (!core.empty()) {
+    // density from H before the first build (src/scf.rs:452-471)
+    for (int si : core) {
+      h_ctrl[si] = BCtrl{};
+      h_ctrl[si].src = 3;
+      h_ctrl[si].nocc = slots[si].nocc;
+    }
+    HIP_CHECK(hipMemcpyAsync(d_ctrl.p, h_ctrl.data(), h_ctrl.size() * sizeof(BCtrl), hipMemcpyHostToDevice,
+                             fleet->stream));
+    density(core);
+    for (int si : core) {
+      const Slot& s = slots[si];
+      HIP_CHECK(hipMemcpyAsync(s.mol.Pprev, s.mol.P, (size_t)s.n * s.n * 8, hipMemcpyDeviceToDevice, fleet->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(fleet->stream));
+  }
   seconds_setup += now() - ts;
-  return false;
 }
 
 // Density of the listed slots from their Fu (k_b_store wrote it): Fo = X^T Fu
@@ -762,13 +829,11 @@ void Server::finish_step(int si) {
   r.n = s.n;
   if (s.step + 1 < (int)R.steps.size()) {
     // warm start of the next geometry (run_scf_with_initial_density,
-    // src/scf.rs:630-639): the final density, same basis functions
-    std::vector<double> p((size_t)s.n * s.n);
-    download(p.data(), s.mol.P, p.size(), fleet->stream);
-    const int ri = s.req, next = s.step + 1;
-    const bool same_n = (int)R.steps[next].atom.size() == s.n;
-    resident -= s.footprint;
-    if (admit(si, ri, next, same_n ? p.data() : nullptr)) resident += s.footprint;
+    // src/scf.rs:630-639): the final density is kept for the next step's
+    // set-up in the coming admission round; the slot stays reserved
+    s.guess.resize((size_t)s.n * s.n);
+    download(s.guess.data(), s.mol.P, s.guess.size(), fleet->stream);
+    s.phase = NEXT;
     return;
   }
   if (R.keep_density) {
@@ -794,7 +859,11 @@ void Server::run() {
   queue.assign(order.begin(), order.end());
   std::vector<int> prev_active;
   for (;;) {
-    // admission: arrived requests into free slots while the footprint fits
+    // admission: the next geometry steps of requests in flight, then arrived
+    // requests into free slots while the footprint fits; all set up at once
+    std::vector<std::array<int, 3>> todo;
+    for (int si = 0; si < opt.capacity; ++si)
+      if (slots[si].phase == NEXT) todo.push_back({si, slots[si].req, slots[si].step + 1});
     const double tnow = now();
     for (int si = 0; si < opt.capacity && !queue.empty(); ++si) {
       if (slots[si].phase != FREE) continue;
@@ -803,8 +872,11 @@ void Server::run() {
       const size_t fp = footprint((int)reqs[ri].steps[0].atom.size());
       if (resident > 0 && resident + fp > budget) break;
       queue.pop_front();
-      if (admit(si, ri, 0, nullptr)) resident += slots[si].footprint;
+      resident += fp;
+      slots[si].phase = NEXT;  // reserved until its set-up
+      todo.push_back({si, ri, 0});
     }
+    admit_all(todo);
     std::vector<int> act;
     for (int si = 0; si < opt.capacity; ++si)
       if (slots[si].phase != FREE) act.push_back(si);
@@ -846,7 +918,9 @@ void Server::run() {
       prev_active = act;
       dirty = false;
     }
-    fleet->build();
+    bool all_full = true;
+    for (int si : act) all_full = all_full && h_ctrl[si].build == 0;
+    fleet->build(all_full);
     ++fused_builds;
     hipLaunchKernelGGL(k_b_fock, ge, dim3(256), 0, stq, L0, d_mols.as<BMol>(), d_ctrl.as<BCtrl>());
     hipLaunchKernelGGL(k_b_energy, ge, dim3(256), 0, stq, L0, d_mols.as<BMol>(), d_stat.as<BStat>());
@@ -1139,5 +1213,7 @@ int hfg_server_stats(const hfg_server* srv, hfg_server_stats_t* st) {
   st->seconds_setup = S.seconds_setup;
   st->seconds_sync = S.seconds_sync;
   st->hbm_budget_bytes = (double)S.budget;
+  st->jk_config = S.fleet->tuned ? S.fleet->current : -1;
+  st->jk_candidates = (int32_t)S.fleet->candidates.size();
   return HFG_OK;
 }

@@ -44,7 +44,6 @@ static int nc_of(const Engine& e, int pair) {
 
 namespace hfg {
 void eri_mo(Engine& e, const double* C, int nmo, double* out);
-void jk_device_noalloc(Engine& e, const double* dP, double* dJ, double* dK);
 int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
             hfg_scf_result* res, double* p_out, double* f_out, double* eps_out, const double* p_guess);
 }

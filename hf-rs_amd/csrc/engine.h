@@ -81,6 +81,8 @@ void launch_jk_class(int cls, const JKArgs& a, int max_blocks, hipStream_t s);
 // or dd bra, (dp|ss) .. (dd|dd), unless HFG_SIDE_CLASSES lists class indices
 // (comma-separated; measured on (H2O)16/cc-pVDZ).
 bool side_class(int cls);
+// The classes side_class() names, as a bit mask.
+unsigned default_side_mask();
 
 // Arguments of the block kernels: one thread per task computes the unscaled
 // block of pair `bra` with pair `ket` (kernel orientation) at out + off.
@@ -134,6 +136,8 @@ struct JKLaunch {
   double screen0;
   bool use_rows;
   hipEvent_t* events;
+  unsigned side_mask;  // classes on the side stream (bit c)
+  int max_blocks;      // workgroup cap of a main-stream class; 0: the persistent default
 };
 // The J/K class launches of one build over directory dir (device copy d_dir)
 // of the tables at d_tabs; accumulators must be cleared beforehand.  Classes

@@ -132,7 +132,7 @@ JKDirectory make_directory(const std::vector<int>& mol, const std::vector<const 
   return d;
 }
 
-bool side_class(int cls) {
+unsigned default_side_mask() {
   static const unsigned mask = [] {
     unsigned m = 0;
     for (int c = 9; c < HFG_NCLASS; ++c) m |= 1u << c;  // (ds|ds), (dp|ss) .. (dd|dd)
@@ -148,8 +148,10 @@ bool side_class(int cls) {
     }
     return m;
   }();
-  return cls >= 0 && cls < HFG_NCLASS && ((mask >> cls) & 1u);
+  return mask;
 }
+
+bool side_class(int cls) { return cls >= 0 && cls < HFG_NCLASS && ((default_side_mask() >> cls) & 1u); }
 // ---------------------------------------------------------------------------
 typedef void (*BlockLaunch)(const BlockArgs&, hipStream_t);
 
@@ -499,7 +501,7 @@ void launch_jk_set(const JKLaunch& L, const MolTab* d_tabs, const JKDirectory& d
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, main, L.counters, HFG_NCLASS);
   // The high-L classes, few and long quartets, run on the side stream with a
   // capped grid and fill the room the low-L classes leave.
-  auto on_side = [&](int c) { return side_cap > 0 && side_stream && side_class(c); };
+  auto on_side = [&](int c) { return side_cap > 0 && side_stream && ((L.side_mask >> c) & 1u); };
   const bool two = side_cap > 0 && side_stream;
   if (two) {
     HIP_CHECK(hipEventRecord(L.fork, main));
@@ -525,7 +527,7 @@ void launch_jk_set(const JKLaunch& L, const MolTab* d_tabs, const JKDirectory& d
       a.ntask = dr.ntask[c];
       a.job_counter = L.counters + c;
       if (L.events) HIP_CHECK(hipEventRecord(L.events[2 * c], s));
-      launch_jk_class(c, a, on_side(c) ? side_cap : g_max_blocks, s);
+      launch_jk_class(c, a, on_side(c) ? side_cap : (L.max_blocks > 0 ? L.max_blocks : g_max_blocks), s);
       HIP_CHECK(hipGetLastError());
       if (L.events) HIP_CHECK(hipEventRecord(L.events[2 * c + 1], s));
     }
@@ -537,7 +539,7 @@ void launch_jk_set(const JKLaunch& L, const MolTab* d_tabs, const JKDirectory& d
 
 JKLaunch Engine::launch_ctx() {
   return JKLaunch{d_counters.as<int>(), ev_fork, ev_join, d_boys.as<double>(), screen, use_rows,
-                  profile ? events.data() : nullptr};
+                  profile ? events.data() : nullptr, default_side_mask(), 0};
 }
 
 void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* dH, double* dF, double dmax) {

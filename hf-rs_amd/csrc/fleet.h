@@ -20,6 +20,15 @@
 
 namespace hfg {
 
+// One way to run a build's class launches: which classes go to the side
+// stream (bit mask), that stream's workgroup cap (0: one stream) and the
+// main-stream classes' workgroups per CU.
+struct JKConfig {
+  unsigned side_mask;
+  int side_cap;
+  int blocks_per_cu;
+};
+
 struct Fleet {
   hipStream_t stream = nullptr, side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -39,6 +48,22 @@ struct Fleet {
   std::vector<double> class_ms;
   std::vector<int64_t> class_launches;
   void set_profile(bool on);
+  // Autotuner: the first full builds of the fleet each run one candidate
+  // configuration, timed by events on the stream; the score is milliseconds
+  // per surviving shell quartet of the build's molecules, and the best
+  // candidate runs every later build.  (The results do not depend on the
+  // configuration, only the time does.)  HFG_AUTOTUNE=0 keeps the default.
+  std::vector<JKConfig> candidates;
+  std::vector<double> score;       // per candidate (< 0: not measured)
+  int current = 0;                 // candidate of the next build
+  int pending = -1;                // candidate whose timing is in flight
+  double pending_work = 0;
+  bool tuned = false;
+  hipEvent_t tb = nullptr, te = nullptr;
+  std::vector<double> slot_quartets;  // per slot (tuner's work measure)
+  int cus = 256;
+  void tune_step(bool full_build);
+  const JKConfig& config() const { return candidates[current]; }
 
   explicit Fleet(int capacity);
   ~Fleet();
@@ -46,14 +71,18 @@ struct Fleet {
   Fleet& operator=(const Fleet&) = delete;
   // An engine created on this fleet's stream (no streams of its own).
   std::unique_ptr<Engine> make_engine(const hfg_basis_view& b);
-  // Puts engine e's table into slot s (its density buffer d_D).
-  void attach(int s, const Engine& e);
+  // Puts engine e's table into slot s (its density buffer d_D), copied on
+  // stream st (the fleet's when null) before returning.  Slots set up by
+  // different threads at once must differ.
+  void attach(int s, const Engine& e, hipStream_t st = nullptr);
   void detach(int s);
   // The slots of the next builds; uploads their directory.
   void set_active(const std::vector<int>& slots);
   // Clears the accumulators of the active slots and runs the class launches;
   // the caller reduces the per-XCC copies (k_fleet_jk_out or its own kernel).
-  void build();
+  // full_build: every active slot builds the whole quartet set (the tuner
+  // only times such builds).
+  void build(bool full_build = true);
   // build() followed by J and K of every active slot into jout[i], kout[i]
   // (device arrays of device pointers, one per active slot).
   void build_jk(double* const* d_jout, double* const* d_kout);

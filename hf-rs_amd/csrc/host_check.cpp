@@ -194,3 +194,9 @@ extern "C" int hfgc_solve_small(const double* a, const double* b, int n, double*
   std::copy(xx.begin(), xx.end(), x);
   return 1;
 }
+
+#include "bscf_layout.h"
+
+// Offsets of a server slot's matrices (bscf_layout.h) and the buffer size,
+// for the CPU test that checks the carving stays inside the buffer.
+extern "C" size_t hfgc_slot_layout(size_t n, size_t m, size_t* off) { return hfg::slot_layout(n, m, off); }

@@ -312,3 +312,57 @@ extern "C" int hfgc_jk_plan_work2(const hfg_basis_view* b, const int* caps, doub
     return e.code;
   }
 }
+
+// Tasks, jobs and active lanes (ket views summed over tasks) of the plan per
+// class (out[3c], out[3c + 1], out[3c + 2]): the work granularity of the J/K
+// kernels (a task is one wave of 64 lanes; a job, one workgroup's unit).
+extern "C" int hfgc_jk_plan_jobs(const hfg_basis_view* b, double* out) {
+  try {
+    std::vector<Shell> sh = group_shells(*b);
+    const int n = b->n_functions;
+    std::vector<PairInfo> pairs;
+    PrimPairHost pp;
+    build_pairs(sh, pairs, pp);
+    std::vector<double> scale(n, 1.0);
+    for (auto& s : sh)
+      for (size_t c = 0; c < s.scale.size(); ++c) scale[s.ao + c] = s.scale[c];
+    const PrimPairs pv = view_of(pp);
+    std::vector<double> Q(pairs.size()), blk(1296);
+    for (size_t i = 0; i < pairs.size(); ++i) {
+      const PairInfo& p = pairs[i];
+      const int pc = pair_class(p.la, p.lb);
+      BLOCK_HOST[quartet_class(pc, pc)](pv, p.p0, p.p1, pv, p.p0, p.p1, p.AB, p.AB, blk.data());
+      const int na = ncart(p.la), nb = ncart(p.lb), nab = na * nb;
+      double m = 0.0;
+      for (int x = 0; x < na; ++x)
+        for (int y = 0; y < nb; ++y) {
+          const int ij = x * nb + y;
+          const double f = scale[p.ao_a + x] * scale[p.ao_b + y];
+          m = std::max(m, blk[(size_t)ij * nab + ij] * f * f);
+        }
+      Q[p.hf_index] = std::sqrt(m);
+    }
+    BlockFn host_blocks = [](const PrimPairHost& prims, const std::vector<PairDev>& tab,
+                             const std::vector<BlockTask>& tasks, const std::vector<int>& cls,
+                             std::vector<double>& o) {
+      const PrimPairs v = view_of(prims);
+      for (size_t t = 0; t < tasks.size(); ++t) {
+        const PairDev& pb = tab[tasks[t].bra];
+        const PairDev& pk = tab[tasks[t].ket];
+        BLOCK_HOST[cls[t]](v, pb.p0, pb.p1, v, pk.p0, pk.p1, pb.AB, pk.AB, &o[tasks[t].off]);
+      }
+    };
+    JKPlan plan = make_jk_plan(sh, Q, scale, 1e-12, 1e-20, host_blocks);
+    for (int c = 0; c < HFG_NCLASS; ++c) {
+      out[3 * c] = (double)(plan.task_begin[c + 1] - plan.task_begin[c]);
+      out[3 * c + 1] = (double)(plan.job_begin[c + 1] - plan.job_begin[c]);
+      double lanes = 0;
+      for (size_t t = plan.task_begin[c]; t < plan.task_begin[c + 1]; ++t) lanes += plan.tasks[t].nket;
+      out[3 * c + 2] = lanes;
+    }
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}

@@ -266,6 +266,9 @@ typedef struct {
 typedef struct {
   int64_t fused_builds, iterations;
   double seconds, seconds_setup, seconds_sync, hbm_budget_bytes;
+  /* the class-launch configuration the autotuner chose (-1: still tuning)
+   * out of jk_candidates */
+  int32_t jk_config, jk_candidates;
 } hfg_server_stats_t;
 int hfg_server_create(const hfg_server_options* opt, hfg_server** out);
 void hfg_server_destroy(hfg_server* server);
