@@ -44,6 +44,7 @@ static int nc_of(const Engine& e, int pair) {
 
 namespace hfg {
 void eri_mo(Engine& e, const double* C, int nmo, double* out);
+void jk_device_noalloc(Engine& e, const double* dP, double* dJ, double* dK);
 int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
             hfg_scf_result* res, double* p_out, double* f_out, double* eps_out, const double* p_guess);
 }
@@ -218,6 +219,15 @@ int hfg_contract_jk_device(hfg_ctx* ctx, const double* d_p, double* d_j, double*
   if (!ctx || !d_p) return set_err(HFG_INVALID_INPUT, "null argument");
   GUARD({
     ctx->e->jk_device(d_p, d_j, d_k);
+    ctx->e->sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_contract_jk_device_noalloc(hfg_ctx* ctx, const double* d_p, double* d_j, double* d_k) {
+  if (!ctx || !d_p) return set_err(HFG_INVALID_INPUT, "null argument");
+  GUARD({
+    jk_device_noalloc(*ctx->e, d_p, d_j, d_k);
     ctx->e->sync();
     return HFG_OK;
   })

@@ -93,6 +93,16 @@ int hfg_build_fock(hfg_ctx* ctx, const double* h, const double* p, double* f);
  * context's stream; returns after the results are complete. */
 int hfg_contract_jk_device(hfg_ctx* ctx, const double* d_p, double* d_j, double* d_k);
 
+/* The same as hfg_contract_jk_device with no device allocation in the call:
+ * the build's scratch memory (the molecule's table, the job counters and the
+ * per-XCC J/K accumulators) lives in one file-scope device array of the
+ * library, about 16.8 MB per GPU, kept until the process ends.  For callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc); no counterpart in the reference.  Limit:
+ * n <= 256 basis functions (HFG_INVALID_INPUT beyond).  Every call on one GPU
+ * shares the array: calls that overlap on one GPU race. */
+int hfg_contract_jk_device_noalloc(hfg_ctx* ctx, const double* d_p, double* d_j, double* d_k);
+
 /* Resident-buffer form used by throughput measurements: the density is
  * uploaded once, J and K stay in HBM until fetched.  (Same arithmetic as
  * hfg_contract_jk; no reference counterpart: hf-rs has no device memory.) */
