@@ -10,8 +10,9 @@ shell-quartet class, the DIIS and purification linear algebra batched),
 against the plain single-molecule GPU driver (hfg_scf_rhf) on a sample of
 the same molecules, with the energies compared.  A step (--steps) is one
 fused Fock build of the whole 512-molecule batch (J and K of every
-molecule's resident density), timed with its per-class launches for the
-roofline of the dominant kernel.
+molecule's resident density); as many builds again with every class on one
+stream give each class kernel's own time, for the roofline of the dominant
+kernel.
 
 Also reported: the fused against the molecule-by-molecule Fock builds of the
 batch; the served stream (Poisson arrivals of geometry trajectories, p50/p99
@@ -142,12 +143,13 @@ PROFILE = os.path.join(ROOT, "profiles", "fused_jk_config3_def2svp.json")
 def fused_report(hf, M, count, steps, warmup, unbatched=True):
     """The hot path of the batched workload: one step = J and K of all
     `count` molecules' resident densities in fused launches (one per
-    shell-quartet class over the whole batch).  Timed over `steps` builds with
-    per-class HIP events on the stream each class runs on; next to it the same
-    builds molecule after molecule (each molecule's own launches).  The
-    roofline is that of the class kernel with the largest measured time per
-    launch: its executed FP64 work per launch (rocprofv3 SQ instruction
-    counters, profiles/) over that time, against the FP64 peak."""
+    shell-quartet class over the whole batch).  Timed over `steps` builds;
+    then `steps` builds with every class on one stream and per-class HIP
+    events give each class kernel's own time; next to it the same builds
+    molecule after molecule (each molecule's own launches).  The roofline is
+    that of the class kernel with the largest time per launch: its executed
+    FP64 work per launch (rocprofv3 SQ instruction counters, profiles/) over
+    that time, against the FP64 peak."""
     mols = M.batch_config3(count, seed=3)
     text = M.basis_text("def2-svp")
     bases = [hf.Basis.from_g94(text, m.symbols, m.xyz) for m in mols]
@@ -163,13 +165,18 @@ def fused_report(hf, M, count, steps, warmup, unbatched=True):
     for _ in range(warmup):
         batch.jk()
     lat = []
-    batch.profile(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         s0 = time.perf_counter()
         batch.jk()  # returns after the build is complete on the device
         lat.append(time.perf_counter() - s0)
     elapsed = time.perf_counter() - t0
+    # Per-class times from as many builds again with every class on one
+    # stream, so that each class kernel's time is its own (in the timed builds
+    # the high-L classes overlap the others on a second stream).
+    batch.profile(True, exclusive=True)
+    for _ in range(steps):
+        batch.jk()
     batch.profile(False)
     stats = batch.class_stats()
     each = float("nan")
@@ -182,7 +189,12 @@ def fused_report(hf, M, count, steps, warmup, unbatched=True):
     fused = elapsed / steps
     dom = max(stats, key=lambda x: x["ms"] / max(1, x["launches"]))
     dom_ms = dom["ms"] / max(1, dom["launches"])
-    kname = "k_jk_rows<%d,%d,%d,%d>" % tuple(dom["l"])
+    prof = json.load(open(PROFILE))["per_kernel"] if os.path.exists(PROFILE) else {}
+    # The class's kernel is k_jk_rows<la,lb,lc,ld,variant>: the profile entry of
+    # that class with the longest dispatches (a batch of one basis runs one variant).
+    prefix = "k_jk_rows<%d,%d,%d,%d," % tuple(dom["l"])
+    cand = [k for k in prof if k.startswith(prefix)]
+    kname = max(cand, key=lambda k: prof[k].get("avg_us", 0)) if cand else prefix + "?>"
     roof = {"bound": "fp64-valu", "kernel": kname, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "ms_per_launch": dom_ms, "achieved": None, "frac": None, "traffic": None,
             "md_equivalent": {"tflops": dom["flops"] / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else None,
@@ -190,8 +202,8 @@ def fused_report(hf, M, count, steps, warmup, unbatched=True):
                               "model": "McMurchie-Davidson loop count per primitive quartet of every surviving "
                                        "shell quartet (the reference's work; general contraction and HGP do less, "
                                        "so this rate can exceed the peak)"}}
-    if os.path.exists(PROFILE):
-        t = json.load(open(PROFILE))["per_kernel"].get(kname)
+    if prof:
+        t = prof.get(kname)
         if t and dom_ms > 0:
             ef = t["executed"]["f64_flops"]
             ach = ef / (dom_ms * 1e-3) / 1e12
@@ -396,9 +408,10 @@ def serve_report(hf, M, n_req=16, lam=20.0, steps=10, capacity=16):
             "gradient": "not computed (energy only)"}
 
 
-def summarize_profiles(trace_csv, pmc_dirs, copy_dirs, out_json, workload, commands):
+def summarize_profiles(trace_csv, pmc_dirs, copy_dirs, out_json, workload, commands, last=3):
     """Turns rocprofv3 CSV output into the per-kernel summary bench.py reads:
-    average duration (kernel trace), HBM bytes per dispatch (FETCH_SIZE and
+    average duration of each kernel's last `last` dispatches (kernel trace:
+    the timed builds, after the warm-up and tuning ones), HBM bytes per dispatch (FETCH_SIZE and
     WRITE_SIZE in KiB, scaled by the same counters on a copy kernel of known
     size) and executed FP64 FLOPs per dispatch (SQ instruction counters x 64
     lanes, FMA = 2 FLOPs)."""
@@ -415,8 +428,14 @@ def summarize_profiles(trace_csv, pmc_dirs, copy_dirs, out_json, workload, comma
         k = name.replace(" ", "").split("(")[0].replace("hfg::", "")
         return k[4:] if k.startswith("void") else k
     per = {}
-    for r in rows(trace_csv, "*kernel_stats.csv"):
-        per.setdefault(short(r["Name"]), {})["avg_us"] = float(r["AverageNs"]) / 1e3
+    disp = {}
+    for r in rows(trace_csv, "*kernel_trace.csv"):
+        disp.setdefault(short(r["Kernel_Name"]), []).append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    for k, v in disp.items():
+        v.sort()
+        d = [(e - s0) / 1e3 for s0, e in v[-last:]]
+        per.setdefault(k, {})["avg_us"] = sum(d) / len(d)
+        per[k]["dispatches"] = len(v)
     cal = {}
     for ctr, d in copy_dirs.items():
         v = [float(r["Counter_Value"]) for r in rows(d, "*counter_collection.csv")
@@ -468,13 +487,14 @@ def main():
                     help="write the per-kernel profile summary from rocprofv3 CSV directories "
                          "(PMC_DIRS comma-separated; COPY_DIRS as FETCH_SIZE=dir,WRITE_SIZE=dir) and exit")
     ap.add_argument("--profile-out", default=PROFILE)
+    ap.add_argument("--profile-last", type=int, default=3, help="timed dispatches per kernel in the trace")
     args = ap.parse_args()
     if args.summarize_profiles:
         tdir, pdirs, cdirs = args.summarize_profiles
         copy = dict(x.split("=", 1) for x in cdirs.split(",") if x)
         summarize_profiles(tdir, pdirs.split(","), copy, args.profile_out,
                            "fused J/K of the %d-molecule def2-SVP batch (seed 3), per launch" % args.batch_count,
-                           {"trace": tdir, "pmc": pdirs, "copy": cdirs})
+                           {"trace": tdir, "pmc": pdirs, "copy": cdirs}, last=args.profile_last)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine's workloads are single-GPU (DESIGN.md describes how the path "

@@ -289,8 +289,10 @@ class Batch:
         """The same builds molecule after molecule (unbatched launches)."""
         _check(lib().hfg_batch_jk_each(self.h))
 
-    def profile(self, on=True):
-        _check(lib().hfg_batch_profile(self.h, int(on)))
+    def profile(self, on=True, exclusive=False):
+        """Per-class timing of the following builds; exclusive: every class on
+        one stream, so each time is the class's alone."""
+        _check(lib().hfg_batch_profile(self.h, (2 if exclusive else 1) if on else 0))
 
     def class_stats(self):
         out = []

@@ -60,7 +60,7 @@ Fleet::Fleet(int cap) : capacity(cap) {
   occupied.assign(cap, 0);
   d_tabs.alloc(cap * sizeof(MolTab));
   d_active.alloc(cap * sizeof(int));
-  d_counters.alloc(HFG_NCLASS * sizeof(int));
+  d_counters.alloc(HFG_NSLOT * sizeof(int));
   auto tab = boys_table();
   d_boys.alloc(tab.size() * sizeof(double));
   HIP_CHECK(hipMemcpyAsync(d_boys.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -166,12 +166,12 @@ void Fleet::build(bool full_build) {
   JKLaunch L{d_counters.as<int>(), ev_fork, ev_join, d_boys.as<double>(), screen0, use_rows,
              profile ? events.data() : nullptr, cfg.side_mask, cfg.blocks_per_cu * cus};
   if (pending >= 0) HIP_CHECK(hipEventRecord(tb, stream));
-  launch_jk_set(L, d_tabs.as<MolTab>(), dir, d_dir.as<int>(), stream, side, cfg.side_cap);
+  launch_jk_set(L, d_tabs.as<MolTab>(), dir, d_dir.as<int>(), stream, side, profile && exclusive ? 0 : cfg.side_cap);
   if (pending >= 0) HIP_CHECK(hipEventRecord(te, stream));
   if (profile) {
     sync();
     for (int c = 0; c < HFG_NCLASS; ++c) {
-      if (!dir.count[c]) continue;
+      if (!dir.count[c] && !dir.count[c + HFG_NCLASS]) continue;
       float ms = 0;
       HIP_CHECK(hipEventElapsedTime(&ms, events[2 * c], events[2 * c + 1]));
       class_ms[c] += ms;
@@ -180,7 +180,8 @@ void Fleet::build(bool full_build) {
   }
 }
 
-void Fleet::set_profile(bool on) {
+void Fleet::set_profile(bool on, bool exclusive_) {
+  exclusive = on && exclusive_;
   if (on && events.empty()) {
     events.resize(2 * HFG_NCLASS);
     for (auto& e : events) HIP_CHECK(hipEventCreate(&e));
@@ -293,7 +294,7 @@ int hfg_batch_info(const hfg_batch* b, int64_t* count, int64_t* n_max, int64_t* 
 int hfg_batch_profile(hfg_batch* b, int enable) {
   if (!b) return set_error(HFG_INVALID_INPUT, "null batch");
   BGUARD({
-    b->b->fleet->set_profile(enable != 0);
+    b->b->fleet->set_profile(enable != 0, enable == 2);
     return HFG_OK;
   })
 }

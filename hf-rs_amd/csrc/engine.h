@@ -40,7 +40,7 @@ struct MolTab {
   size_t nn;            // n*n, the stride between per-XCC accumulator copies
   double screen;        // Schwarz threshold of a combination pair in this build
   int n;
-  int pad;
+  unsigned unit_mask;   // classes planned with one member per shell (kernel variant 1), bit c
   int task_begin[HFG_NCLASS + 1];
   int job_begin[HFG_NCLASS + 1];
 };
@@ -61,22 +61,24 @@ struct JKArgs {
   int* job_counter;    // zero at launch (k_jk_rows)
   int use_rows;        // 0: the global-memory kernel k_jk
 };
-// Per-class directory arrays of one launch set, in one int buffer: class c's
-// entries start at off[c] (dir_mol, then dir_job and dir_task with one extra
-// entry each).
+// Per-slot directory arrays of one launch set, in one int buffer: slot
+// c + v * HFG_NCLASS holds the molecules that run class c in kernel variant v
+// (MolTab::unit_mask); its entries start at off[slot] (dir_mol, then dir_job
+// and dir_task with one extra entry each).
 struct JKDirectory {
   std::vector<int> host;
-  std::vector<int> off;     // HFG_NCLASS entries into host
-  std::vector<int> count;   // entries per class
+  std::vector<int> off;     // HFG_NSLOT entries into host
+  std::vector<int> count;   // entries per slot
   std::vector<int> njob, ntask;
   int nmax = 0;
 };
-// Directory of the molecules (tab indices) for every class, skipping entries
+// Directory of the molecules (tab indices) for every slot, skipping entries
 // without work in a class; tabs_host holds the molecules' tables.
 JKDirectory make_directory(const std::vector<int>& mol, const std::vector<const MolTab*>& tabs_host);
-// Launch of class cls's J/K kernel with at most max_blocks workgroups; the
-// kernels are split over five translation units.
-void launch_jk_class(int cls, const JKArgs& a, int max_blocks, hipStream_t s);
+// Launch of class cls's J/K kernel in variant v (1: one contraction member per
+// shell) with at most max_blocks workgroups; the kernels are split over five
+// translation units.
+void launch_jk_class(int cls, int v, const JKArgs& a, int max_blocks, hipStream_t s);
 // Whether class cls runs on the side stream: (ds|ds) and the classes with a dp
 // or dd bra, (dp|ss) .. (dd|dd), unless HFG_SIDE_CLASSES lists class indices
 // (comma-separated; measured on (H2O)16/cc-pVDZ).
@@ -126,7 +128,7 @@ double md_flops(int la, int lb, int lc, int ld);
 int set_error(int code, const std::string& msg);
 
 // What one set of class launches needs besides the directory: job counters
-// (HFG_NCLASS ints), fork/join events of the side stream, the Boys table,
+// (HFG_NSLOT ints), fork/join events of the side stream, the Boys table,
 // the threshold the task lists were built with, the LDS-row choice and
 // optional per-class event pairs.
 struct JKLaunch {

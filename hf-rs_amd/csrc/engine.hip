@@ -105,25 +105,26 @@ __global__ void k_tab_set(MolTab* tab, const double* D, double screen) {
 
 JKDirectory make_directory(const std::vector<int>& mol, const std::vector<const MolTab*>& tabs_host) {
   JKDirectory d;
-  d.off.assign(HFG_NCLASS, 0);
-  d.count.assign(HFG_NCLASS, 0);
-  d.njob.assign(HFG_NCLASS, 0);
-  d.ntask.assign(HFG_NCLASS, 0);
+  d.off.assign(HFG_NSLOT, 0);
+  d.count.assign(HFG_NSLOT, 0);
+  d.njob.assign(HFG_NSLOT, 0);
+  d.ntask.assign(HFG_NSLOT, 0);
   for (size_t i = 0; i < mol.size(); ++i) d.nmax = std::max(d.nmax, tabs_host[i]->n);
-  for (int c = 0; c < HFG_NCLASS; ++c) {
+  for (int slot = 0; slot < HFG_NSLOT; ++slot) {
+    const int c = slot % HFG_NCLASS, v = slot / HFG_NCLASS;
     std::vector<int> ids, jp{0}, tp{0};
     for (size_t i = 0; i < mol.size(); ++i) {
       const MolTab& t = *tabs_host[i];
       const int nj = t.job_begin[c + 1] - t.job_begin[c], nt = t.task_begin[c + 1] - t.task_begin[c];
-      if (!nt) continue;
+      if (!nt || (int)((t.unit_mask >> c) & 1u) != v) continue;
       ids.push_back(mol[i]);
       jp.push_back(jp.back() + nj);
       tp.push_back(tp.back() + nt);
     }
-    d.off[c] = (int)d.host.size();
-    d.count[c] = (int)ids.size();
-    d.njob[c] = jp.back();
-    d.ntask[c] = tp.back();
+    d.off[slot] = (int)d.host.size();
+    d.count[slot] = (int)ids.size();
+    d.njob[slot] = jp.back();
+    d.ntask[slot] = tp.back();
     d.host.insert(d.host.end(), ids.begin(), ids.end());
     d.host.insert(d.host.end(), jp.begin(), jp.end());
     d.host.insert(d.host.end(), tp.begin(), tp.end());
@@ -409,7 +410,7 @@ void Engine::build_tasks() {
   up_load(d_combos, plan.combos.data(), plan.combos.size() * sizeof(ComboDev));
   up_load(d_tasks, plan.tasks.data(), plan.tasks.size() * sizeof(JKTask));
   up_load(d_jobs, plan.jobs.data(), plan.jobs.size() * sizeof(int));
-  d_counters.alloc(HFG_NCLASS * sizeof(int));
+  d_counters.alloc(HFG_NSLOT * sizeof(int));
   job_begin = plan.job_begin;
   if (const char* env = std::getenv("HFG_JK_ROWS")) use_rows = std::atoi(env) != 0;
   vb_size = plan.vb_size;
@@ -436,6 +437,7 @@ void Engine::build_tasks() {
   tab_host.nn = (size_t)n * n;
   tab_host.screen = screen;
   tab_host.n = n;
+  tab_host.unit_mask = plan.unit_mask;
   for (int c = 0; c <= HFG_NCLASS; ++c) {
     tab_host.task_begin[c] = (int)task_begin[c];
     tab_host.job_begin[c] = (int)job_begin[c];
@@ -498,7 +500,7 @@ void Engine::count_work() {
 
 void launch_jk_set(const JKLaunch& L, const MolTab* d_tabs, const JKDirectory& dr, const int* d_dr, hipStream_t main,
                    hipStream_t side_stream, int side_cap) {
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, main, L.counters, HFG_NCLASS);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, main, L.counters, HFG_NSLOT);
   // The high-L classes, few and long quartets, run on the side stream with a
   // capped grid and fill the room the low-L classes leave.
   auto on_side = [&](int c) { return side_cap > 0 && side_stream && ((L.side_mask >> c) & 1u); };
@@ -514,21 +516,27 @@ void launch_jk_set(const JKLaunch& L, const MolTab* d_tabs, const JKDirectory& d
   a.screen0 = L.screen0;
   a.boys = L.boys;
   a.use_rows = L.use_rows ? 1 : 0;
+  // A class's two variants run back to back on one stream; its event pair
+  // brackets both.
   for (int pass = 0; pass < 2; ++pass)
     for (int c = 0; c < HFG_NCLASS; ++c) {
-      if (!dr.count[c] || on_side(c) != (pass == 0)) continue;
+      if ((!dr.count[c] && !dr.count[c + HFG_NCLASS]) || on_side(c) != (pass == 0)) continue;
       hipStream_t s = on_side(c) ? side_stream : main;
-      const int* base = d_dr + dr.off[c];
-      a.ndir = dr.count[c];
-      a.dir_mol = base;
-      a.dir_job = base + dr.count[c];
-      a.dir_task = base + 2 * dr.count[c] + 1;
-      a.njob = dr.njob[c];
-      a.ntask = dr.ntask[c];
-      a.job_counter = L.counters + c;
       if (L.events) HIP_CHECK(hipEventRecord(L.events[2 * c], s));
-      launch_jk_class(c, a, on_side(c) ? side_cap : (L.max_blocks > 0 ? L.max_blocks : g_max_blocks), s);
-      HIP_CHECK(hipGetLastError());
+      for (int v = 0; v < 2; ++v) {
+        const int slot = c + v * HFG_NCLASS;
+        if (!dr.count[slot]) continue;
+        const int* base = d_dr + dr.off[slot];
+        a.ndir = dr.count[slot];
+        a.dir_mol = base;
+        a.dir_job = base + dr.count[slot];
+        a.dir_task = base + 2 * dr.count[slot] + 1;
+        a.njob = dr.njob[slot];
+        a.ntask = dr.ntask[slot];
+        a.job_counter = L.counters + slot;
+        launch_jk_class(c, v, a, on_side(c) ? side_cap : (L.max_blocks > 0 ? L.max_blocks : g_max_blocks), s);
+        HIP_CHECK(hipGetLastError());
+      }
       if (L.events) HIP_CHECK(hipEventRecord(L.events[2 * c + 1], s));
     }
   if (two) {

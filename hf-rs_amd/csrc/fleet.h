@@ -42,12 +42,14 @@ struct Fleet {
   double screen0 = 1e-12;
   bool use_rows = true;
   // per-class launch times (HIP events on the stream each class runs on;
-  // the side-stream classes' times include their overlap with the others)
-  bool profile = false;
+  // the side-stream classes' times include their overlap with the others,
+  // unless exclusive: then the profiled builds run every class on the main
+  // stream, one after the other, and each time is the class's alone)
+  bool profile = false, exclusive = false;
   std::vector<hipEvent_t> events;
   std::vector<double> class_ms;
   std::vector<int64_t> class_launches;
-  void set_profile(bool on);
+  void set_profile(bool on, bool exclusive_ = false);
   // Autotuner: the first full builds of the fleet each run one candidate
   // configuration, timed by events on the stream; the score is milliseconds
   // per surviving shell quartet of the build's molecules, and the best

@@ -290,8 +290,15 @@ GC_CAPS = {
 }
 
 
-def gc_caps(c):
-    return GC_CAPS.get(tuple(c), (1, 1, 1, 1))
+def gc_caps(c, V=0):
+    """Caps of class c in kernel variant V: 0 the caps above, 1 one member per
+    shell (segmented bases, where no shells share exponents: the variant-0
+    kernels would carry every combination's arithmetic for one nonzero)."""
+    return (1, 1, 1, 1) if V else GC_CAPS.get(tuple(c), (1, 1, 1, 1))
+
+
+def has_unit_variant(c):
+    return gc_caps(c) != (1, 1, 1, 1)
 
 
 # Output slices of the high-L classes.  A slice is the part of a contracted
@@ -358,8 +365,8 @@ BRA_COEF_SS = 5
 BRA_COEF = 9
 
 
-def gen_class_gc(la, lb, lc, ld, S=0):
-    """eri_gc<la,lb,lc,ld,S>: slice S of the blocks of CA x CC contraction combinations of one
+def gen_class_gc(la, lb, lc, ld, S=0, V=0):
+    """eri_gc<la,lb,lc,ld,S,V>: slice S of the blocks of CA x CC contraction combinations of one
     bra pair and one ket pair that share their primitive pairs.
 
     Bra primitives are records of BRA_REC doubles at br (wave-uniform, read
@@ -372,7 +379,7 @@ def gen_class_gc(la, lb, lc, ld, S=0):
     primitive the bra sums t[gab] = sum_b c_b[gab] [e0|f0]_b are formed, then
     a[gab][gcd] += c_k[gcd] t[gab].  Output: I[(gab * CC + gcd) * NAB * NCD + ...],
     only the entries of slice S (slice_keep) written."""
-    RA, RB, RC, RD = gc_caps((la, lb, lc, ld))
+    RA, RB, RC, RD = gc_caps((la, lb, lc, ld), V)
     CA, CC = RA * RB, RC * RD
     coff = BRA_COEF_SS if la + lb == 0 else BRA_COEF
     assert CA == 1 or coff + CA <= BRA_REC
@@ -386,9 +393,9 @@ def gen_class_gc(la, lb, lc, ld, S=0):
     hrr_lines(la, lb, lc, ld, {(e, f): "x" for e in E for f in Fk}, "", 0, keep, used)
     E_f = [(e, f) for e in E for f in Fk if (e, f) in used]  # accumulators the slice reaches
     out = []
-    out.append("// (%s%s|%s%s) x %d x %d combinations, slice %d of %d: %d accumulators per combination" % (
-        "spdfg"[la], "spdfg"[lb], "spdfg"[lc], "spdfg"[ld], CA, CC, S, slice_count((la, lb, lc, ld)), len(E_f)))
-    out.append("template <> HFG_DEV void eri_gc<%d,%d,%d,%d,%d>(" % (la, lb, lc, ld, S))
+    out.append("// (%s%s|%s%s) x %d x %d combinations, slice %d of %d, variant %d: %d accumulators per combination" % (
+        "spdfg"[la], "spdfg"[lb], "spdfg"[lc], "spdfg"[ld], CA, CC, S, slice_count((la, lb, lc, ld)), V, len(E_f)))
+    out.append("template <> HFG_DEV void eri_gc<%d,%d,%d,%d,%d,%d>(" % (la, lb, lc, ld, S, V))
     out.append("    const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,")
     out.append("    const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],")
     out.append("    const double* __restrict__ boys, double* __restrict__ I) {")
@@ -507,6 +514,9 @@ def main(outdir):
         for c in jk:
             for S in range(slice_count(c)):
                 f.write(gen_class_gc(*c, S=S))
+            if has_unit_variant(c):
+                assert slice_count(c) == 1
+                f.write(gen_class_gc(*c, S=0, V=1))
     with open(os.path.join(outdir, "eri_classes.txt"), "w") as f:
         f.write("\n".join(report) + "\n")
     # class list for the C++ side, in quartet_class() order
@@ -516,6 +526,8 @@ def main(outdir):
         f.write("#define HFG_NCLASS_ALL %d\n" % len(classes()))
         f.write("// Contraction-member caps (RA, RB, RC, RD) of the J/K classes, quartet_class() order.\n")
         f.write("#define HFG_GC_CAPS " + ", ".join("{%d, %d, %d, %d}" % gc_caps(c) for c in jk) + "\n")
+        f.write("// Classes with a one-member (segmented-basis) kernel variant, bit per quartet_class() index.\n")
+        f.write("#define HFG_UNIT_VARIANT_MASK %#xu\n" % sum(1 << i for i, c in enumerate(jk) if has_unit_variant(c)))
         f.write("// Output-slice mode of the J/K classes (0 whole block, 1 per a component, 2 per c, 3 per (a, c)).\n")
         f.write("#define HFG_SLICE_MODES " + ", ".join("%d" % slice_mode(c) for c in jk) + "\n")
         f.write("// Bra primitive record of the J/K kernels: doubles per record, first coefficient (ss / other pairs).\n")

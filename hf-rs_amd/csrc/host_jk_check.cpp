@@ -16,7 +16,7 @@
 #include "jk_plan.h"
 
 namespace hfg {
-template <int LA, int LB, int LC, int LD, int S>
+template <int LA, int LB, int LC, int LD, int S, int V>
 HFG_DEV void eri_gc(const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,
                     const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],
                     const double* __restrict__ boys, double* __restrict__ I);
@@ -77,19 +77,21 @@ struct Replay {
 };
 
 // Every output slice of a block, so that I holds the whole block.
-template <int LA, int LB, int LC, int LD, int CLS, int S = 0>
+template <int LA, int LB, int LC, int LD, int CLS, int V, int S = 0>
 void eri_gc_all(const double* br, int b0, int b1, const PrimPairs& kp, const double* kc, int k0, int nk, int ks,
                 const double AB[3], const double CD[3], const double* boys, double* I) {
-  eri_gc<LA, LB, LC, LD, S>(br, b0, b1, kp, kc, k0, nk, ks, AB, CD, boys, I);
+  eri_gc<LA, LB, LC, LD, S, V>(br, b0, b1, kp, kc, k0, nk, ks, AB, CD, boys, I);
   if constexpr (S + 1 < slice_count(CLS, LA, LC))
-    eri_gc_all<LA, LB, LC, LD, CLS, S + 1>(br, b0, b1, kp, kc, k0, nk, ks, AB, CD, boys, I);
+    eri_gc_all<LA, LB, LC, LD, CLS, V, S + 1>(br, b0, b1, kp, kc, k0, nk, ks, AB, CD, boys, I);
 }
 
-template <int LA, int LB, int LC, int LD, int CLS>
-void replay_class(Replay& r) {
+// The class's tasks evaluated with the kernels' generated recurrences of
+// variant V and digested element by element.
+template <int LA, int LB, int LC, int LD, int CLS, int V>
+void replay_class_v(Replay& r) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NCD = NC * ND, NBLK = NA * NB * NCD;
-  constexpr int CA = GC_CAPS[CLS][0] * GC_CAPS[CLS][1], CC = GC_CAPS[CLS][2] * GC_CAPS[CLS][3];
+  constexpr int CA = gc_cap(CLS, V, 0) * gc_cap(CLS, V, 1), CC = gc_cap(CLS, V, 2) * gc_cap(CLS, V, 3);
   const JKPlan& P = r.P;
   const PrimPairs kp = view_cols(P.kt, P.kt_size);
   const int n = r.n;
@@ -102,7 +104,7 @@ void replay_class(Replay& r) {
       const int ket = task.ket0 + lane;
       const ViewDev& vk = P.views[ket];
       const int j = ket - g.start;
-      eri_gc_all<LA, LB, LC, LD, CLS>(P.vb.data(), vb.p0, vb.p1, kp, P.kcoef.data() + g.cbase + j, g.base + j,
+      eri_gc_all<LA, LB, LC, LD, CLS, V>(P.vb.data(), vb.p0, vb.p1, kp, P.kcoef.data() + g.cbase + j, g.base + j,
                                       g.npp, g.size, vb.AB, vk.AB, slice<LA + LB + LC + LD>(), I.data());
       double w = 1.0;
       if (vb.same) w *= 0.5;
@@ -134,11 +136,21 @@ void replay_class(Replay& r) {
   }
 }
 
+// The class in the variant its plan chose.
+template <int LA, int LB, int LC, int LD, int CLS>
+void replay_class(Replay& r) {
+  if constexpr (((UNIT_VARIANT_MASK >> CLS) & 1u) != 0)
+    if ((r.P.unit_mask >> CLS) & 1u) return replay_class_v<LA, LB, LC, LD, CLS, 1>(r);
+  if ((r.P.unit_mask >> CLS) & 1u) throw HfgError{HFG_UNSUPPORTED, "no one-member variant of this class"};
+  replay_class_v<LA, LB, LC, LD, CLS, 0>(r);
+}
+
 }  // namespace
 
 // J and K of density D (n x n, row-major) through the J/K plan, on the CPU.
 // Outputs the plan's size figures in stats (if not null): generally
-// contracted shells, views, primitive pairs kept, tasks.
+// contracted shells, views, primitive pairs kept, tasks, and the mask of the
+// classes planned with one member per shell.
 extern "C" int hfgc_jk(const hfg_basis_view* b, const double* D, double* J, double* K, long long* stats) {
   try {
     std::vector<Shell> sh = group_shells(*b);
@@ -216,6 +228,7 @@ extern "C" int hfgc_jk(const hfg_basis_view* b, const double* D, double* J, doub
       stats[1] = (long long)plan.views.size();
       stats[2] = (long long)plan.prim_pairs_kept;
       stats[3] = (long long)plan.tasks.size();
+      stats[4] = (long long)plan.unit_mask;
     }
     return 0;
   } catch (HfgError& e) {

@@ -4,15 +4,15 @@
 
 namespace hfg {
 
-bool launch_jk_a(int cls, const JKArgs& a, int max_blocks, hipStream_t s);
-bool launch_jk_b(int cls, const JKArgs& a, int max_blocks, hipStream_t s);
-bool launch_jk_d(int cls, const JKArgs& a, int max_blocks, hipStream_t s);
-bool launch_jk_e(int cls, const JKArgs& a, int max_blocks, hipStream_t s);
+bool launch_jk_a(int cls, int v, const JKArgs& a, int max_blocks, hipStream_t s);
+bool launch_jk_b(int cls, int v, const JKArgs& a, int max_blocks, hipStream_t s);
+bool launch_jk_d(int cls, int v, const JKArgs& a, int max_blocks, hipStream_t s);
+bool launch_jk_e(int cls, int v, const JKArgs& a, int max_blocks, hipStream_t s);
 
-static bool launch_jk_c(int cls, const JKArgs& a, int max_blocks, hipStream_t s) {
+static bool launch_jk_c(int cls, int v, const JKArgs& a, int max_blocks, hipStream_t s) {
   switch (cls) {
-    case 17: launch_jk<2, 2, 1, 1>(a, max_blocks, s); return true;
-    case 18: launch_jk<2, 2, 2, 0>(a, max_blocks, s); return true;
+    case 17: launch_jk_v<2, 2, 1, 1>(v, a, max_blocks, s); return true;
+    case 18: launch_jk_v<2, 2, 2, 0>(v, a, max_blocks, s); return true;
     default: return false;
   }
 }
@@ -23,10 +23,10 @@ static_assert(class_index<0, 0, 0, 0>() == 0 && class_index<1, 1, 0, 0>() == 3 &
                   class_index<2, 2, 1, 1>() == 17 && class_index<2, 2, 2, 2>() == 20,
               "class order");
 
-void launch_jk_class(int cls, const JKArgs& a, int max_blocks, hipStream_t s) {
-  if (!launch_jk_a(cls, a, max_blocks, s) && !launch_jk_b(cls, a, max_blocks, s) &&
-      !launch_jk_c(cls, a, max_blocks, s) && !launch_jk_d(cls, a, max_blocks, s) &&
-      !launch_jk_e(cls, a, max_blocks, s))
+void launch_jk_class(int cls, int v, const JKArgs& a, int max_blocks, hipStream_t s) {
+  if (!launch_jk_a(cls, v, a, max_blocks, s) && !launch_jk_b(cls, v, a, max_blocks, s) &&
+      !launch_jk_c(cls, v, a, max_blocks, s) && !launch_jk_d(cls, v, a, max_blocks, s) &&
+      !launch_jk_e(cls, v, a, max_blocks, s))
     throw HfgError{HFG_UNSUPPORTED, "no J/K kernel for class " + std::to_string(cls)};
 }
 

@@ -4,9 +4,9 @@
 
 namespace hfg {
 
-bool launch_jk_e(int cls, const JKArgs& a, int max_blocks, hipStream_t s) {
+bool launch_jk_e(int cls, int v, const JKArgs& a, int max_blocks, hipStream_t s) {
   if (cls != 20) return false;
-  launch_jk<2, 2, 2, 2>(a, max_blocks, s);
+  launch_jk_v<2, 2, 2, 2>(v, a, max_blocks, s);
   return true;
 }
 

@@ -26,9 +26,9 @@
 
 namespace hfg {
 
-// Slice S of the blocks of CA x CC contraction combinations (generated, see
-// gen_eri.py).
-template <int LA, int LB, int LC, int LD, int S>
+// Slice S of the blocks of CA x CC contraction combinations, caps of variant
+// V (generated, see gen_eri.py).
+template <int LA, int LB, int LC, int LD, int S, int V>
 HFG_DEV void eri_gc(const double* __restrict__ br, int b0, int b1, const PrimPairs& kp,
                     const double* __restrict__ kc, int k0, int nk, int ks, const double AB[3], const double CD[3],
                     const double* __restrict__ boys, double* __restrict__ I);
@@ -192,13 +192,14 @@ __device__ __forceinline__ int dir_entry(const int* __restrict__ pre_, int nd, i
 // ROWS: rows is the workgroup's LDS accumulator of the bra's K rows and bra
 // Coulomb block (see k_jk_rows); otherwise J and K are updated in global
 // memory only.
-template <int LA, int LB, int LC, int LD, bool ROWS, int S>
+// V: the kernel variant (caps gc_cap(CLS, V, .)).
+template <int LA, int LB, int LC, int LD, bool ROWS, int S, int V>
 __device__ __forceinline__ void jk_task(const MolView& a, double screen0, const double* __restrict__ boys, int task,
                                         int lane, double* __restrict__ rows) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND, NBLK = NAB * NCD;
   constexpr int CLS = class_index<LA, LB, LC, LD>();
-  constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1], RC = GC_CAPS[CLS][2], RD = GC_CAPS[CLS][3];
+  constexpr int RA = gc_cap(CLS, V, 0), RB = gc_cap(CLS, V, 1), RC = gc_cap(CLS, V, 2), RD = gc_cap(CLS, V, 3);
   constexpr int CA = RA * RB, CC = RC * RD;
   constexpr int NR = RA * NA + RB * NB;  // K rows of the bra
   const JKTask t = a.tasks[task];
@@ -229,7 +230,7 @@ __device__ __forceinline__ void jk_task(const MolView& a, double screen0, const 
   const KetGroup g = a.groups[grp];
   const int j0 = ket - g.start;
   double I[CA * CC * NBLK];
-  eri_gc<LA, LB, LC, LD, S>(a.brec, vb.p0, vb.p1, a.kt, a.kcoef + g.cbase + j0, g.base + j0,
+  eri_gc<LA, LB, LC, LD, S, V>(a.brec, vb.p0, vb.p1, a.kt, a.kcoef + g.cbase + j0, g.base + j0,
                          g.npp, g.size, vb.AB, vk.AB, boys, I);  // idle lanes run a valid pair, masked below
 
   double w = 1.0;
@@ -466,16 +467,16 @@ __device__ __forceinline__ void jk_task(const MolView& a, double screen0, const 
 }
 
 // Slice s (wave-uniform) of a task.
-template <int LA, int LB, int LC, int LD, bool ROWS, int S = 0>
+template <int LA, int LB, int LC, int LD, bool ROWS, int V, int S = 0>
 __device__ __forceinline__ void jk_task_slice(const MolView& a, double screen0, const double* __restrict__ boys,
                                               int task, int s, int lane, double* __restrict__ rows) {
   if constexpr (S + 1 < slice_count<LA, LB, LC, LD>()) {
     if (s != S) {
-      jk_task_slice<LA, LB, LC, LD, ROWS, S + 1>(a, screen0, boys, task, s, lane, rows);
+      jk_task_slice<LA, LB, LC, LD, ROWS, V, S + 1>(a, screen0, boys, task, s, lane, rows);
       return;
     }
   }
-  jk_task<LA, LB, LC, LD, ROWS, S>(a, screen0, boys, task, lane, rows);
+  jk_task<LA, LB, LC, LD, ROWS, S, V>(a, screen0, boys, task, lane, rows);
 }
 
 // Persistent waves: every wave walks the launch's tasks (over all directory
@@ -483,7 +484,7 @@ __device__ __forceinline__ void jk_task_slice(const MolView& a, double screen0, 
 // with a stride of the total wave count, so the LDS copy of the Boys slice is
 // made once per workgroup.  J and K are updated in global memory (per-XCC
 // copies).
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int V>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
   __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
   load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
@@ -497,7 +498,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     const MolTab* T = a.tabs + ((ciptr)a.dir_mol)[e];
     const MolView m = mol_view(T);
     const int task = tab_task_begin(T, CLS) + gt - ((ciptr)a.dir_task)[e];
-    jk_task_slice<LA, LB, LC, LD, false>(m, a.screen0, boys, task, __builtin_amdgcn_readfirstlane(v - gt * NS),
+    jk_task_slice<LA, LB, LC, LD, false, V>(m, a.screen0, boys, task, __builtin_amdgcn_readfirstlane(v - gt * NS),
                                          lane, nullptr);
   }
 }
@@ -509,11 +510,11 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
 // rows plus CA x NAB; after the job the workgroup adds the nonzero entries to
 // the molecule's global copies and clears them.  Only the ket Coulomb updates
 // go to global memory directly.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int V>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves<LA, LB, LC, LD>()))) void k_jk_rows(JKArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB;
   constexpr int CLS = class_index<LA, LB, LC, LD>();
-  constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1], CA = RA * RB;
+  constexpr int RA = gc_cap(CLS, V, 0), RB = gc_cap(CLS, V, 1), CA = RA * RB;
   constexpr int NR = RA * NA + RB * NB;
   constexpr int NS = slice_count<LA, LB, LC, LD>();
   __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
@@ -536,7 +537,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     const int t0 = ((ciptr)m.jobs)[lj], t1 = ((ciptr)m.jobs)[lj + 1];
     for (int v = t0 * NS + wave; v < t1 * NS; v += nwave) {
       const int task = __builtin_amdgcn_readfirstlane(v / NS);
-      jk_task_slice<LA, LB, LC, LD, true>(m, a.screen0, boys, task, __builtin_amdgcn_readfirstlane(v - task * NS),
+      jk_task_slice<LA, LB, LC, LD, true, V>(m, a.screen0, boys, task, __builtin_amdgcn_readfirstlane(v - task * NS),
                                           lane, rows);
     }
     __syncthreads();
@@ -565,14 +566,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 }
 
 // LDS bytes of k_jk_rows for n basis functions, beyond the static Boys slice.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int V>
 constexpr size_t jk_rows_bytes(int n) {
   constexpr int CLS = class_index<LA, LB, LC, LD>();
-  constexpr int RA = GC_CAPS[CLS][0], RB = GC_CAPS[CLS][1];
+  constexpr int RA = gc_cap(CLS, V, 0), RB = gc_cap(CLS, V, 1);
   return sizeof(double) * ((size_t)(RA * ncart(LA) + RB * ncart(LB)) * n + RA * RB * ncart(LA) * ncart(LB));
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int V>
 void launch_jk(const JKArgs& a, int max_blocks, hipStream_t s) {
   const int waves_per_block = 4;
   // Device limits and the resident-workgroup count of this kernel for the
@@ -582,7 +583,7 @@ void launch_jk(const JKArgs& a, int max_blocks, hipStream_t s) {
   static thread_local size_t cached_lds = 0;
   int dev = 0;
   HIP_CHECK(hipGetDevice(&dev));
-  const size_t lds = jk_rows_bytes<LA, LB, LC, LD>(a.nmax);
+  const size_t lds = jk_rows_bytes<LA, LB, LC, LD, V>(a.nmax);
   if (dev != cached_dev) {
     HIP_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -592,20 +593,36 @@ void launch_jk(const JKArgs& a, int max_blocks, hipStream_t s) {
   if (a.use_rows && a.njob > 0 && lds + sizeof(double) * BOYS_ROWS * BOYS_SLICE + 64 <= (size_t)lds_limit) {
     if (lds != cached_lds) {
       int per_cu = 0;
-      HIP_CHECK(hipFuncSetAttribute((const void*)k_jk_rows<LA, LB, LC, LD>,
+      HIP_CHECK(hipFuncSetAttribute((const void*)k_jk_rows<LA, LB, LC, LD, V>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jk_rows<LA, LB, LC, LD>,
+      HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jk_rows<LA, LB, LC, LD, V>,
                                                              64 * waves_per_block, lds));
       resident = std::max(1, per_cu) * cus;
       cached_lds = lds;
     }
     const int blocks = std::max(1, std::min(std::min(a.njob, max_blocks), resident));
-    hipLaunchKernelGGL((k_jk_rows<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), lds, s, a);
+    hipLaunchKernelGGL((k_jk_rows<LA, LB, LC, LD, V>), dim3(blocks), dim3(64 * waves_per_block), lds, s, a);
     return;
   }
   if (a.ntask <= 0) return;
   const int blocks = std::min(max_blocks, (a.ntask + waves_per_block - 1) / waves_per_block);
-  hipLaunchKernelGGL((k_jk<LA, LB, LC, LD>), dim3(blocks), dim3(64 * waves_per_block), 0, s, a);
+  hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, V>), dim3(blocks), dim3(64 * waves_per_block), 0, s, a);
+}
+
+// Launch of the class in variant v; v = 1 exists for the classes of
+// UNIT_VARIANT_MASK only (the plan never asks for another).
+template <int LA, int LB, int LC, int LD>
+void launch_jk_v(int v, const JKArgs& a, int max_blocks, hipStream_t s) {
+  constexpr int CLS = class_index<LA, LB, LC, LD>();
+  if constexpr (((UNIT_VARIANT_MASK >> CLS) & 1u) != 0) {
+    if (v) {
+      launch_jk<LA, LB, LC, LD, 1>(a, max_blocks, s);
+      return;
+    }
+  } else {
+    if (v) throw HfgError{HFG_UNSUPPORTED, "no one-member kernel variant for J/K class " + std::to_string(CLS)};
+  }
+  launch_jk<LA, LB, LC, LD, 0>(a, max_blocks, s);
 }
 
 }  // namespace hfg

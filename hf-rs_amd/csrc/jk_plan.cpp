@@ -112,7 +112,6 @@ int ncomp(int L) { return (L + 1) * (L + 2) / 2; }
 JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>& Q, const std::vector<double>& ao_scale,
                     double screen, double prim_screen, const BlockFn& blocks, const int (*caps)[4]) {
   JKPlan plan;
-  if (!caps) caps = GC_CAPS;
   const bool trace = std::getenv("HFG_TRACE") != nullptr;
   auto t0 = std::chrono::steady_clock::now();
   auto lap = [&](const char* what) {
@@ -123,6 +122,22 @@ JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>&
   };
   const std::vector<GenShell> gs = merge_shells(shells);
   plan.n_gen_shells = (int)gs.size();
+  int auto_caps[HFG_NCLASS][4];
+  if (!caps) {
+    static const bool unit_on = [] {
+      const char* e = std::getenv("HFG_UNIT_VARIANT");
+      return !(e && e[0] == '0');
+    }();
+    int members[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // largest member count per L
+    for (const GenShell& g : gs) members[g.L & 7] = std::max(members[g.L & 7], (int)g.mem.size());
+    for (int c = 0; c < HFG_NCLASS; ++c) {
+      bool single = unit_on && ((UNIT_VARIANT_MASK >> c) & 1u);
+      for (int k = 0; k < 4; ++k) single = single && members[CLASS_L4[c][k]] <= 1;
+      for (int k = 0; k < 4; ++k) auto_caps[c][k] = gc_cap(c, single ? 1 : 0, k);
+      if (single) plan.unit_mask |= 1u << c;
+    }
+    caps = auto_caps;
+  }
   auto seg_pair = [](int a, int b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; };
 
   // ---- merged shell pairs and their unit-coefficient primitive pairs

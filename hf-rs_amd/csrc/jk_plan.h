@@ -62,6 +62,16 @@ struct BlockTask {
 
 // Contraction-member caps (RA, RB, RC, RD) of the J/K classes.
 constexpr int GC_CAPS[HFG_NCLASS][4] = {HFG_GC_CAPS};
+// Classes whose kernels are also compiled with one contraction member per
+// shell (bit c for class c): a molecule whose shells of the class's angular
+// momenta are all single -- every shell of a segmented basis -- runs that
+// variant, which does not carry the RA x RB x RC x RD combinations the capped
+// kernels evaluate for one nonzero.
+constexpr unsigned UNIT_VARIANT_MASK = HFG_UNIT_VARIANT_MASK;
+// Caps of class cls in kernel variant v (0: GC_CAPS, 1: one member per shell).
+constexpr int gc_cap(int cls, int v, int k) { return v ? 1 : GC_CAPS[cls][k]; }
+// Launch slots of a build: class c in variant v is slot c + v * HFG_NCLASS.
+constexpr int HFG_NSLOT = 2 * HFG_NCLASS;
 
 // Output-slice mode of the J/K classes: the Cartesian block of a quartet is
 // evaluated whole (0) or in slices of one a component (1), one c component
@@ -92,6 +102,8 @@ struct JKPlan {
   std::vector<size_t> job_begin;  // HFG_NCLASS + 1 entries
   size_t vb_size = 0, kt_size = 0, prim_pairs_kept = 0;
   int n_gen_shells = 0;
+  // Classes planned with one member per shell (kernel variant 1), bit c.
+  unsigned unit_mask = 0;
 };
 
 // Unscaled blocks of the listed quartets (one task per quartet, grouped by
@@ -103,9 +115,11 @@ using BlockFn = std::function<void(const PrimPairHost& prims, const std::vector<
 
 // Q: Schwarz bound per hf-rs shell pair (index a(a+1)/2 + b, a >= b);
 // ao_scale: component factor per AO.
-// caps: contraction-member caps per class (GC_CAPS when null; other
-// values only for work-count studies on the CPU, the kernels are compiled for
-// GC_CAPS).
+// caps: contraction-member caps per class.  When null, GC_CAPS, except that
+// a class of UNIT_VARIANT_MASK whose angular momenta have only single-member
+// shells in this basis gets unit caps and its unit_mask bit (HFG_UNIT_VARIANT=0
+// turns that off); other values only for work-count studies on the CPU, the
+// kernels are compiled for these two variants.
 JKPlan make_jk_plan(const std::vector<Shell>& shells, const std::vector<double>& Q, const std::vector<double>& ao_scale,
                     double screen, double prim_screen, const BlockFn& blocks, const int (*caps)[4] = nullptr);
 

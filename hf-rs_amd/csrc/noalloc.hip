@@ -29,7 +29,7 @@ __global__ void k_symmetrize(const double* Jacc, const double* Kacc, double* J, 
 constexpr int NOALLOC_MAX_N = 256;
 constexpr size_t NOALLOC_HEAD = 1024;
 constexpr size_t NOALLOC_ACC = (size_t)NXCC_SLOTS * NOALLOC_MAX_N * NOALLOC_MAX_N;
-static_assert(sizeof(MolTab) + HFG_NCLASS * sizeof(int) + 256 <= NOALLOC_HEAD * sizeof(double), "head too small");
+static_assert(sizeof(MolTab) + HFG_NSLOT * sizeof(int) + 256 <= NOALLOC_HEAD * sizeof(double), "head too small");
 
 static __device__ __align__(256) double scratch[NOALLOC_HEAD + 2 * NOALLOC_ACC];
 

@@ -243,7 +243,10 @@ int hfg_batch_jk_each(hfg_batch* batch);
 int hfg_batch_get_jk(hfg_batch* batch, int mol, double* j, double* k);
 /* Instrumentation of the fused builds, as hfg_profile / hfg_class_stats for
  * one molecule: per class, the algorithmic FLOPs and quartets of one build of
- * the whole batch, and the measured milliseconds over the profiled builds. */
+ * the whole batch, and the measured milliseconds over the profiled builds.
+ * enable = 2 profiles exclusively: the profiled builds run every class on one
+ * stream, so that each class's time is its own and not shared with the
+ * classes that otherwise overlap it on the side stream. */
 int hfg_batch_profile(hfg_batch* batch, int enable);
 int hfg_batch_class_stats(hfg_batch* batch, int cls, int32_t* l4, double* flops_per_launch, int64_t* quartets,
                           double* ms_total, int64_t* launches);
