@@ -21,7 +21,6 @@ struct hfg_basis {
 
 namespace {
 thread_local std::string g_berr;
-const char* ELEMENTS[] = {"H", "HE", "LI", "BE", "B", "C", "N", "O", "F", "NE"};
 
 struct ShellText {
   int L;  // -1: SP

@@ -12,10 +12,11 @@ import os
 import numpy as np
 
 ANGSTROM_TO_BOHR = 1.0 / 0.529177210903
-Z = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10}
+Z = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10,
+     "Na": 11, "Mg": 12, "Al": 13, "Si": 14, "P": 15, "S": 16, "Cl": 17, "Ar": 18}
 BASIS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "basis")
 BASIS_FILES = {
-    "sto-3g": "sto-3g.g94", "6-31g*": "6-31g_star.g94", "def2-svp": "def2-svp.g94", "cc-pvdz": "cc-pvdz.g94",
+    "sto-3g": "sto-3g.g94", "6-31g": "6-31g.g94", "6-31g*": "6-31g_star.g94", "def2-svp": "def2-svp.g94", "cc-pvdz": "cc-pvdz.g94",
     "cc-pvtz": "cc-pvtz.g94", "def2-universal-jkfit": "def2-universal-jkfit.g94",
     "def2-svp-rifit": "def2-svp-rifit.g94", "def2-svp-ri": "def2-svp-rifit.g94",
 }

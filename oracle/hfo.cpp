@@ -1992,6 +1992,161 @@ int hfo_df_jk(const double* bt, int rank, int n, const double* cocc, int nocc, d
   return 0;
 }
 
+// Pivoted Cholesky of the AO two-electron supermatrix: src/cholesky.rs:44-192.
+// The diagonal (pq|pq) comes from the (AB|AB) shell quartets; each step takes
+// the largest residual diagonal as the pivot (values within 1e-12 relative of
+// it are ties, resolved by the smallest packed pair index), the pivot's
+// column (pq|rs) from the complete shell quartets of the pivot's shell pair
+// (cached while consecutive pivots share it), and forms the new factor row
+// L(pq) = ((pq|rs) - sum_K L_K(pq) L_K(rs)) / sqrt(D_rs), L(rs) = sqrt(D_rs);
+// the residual diagonal drops by L^2.  It stops when the largest residual is
+// at most the threshold (or every pair is a pivot).  bt: rank x np, packed.
+int cholesky_eri(const hfo_basis& B, double threshold, double budget, int& rank, Mat& bt, double& max_residual,
+                 std::vector<int>& pivots) {
+  const int n = (int)B.f.size();
+  if (n == 0 || !std::isfinite(threshold) || threshold < 0)
+    return fail("Cholesky requires a basis and a finite, nonnegative threshold");
+  const int ns = (int)B.shell_start.size() - 1;
+  const size_t np = (size_t)n * (n + 1) / 2;
+  std::vector<int> shell_of(n);
+  for (int s = 0; s < ns; ++s)
+    for (int p = B.shell_start[s]; p < B.shell_start[s + 1]; ++p) shell_of[p] = s;
+  const size_t nsp = (size_t)ns * (ns + 1) / 2;
+  std::vector<std::pair<int, int>> sp_list(nsp);
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b <= a; ++b) sp_list[(size_t)a * (a + 1) / 2 + b] = {a, b};
+  std::vector<size_t> sp_of(np);
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q <= p; ++q) {
+      const size_t a = shell_of[p], b = shell_of[q];
+      sp_of[compound(p, q)] = a * (a + 1) / 2 + b;
+    }
+  auto start = [&](int s) { return B.shell_start[s]; };
+  auto len = [&](int s) { return B.shell_start[s + 1] - B.shell_start[s]; };
+  auto block = [&](int a, int b, int c, int d, std::vector<double>& r) {
+    ShellPair ab = make_pair(B, a, b), cd = make_pair(B, c, d);
+    quartet(B, ab, cd, false, r);
+  };
+  Mat diag(np, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (long x = 0; x < (long)nsp; ++x) {
+    const int a = sp_list[x].first, b = sp_list[x].second;
+    std::vector<double> r;
+    block(a, b, a, b, r);
+    const int na = len(a), nb = len(b);
+    for (int i = 0; i < na; ++i)
+      for (int j = 0; j < nb; ++j) {
+        const int p = start(a) + i, q = start(b) + j;
+        if (p < q) continue;
+        const int ij = i * nb + j;
+        diag[compound(p, q)] = r[(size_t)ij * na * nb + ij];
+      }
+  }
+  double scale = 1.0;
+  for (double v : diag) scale = std::max(scale, v);
+  for (double v : diag)
+    if (v < -1e-12 * scale) return fail("negative ERI diagonal; the Coulomb metric is not positive semidefinite");
+  std::vector<double> qsp(nsp, 0.0);
+  for (size_t x = 0; x < np; ++x) qsp[sp_of[x]] = std::max(qsp[sp_of[x]], std::sqrt(std::max(diag[x], 0.0)));
+  // columns of every AO pair of shell pair sp (those with r >= s), over all packed rows
+  auto columns_of = [&](size_t sp, std::vector<size_t>& idx, std::vector<Mat>& cols) {
+    const int r = sp_list[sp].first, s2 = sp_list[sp].second, nr = len(r), nq = len(s2);
+    std::vector<std::array<int, 2>> tg;
+    idx.clear();
+    for (int k = 0; k < nr; ++k)
+      for (int l = 0; l < nq; ++l)
+        if (start(r) + k >= start(s2) + l) {
+          tg.push_back({k, l});
+          idx.push_back(compound(start(r) + k, start(s2) + l));
+        }
+    cols.assign(tg.size(), Mat(np, 0.0));
+#pragma omp parallel for schedule(dynamic)
+    for (long x = 0; x < (long)nsp; ++x) {
+      if (!(qsp[x] * qsp[sp] > 1e-15)) continue;
+      const int a = sp_list[x].first, b = sp_list[x].second, na = len(a), nb = len(b);
+      std::vector<double> blk;
+      block(a, b, r, s2, blk);
+      for (int i = 0; i < na; ++i)
+        for (int j = 0; j < nb; ++j) {
+          const int p = start(a) + i, q = start(b) + j;
+          if (p < q) continue;
+          const size_t row = compound(p, q);
+          for (size_t t = 0; t < tg.size(); ++t) cols[t][row] = blk[((size_t)(i * nb + j) * nr + tg[t][0]) * nq + tg[t][1]];
+        }
+    }
+  };
+  std::vector<Mat> factors;
+  pivots.clear();
+  size_t cached_sp = (size_t)-1;
+  std::vector<size_t> cidx;
+  std::vector<Mat> cols;
+  for (;;) {
+    double largest = -INFINITY;
+    for (double v : diag) largest = std::max(largest, v);
+    size_t piv = 0;
+    while (!(diag[piv] >= largest - 1e-12 * std::fabs(largest))) ++piv;
+    const double maximum = diag[piv];
+    if (maximum <= threshold || factors.size() == np) {
+      max_residual = std::max(maximum, 0.0);
+      break;
+    }
+    if (8.0 * (double)np * (double)(factors.size() + 2) > budget) return fail("factor storage exceeds the memory budget");
+    const size_t sp = sp_of[piv];
+    if (sp != cached_sp) {
+      columns_of(sp, cidx, cols);
+      cached_sp = sp;
+    }
+    const Mat& col = cols[std::find(cidx.begin(), cidx.end(), piv) - cidx.begin()];
+    const double root = std::sqrt(maximum);
+    Mat row(np);
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < (long)np; ++k) {
+      double acc = 0.0;
+      for (const Mat& L : factors) acc += L[k] * L[piv];
+      row[k] = (col[k] - acc) / root;
+    }
+    row[piv] = root;
+    for (size_t k = 0; k < np; ++k) {
+      diag[k] -= row[k] * row[k];
+      if (diag[k] < -1e-10 * scale) return fail("nonpositive residual Coulomb diagonal");
+      diag[k] = std::max(diag[k], 0.0);
+    }
+    factors.push_back(std::move(row));
+    pivots.push_back((int)piv);
+  }
+  rank = (int)factors.size();
+  if (rank == 0) return fail("threshold exceeds every ERI diagonal; no Cholesky vectors were generated");
+  bt.assign((size_t)rank * np, 0.0);
+  for (int k = 0; k < rank; ++k) std::copy(factors[k].begin(), factors[k].end(), bt.begin() + (size_t)k * np);
+  return 0;
+}
+
+int hfo_cholesky_eri(const hfo_basis* b, double threshold, double budget, int* rank, double* max_residual,
+                     int* pivots, int max_pivots, double** b_out) {
+  try {
+    Mat bt;
+    std::vector<int> piv;
+    int r = 0;
+    double mr = 0;
+    if (int st = cholesky_eri(*b, threshold, budget, r, bt, mr, piv)) return st;
+    *rank = r;
+    *max_residual = mr;
+    for (int k = 0; k < r && k < max_pivots; ++k) pivots[k] = piv[k];
+    if (b_out) {
+      *b_out = (double*)std::malloc(bt.size() * sizeof(double));
+      std::memcpy(*b_out, bt.data(), bt.size() * sizeof(double));
+    }
+    return 0;
+  } catch (std::exception& e) {
+    return fail(e.what());
+  }
+}
+
+// Factorized RHF over packed factors bt (rank x np): src/hf_variants/ri_df_hf.rs:160-263.
+static int factorized_rhf(const hfo_basis* b, const Mat& bt, int rank, const std::vector<std::array<double, 4>>& atoms,
+                          double el, int max_iter, double e_tol, double r_tol, double d_tol, hfo_dfhf_result* res,
+                          double* p_out, double* f_out, double* j_out, double* k_out);
+
 // DF-RHF: src/hf_variants/ri_df_hf.rs:122-263.
 int hfo_df_rhf(const hfo_basis* b, const hfo_basis* aux, const double* zxyz, int natom, int charge, int max_iter,
                double e_tol, double r_tol, double d_tol, double cutoff, hfo_dfhf_result* res, double* p_out,
@@ -2009,6 +2164,45 @@ int hfo_df_rhf(const hfo_basis* b, const hfo_basis* aux, const double* zxyz, int
     int rank;
     Mat bt;
     if (int st = df_build(*b, *aux, cutoff, rank, bt)) return st;
+    return factorized_rhf(b, bt, rank, atoms, el, max_iter, e_tol, r_tol, d_tol, res, p_out, f_out, j_out, k_out);
+  } catch (std::exception& e) {
+    return fail(e.what());
+  }
+}
+
+// CD-HF: run_cd_hf (src/hf_variants/cholesky_df.rs:43-95): the Cholesky
+// factors to threshold delta (budget 512 MiB) in the factorized RHF driver,
+// residual tolerance max(sqrt(tol) 1e-2, 1e-11), density tolerance
+// sqrt(tol) 0.1 clamped to [1e-12, 1e-10].
+int hfo_cd_rhf(const hfo_basis* b, const double* zxyz, int natom, int charge, double delta, int max_iter, double tol,
+               hfo_dfhf_result* res, double* p_out) {
+  try {
+    std::memset(res, 0, sizeof(*res));
+    auto atoms = atoms_of(zxyz, natom);
+    double el = -charge;
+    for (auto& a : atoms) el += a[0];
+    if (natom == 0 || b->f.empty() || max_iter == 0 || !(tol > 0))
+      return fail("CD-HF requires finite geometry and positive convergence controls");
+    if (el <= 0 || std::fabs(el - std::round(el)) > 1e-8 || ((long)std::round(el)) % 2)
+      return fail("DF-RHF requires a positive even electron count");
+    int rank;
+    Mat bt;
+    double mr;
+    std::vector<int> piv;
+    if (int st = cholesky_eri(*b, delta, 512.0 * 1024 * 1024, rank, bt, mr, piv)) return st;
+    const double r_tol = std::max(std::sqrt(tol) * 1e-2, 1e-11);
+    const double d_tol = std::min(std::max(std::sqrt(tol) * 0.1, 1e-12), 1e-10);
+    return factorized_rhf(b, bt, rank, atoms, el, max_iter, tol, r_tol, d_tol, res, p_out, nullptr, nullptr, nullptr);
+  } catch (std::exception& e) {
+    return fail(e.what());
+  }
+}
+
+static int factorized_rhf(const hfo_basis* b, const Mat& bt, int rank, const std::vector<std::array<double, 4>>& atoms,
+                          double el, int max_iter, double e_tol, double r_tol, double d_tol, hfo_dfhf_result* res,
+                          double* p_out, double* f_out, double* j_out, double* k_out) {
+  try {
+    const int natom = (int)atoms.size();
     int n = (int)b->f.size(), no = (int)std::lround(el) / 2;
     Mat S, T, V;
     one_electron(*b, atoms, S, T, V);

@@ -109,6 +109,17 @@ int hfo_df_rhf(const hfo_basis* b, const hfo_basis* aux, const double* zxyz, int
                int charge, int max_iter, double e_tol, double r_tol, double d_tol, double cutoff,
                hfo_dfhf_result* res, double* p_out, double* f_out, double* j_out, double* k_out);
 
+/* Pivoted Cholesky of the AO ERI supermatrix to residual diagonal threshold
+ * (src/cholesky.rs:44-192): rank, largest remaining residual diagonal, the
+ * pivots (packed pair indices, first max_pivots of them) and, if b_out is not
+ * null, the factors (rank x np, packed; free with hfo_free). */
+int hfo_cholesky_eri(const hfo_basis* b, double threshold, double budget, int* rank,
+                     double* max_residual, int* pivots, int max_pivots, double** b_out);
+/* CD-HF (src/hf_variants/cholesky_df.rs:43-95): Cholesky factors to delta in
+ * the factorized RHF driver; res->n_aux is the number of Cholesky vectors. */
+int hfo_cd_rhf(const hfo_basis* b, const double* zxyz, int natom, int charge, double delta,
+               int max_iter, double tol, hfo_dfhf_result* res, double* p_out);
+
 /* Appendix-A style McMurchie-Davidson flop model of one primitive quartet of
  * angular momenta (la lb|lc ld), mirroring the loops of
  * src/shell_integrals.rs:101-133 and src/integrals.rs:70-101,259-346,905-971. */

@@ -72,6 +72,11 @@ def lib():
         L.hfo_df_rhf.argtypes = [C.c_void_p, C.c_void_p, dptr, C.c_int, C.c_int, C.c_int, C.c_double,
                                  C.c_double, C.c_double, C.c_double, C.POINTER(DfhfResult), C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hfo_cholesky_eri.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_double), C.c_void_p, C.c_int,
+                                       C.POINTER(C.POINTER(C.c_double))]
+        L.hfo_cd_rhf.argtypes = [C.c_void_p, dptr, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                 C.POINTER(DfhfResult), C.c_void_p]
         L.hfo_md_flops.argtypes = [C.c_int] * 4
         L.hfo_md_flops.restype = C.c_double
         L.hfo_fock_flops.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -284,6 +289,33 @@ def df_rhf(basis, aux, zxyz, charge=0, max_iter=100, e_tol=1e-12, r_tol=1e-10, d
     _check(lib().hfo_df_rhf(basis.h, aux.h, zxyz.ravel(), len(zxyz), charge, max_iter, e_tol, r_tol, d_tol, cutoff,
                             C.byref(res), _ptr(p), _ptr(f), _ptr(j), _ptr(k)))
     return res, p, f, j, k
+
+
+CD_BUDGET = 512 * 1024 * 1024  # bytes of stored factors (src/hf_variants/cholesky_df.rs:22)
+
+
+def cholesky_eri(basis, threshold, budget=CD_BUDGET):
+    """Pivoted Cholesky factors of the AO ERIs: (factors rank x np packed,
+    max residual diagonal, pivot list)."""
+    np_ = basis.n * (basis.n + 1) // 2
+    rank = C.c_int()
+    mr = C.c_double()
+    piv = np.zeros(np_, np.int32)
+    ptr = C.POINTER(C.c_double)()
+    _check(lib().hfo_cholesky_eri(basis.h, threshold, float(budget), C.byref(rank), C.byref(mr), _ptr(piv), np_,
+                                  C.byref(ptr)))
+    b = np.ctypeslib.as_array(ptr, shape=(rank.value * np_,)).copy().reshape(rank.value, np_)
+    lib().hfo_free(ptr)
+    return b, mr.value, [int(x) for x in piv[:rank.value]]
+
+
+def cd_rhf(basis, zxyz, delta, charge=0, max_iter=300, tol=1e-12):
+    """CD-HF energy (run_cd_hf): (result, density)."""
+    zxyz = np.ascontiguousarray(zxyz, dtype=np.float64)
+    res = DfhfResult()
+    p = np.zeros((basis.n, basis.n))
+    _check(lib().hfo_cd_rhf(basis.h, zxyz.ravel(), len(zxyz), charge, delta, max_iter, tol, C.byref(res), _ptr(p)))
+    return res, p
 
 
 def md_flops(la, lb, lc, ld):

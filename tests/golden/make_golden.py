@@ -52,6 +52,8 @@ def main(src):
     json.dump({"source": dfhf["source"], "water": water}, open(os.path.join(HERE, "df_hf_water.json"), "w"))
     sad = load("sad_reference.json")
     json.dump(sad, open(os.path.join(HERE, "sad_reference.json"), "w"))
+    cd = load("cd_hf_reference.json")
+    json.dump(cd, open(os.path.join(HERE, "cd_hf_reference.json"), "w"), indent=1)
     print("golden fixtures written to", HERE)
 
 
