@@ -2287,6 +2287,144 @@ static int factorized_rhf(const hfo_basis* b, const Mat& bt, int rank, const std
   }
 }
 
+// ---------------------------------------------------------------------------
+// Analytic RHF gradient for a density P and energy-weighted density W:
+// compute_gradient_from_density (src/gradient.rs:287-512).  Every term is
+// evaluated without the reference's 1e-11 / 1e-15 screening of small
+// products (exact up to rounding); the sums are the reference's:
+//   nuclear repulsion (:220-243);
+//   2 sum_nu [P (dT + dV) - W dS] over the derivative of each basis function
+//   (:411-447);
+//   -2 P_mu,nu (<dmu|V_C|nu> + <mu|V_C|dnu>) for the operator derivative at
+//   each nucleus C (:451-490);
+//   sum_{mu, nu, lambda >= sigma} gamma (dmu nu|lambda sigma) with gamma of
+//   compute_gradient_from_density (:293-303) (:493-509).
+// ---------------------------------------------------------------------------
+void differentiate_cg(const CG& cg, int tau, std::vector<CG>& terms) {  // src/gradient.rs:71-113
+  terms.clear();
+  CG p = cg;
+  p.l[tau] += 1;
+  for (auto& pr : p.prims) pr.c *= 2.0 * pr.e;
+  terms.push_back(p);
+  if (cg.l[tau] > 0) {
+    CG m = cg;
+    m.l[tau] -= 1;
+    const double lf = cg.l[tau];
+    for (auto& pr : m.prims) pr.c *= -lf;
+    terms.push_back(m);
+  }
+}
+
+double contracted_kinetic(const CG& a, const CG& b) {
+  double t = 0;
+  for (auto& pa : a.prims)
+    for (auto& pb : b.prims) t += pa.c * pb.c * prim_kinetic(pa.e, a.l, a.center, pb.e, b.l, b.center);
+  return t;
+}
+
+int rhf_gradient(const hfo_basis& B, const std::vector<std::array<double, 4>>& atoms, const double* P, const double* W,
+                 std::vector<double>& grad) {
+  const int n = (int)B.f.size(), na = (int)atoms.size();
+  grad.assign((size_t)3 * na, 0.0);
+  // nuclear repulsion
+  for (int a = 0; a < na; ++a)
+    for (int b = 0; b < na; ++b) {
+      if (a == b) continue;
+      double d[3], r2 = 0;
+      for (int k = 0; k < 3; ++k) {
+        d[k] = atoms[a][1 + k] - atoms[b][1 + k];
+        r2 += d[k] * d[k];
+      }
+      const double f = atoms[a][0] * atoms[b][0] / (r2 * std::sqrt(r2));
+      for (int k = 0; k < 3; ++k) grad[3 * a + k] -= f * d[k];
+    }
+  std::vector<std::array<std::vector<CG>, 3>> dfn(n);
+  for (int mu = 0; mu < n; ++mu)
+    for (int t = 0; t < 3; ++t) differentiate_cg(B.f[mu], t, dfn[mu][t]);
+  // one-electron basis-function derivatives
+  std::vector<double> g1((size_t)3 * n, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int task = 0; task < 3 * n; ++task) {
+    const int mu = task / 3, t = task % 3;
+    double ds = 0, dt = 0, dv = 0;
+    for (int nu = 0; nu < n; ++nu) {
+      const double p = P[(size_t)mu * n + nu], w = W[(size_t)mu * n + nu];
+      for (const CG& term : dfn[mu][t]) {
+        ds += w * contracted_overlap(term, B.f[nu]);
+        dt += p * contracted_kinetic(term, B.f[nu]);
+        dv += p * contracted_nuclear(term, B.f[nu], atoms);
+      }
+    }
+    g1[task] = 2.0 * (dt + dv - ds);
+  }
+  for (int mu = 0; mu < n; ++mu)
+    for (int t = 0; t < 3; ++t) grad[3 * B.f[mu].atom + t] += g1[3 * mu + t];
+  // operator derivative of the nuclear attraction, per nucleus
+  std::vector<double> g2((size_t)n * 3 * na, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int mu = 0; mu < n; ++mu) {
+    double* out = &g2[(size_t)mu * 3 * na];
+    for (int nu = 0; nu <= mu; ++nu) {
+      const double p = P[(size_t)mu * n + nu];
+      for (int c = 0; c < na; ++c) {
+        const std::vector<std::array<double, 4>> one{atoms[c]};
+        for (int t = 0; t < 3; ++t) {
+          double v = 0;
+          for (const CG& term : dfn[mu][t]) v += contracted_nuclear(term, B.f[nu], one);
+          if (mu != nu)
+            for (const CG& term : dfn[nu][t]) v += contracted_nuclear(term, B.f[mu], one);
+          out[3 * c + t] -= 2.0 * p * v;
+        }
+      }
+    }
+  }
+  for (int mu = 0; mu < n; ++mu)
+    for (size_t x = 0; x < (size_t)3 * na; ++x) grad[x] += g2[(size_t)mu * 3 * na + x];
+  // two-electron: sum over mu, nu, lambda >= sigma of gamma (dmu nu|lambda sigma)
+  std::vector<std::pair<int, int>> ls;
+  std::vector<std::vector<PPData>> lsd;
+  for (int l = 0; l < n; ++l)
+    for (int s2 = 0; s2 <= l; ++s2) {
+      ls.push_back({l, s2});
+      lsd.push_back(basis_pair_data(B.f[l], B.f[s2]));
+    }
+  auto Pm = [&](int i, int j) { return P[(size_t)i * n + j]; };
+  std::vector<double> g3((size_t)3 * n, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int task = 0; task < 3 * n; ++task) {
+    const int mu = task / 3, t = task % 3;
+    std::vector<double> r;
+    double acc = 0;
+    for (int nu = 0; nu < n; ++nu)
+      for (const CG& term : dfn[mu][t]) {
+        const std::vector<PPData> ab = basis_pair_data(term, B.f[nu]);
+        for (size_t x = 0; x < ls.size(); ++x) {
+          const int l = ls[x].first, sg = ls[x].second;
+          const double gam = l == sg ? 2.0 * Pm(mu, nu) * Pm(l, l) - Pm(mu, l) * Pm(nu, l)
+                                     : 4.0 * Pm(mu, nu) * Pm(l, sg) - Pm(mu, l) * Pm(nu, sg) - Pm(mu, sg) * Pm(nu, l);
+          if (gam == 0.0) continue;
+          acc += gam * pair_eri(ab, lsd[x], r);
+        }
+      }
+    g3[task] = acc;
+  }
+  for (int mu = 0; mu < n; ++mu)
+    for (int t = 0; t < 3; ++t) grad[3 * B.f[mu].atom + t] += g3[3 * mu + t];
+  return 0;
+}
+
+int hfo_rhf_gradient(const hfo_basis* b, const double* zxyz, int natom, const double* p, const double* w,
+                     double* grad) {
+  try {
+    std::vector<double> g;
+    if (int st = rhf_gradient(*b, atoms_of(zxyz, natom), p, w, g)) return st;
+    std::memcpy(grad, g.data(), g.size() * sizeof(double));
+    return 0;
+  } catch (std::exception& e) {
+    return fail(e.what());
+  }
+}
+
 // McMurchie-Davidson flop model per primitive quartet (mirrors the loop
 // structure of src/shell_integrals.rs:101-133 and src/integrals.rs:70-101,
 // 259-346, 905-971): setup 20, Boys 8+16(m+1), R scaling 2(L+1), R recursion

@@ -120,6 +120,12 @@ int hfo_cholesky_eri(const hfo_basis* b, double threshold, double budget, int* r
 int hfo_cd_rhf(const hfo_basis* b, const double* zxyz, int natom, int charge, double delta,
                int max_iter, double tol, hfo_dfhf_result* res, double* p_out);
 
+/* Analytic RHF nuclear gradient (natom x 3, Eh/bohr) for density p and
+ * energy-weighted density w (n x n each): compute_gradient_from_density
+ * (src/gradient.rs:287-512), without its screening of small products. */
+int hfo_rhf_gradient(const hfo_basis* b, const double* zxyz, int natom, const double* p,
+                     const double* w, double* grad);
+
 /* Appendix-A style McMurchie-Davidson flop model of one primitive quartet of
  * angular momenta (la lb|lc ld), mirroring the loops of
  * src/shell_integrals.rs:101-133 and src/integrals.rs:70-101,259-346,905-971. */

@@ -72,6 +72,7 @@ def lib():
         L.hfo_df_rhf.argtypes = [C.c_void_p, C.c_void_p, dptr, C.c_int, C.c_int, C.c_int, C.c_double,
                                  C.c_double, C.c_double, C.c_double, C.POINTER(DfhfResult), C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hfo_rhf_gradient.argtypes = [C.c_void_p, dptr, C.c_int, dptr, dptr, dptr]
         L.hfo_cholesky_eri.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int),
                                        C.POINTER(C.c_double), C.c_void_p, C.c_int,
                                        C.POINTER(C.POINTER(C.c_double))]
@@ -289,6 +290,16 @@ def df_rhf(basis, aux, zxyz, charge=0, max_iter=100, e_tol=1e-12, r_tol=1e-10, d
     _check(lib().hfo_df_rhf(basis.h, aux.h, zxyz.ravel(), len(zxyz), charge, max_iter, e_tol, r_tol, d_tol, cutoff,
                             C.byref(res), _ptr(p), _ptr(f), _ptr(j), _ptr(k)))
     return res, p, f, j, k
+
+
+def rhf_gradient(basis, zxyz, p, w):
+    """Nuclear gradient (natom x 3) for density p and energy-weighted density w
+    (compute_gradient_from_density, src/gradient.rs:287-512)."""
+    zxyz = np.ascontiguousarray(zxyz, dtype=np.float64)
+    g = np.zeros((len(zxyz), 3))
+    _check(lib().hfo_rhf_gradient(basis.h, zxyz.ravel(), len(zxyz), np.ascontiguousarray(p, dtype=np.float64),
+                                  np.ascontiguousarray(w, dtype=np.float64), g))
+    return g
 
 
 CD_BUDGET = 512 * 1024 * 1024  # bytes of stored factors (src/hf_variants/cholesky_df.rs:22)

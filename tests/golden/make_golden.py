@@ -32,7 +32,7 @@ def main(src):
     scf = load("scf_reference.json")
     systems = {}
     for name, v in scf["systems"].items():
-        res = {b: {"reference": r["reference"], "energy": r["energy"], "n_ao": r["n_ao"]}
+        res = {b: {"reference": r["reference"], "energy": r["energy"], "n_ao": r["n_ao"], "gradient": r["gradient"]}
                for b, r in v["results"].items()}
         systems[name] = {"geometry_bohr": v["geometry_bohr"], "charge": v["charge"], "spin_2s": v["spin_2s"],
                          "results": res}

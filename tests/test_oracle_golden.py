@@ -347,3 +347,25 @@ def test_cholesky_rejects_invalid_requests():
         O.cholesky_eri(b, 1e-8, budget=16)
     with pytest.raises(RuntimeError):
         O.cholesky_eri(b, 10.0)
+
+
+RHF_GRADIENT_CASES = [(m, b) for m in ("h2", "lih", "h2o", "nh3") for b in ("sto-3g", "6-31g*")]
+
+
+@pytest.mark.parametrize("name,basis", RHF_GRADIENT_CASES)
+def test_rhf_gradient_matches_pyscf(name, basis):
+    """The oracle's analytic gradient (src/gradient.rs:287-512) at the converged
+    RHF density (CDIIS, tol 1e-12, as tests/scf_reference_test.rs:74-94) against
+    PySCF's gradients in the reference's fixture: 1e-6 (the reference's own
+    bound), and the forces sum to zero (translational invariance) to 1e-9.
+    W = 1/2 P F P = 2 C_occ eps C_occ^T at convergence."""
+    sysd = golden("scf_energies.json")["systems"][name]
+    ref = sysd["results"][basis]
+    syms, xyz = parse_geometry(sysd["geometry_bohr"])
+    mol = M.Molecule(syms, xyz)
+    b = O.Basis.build(M.basis_text(basis), syms, xyz)
+    res, p, f, _ = O.scf_rhf(b, mol.zxyz, tol=1e-12, max_iter=200, conv="cdiis")
+    assert res.converged and abs(res.total_energy - ref["energy"]) < 1e-8
+    g = O.rhf_gradient(b, mol.zxyz, p, 0.5 * p @ f @ p)
+    assert np.abs(g - np.array(ref["gradient"])).max() < 1e-6
+    assert np.abs(g.sum(axis=0)).max() < 1e-9
