@@ -58,7 +58,8 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_batch_create", "hfg_batch_destroy", "hfg_batch_info", "hfg_batch_set_density", "hfg_batch_jk",
             "hfg_batch_get_jk", "hfg_batch_jk_each", "hfg_df_jk_ex", "hfg_contract_jk_multi", "hfg_server_create",
             "hfg_server_destroy", "hfg_server_submit", "hfg_server_run", "hfg_server_result", "hfg_server_stats",
-            "hfg_scf_rhf_guess", "hfg_eri_mo", "hfg_batch_profile", "hfg_batch_class_stats"]
+            "hfg_scf_rhf_guess", "hfg_eri_mo", "hfg_batch_profile", "hfg_batch_class_stats", "hfg_cd_build",
+            "hfg_cd_rhf"]
 
 
 class ServerOptions(C.Structure):
@@ -85,6 +86,10 @@ class DfhfOptions(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("energy_tol", C.c_double), ("residual_tol", C.c_double),
                 ("density_tol", C.c_double), ("metric_cutoff", C.c_double), ("k_precision", C.c_int32),
                 ("switch_residual", C.c_double)]
+
+
+class CdInfo(C.Structure):
+    _fields_ = [("rank", C.c_int32), ("max_residual", C.c_double)]
 
 
 class DfhfResult(C.Structure):
@@ -137,6 +142,9 @@ def lib():
         L.hfg_df_jk_ex.argtypes = [vp, vp, i32, vp, vp, i32, C.POINTER(dbl)]
         L.hfg_df_rhf.argtypes = [C.POINTER(BasisView), C.POINTER(BasisView), vp, i32, i32, C.POINTER(DfhfOptions),
                                  C.POINTER(DfhfResult), vp, vp, vp, vp]
+        L.hfg_cd_build.argtypes = [C.POINTER(BasisView), dbl, dbl, C.POINTER(vp), C.POINTER(CdInfo), vp, i32]
+        L.hfg_cd_rhf.argtypes = [C.POINTER(BasisView), vp, i32, i32, dbl, dbl, C.POINTER(DfhfOptions),
+                                 C.POINTER(DfhfResult), C.POINTER(CdInfo), vp, vp]
         L.hfg_batch_create.argtypes = [vp, i32, C.POINTER(vp)]
         L.hfg_batch_destroy.argtypes = [vp]
         L.hfg_batch_info.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3 + [C.POINTER(dbl)]
@@ -509,12 +517,16 @@ class Context:
 
 
 class DfTensor:
-    """Mirror of DfTensor (src/df.rs:80-174), resident in HBM."""
+    """Mirror of DfTensor (src/df.rs:80-174), resident in HBM.  aux None: the
+    factors come from cholesky()."""
 
-    def __init__(self, basis, aux, metric_cutoff=1e-10):
+    def __init__(self, basis, aux, metric_cutoff=1e-10, _handle=None):
         self.basis, self.aux = basis, aux
         self.h = C.c_void_p()
-        _check(lib().hfg_df_build(C.byref(basis.view), C.byref(aux.view), metric_cutoff, C.byref(self.h)))
+        if _handle is not None:
+            self.h = _handle
+        else:
+            _check(lib().hfg_df_build(C.byref(basis.view), C.byref(aux.view), metric_cutoff, C.byref(self.h)))
         v = [C.c_int64() for _ in range(3)]
         _check(lib().hfg_df_info(self.h, *[C.byref(x) for x in v]))
         self.n, self.n_aux, self.rank = (x.value for x in v)
@@ -551,6 +563,44 @@ class DfTensor:
 
 
 DF_PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+CD_HF_FACTOR_BUDGET = 512 * 1024 * 1024  # bytes (src/hf_variants/cholesky_df.rs:22)
+
+
+def cholesky(basis, threshold, budget=CD_HF_FACTOR_BUDGET):
+    """pivoted_cholesky_eri (src/cholesky.rs:44-192): (DfTensor of the
+    Cholesky vectors, info dict with max_residual and pivots)."""
+    h = C.c_void_p()
+    info = CdInfo()
+    np_ = basis.n * (basis.n + 1) // 2
+    piv = np.zeros(np_, np.int32)
+    _check(lib().hfg_cd_build(C.byref(basis.view), threshold, float(budget), C.byref(h), C.byref(info), _p(piv), np_))
+    t = DfTensor(basis, None, _handle=h)
+    return t, {"rank": info.rank, "max_residual": info.max_residual, "pivots": [int(x) for x in piv[:info.rank]]}
+
+
+def cd_rhf(basis, zxyz, delta, charge=0, max_iter=300, tol=1e-12, budget=CD_HF_FACTOR_BUDGET,
+           k_precision="fp64", switch_residual=1e-4):
+    """CD-HF (run_cd_hf, src/hf_variants/cholesky_df.rs:80-95): the Cholesky
+    factors to delta in the factorized RHF driver, with energy tolerance tol,
+    residual tolerance max(sqrt(tol) 1e-2, 1e-11) and density tolerance
+    sqrt(tol) 0.1 clamped to [1e-12, 1e-10]."""
+    if k_precision not in DF_PRECISION:
+        raise ValueError("k_precision must be one of %s" % sorted(DF_PRECISION))
+    r_tol = max(math.sqrt(tol) * 1e-2, 1e-11)
+    d_tol = min(max(math.sqrt(tol) * 0.1, 1e-12), 1e-10)
+    opt = DfhfOptions(max_iter, tol, r_tol, d_tol, 1e-10, DF_PRECISION[k_precision], switch_residual)
+    res = DfhfResult()
+    info = CdInfo()
+    zxyz = _atoms(zxyz)
+    n = basis.n
+    p, f = np.zeros((n, n)), np.zeros((n, n))
+    _check(lib().hfg_cd_rhf(C.byref(basis.view), _p(zxyz), len(zxyz), charge, delta, float(budget), C.byref(opt),
+                            C.byref(res), C.byref(info), _p(p), _p(f)))
+    out = {k_: getattr(res, k_) for k_, _ in DfhfResult._fields_}
+    out.update(density=p, fock=f, n_cholesky_vectors=info.rank, cholesky_max_error=info.max_residual)
+    return out
 
 
 def df_rhf(basis, aux, zxyz, charge=0, max_iter=100, e_tol=1e-12, r_tol=1e-10, d_tol=1e-10, metric_cutoff=1e-10,

@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <future>
 #include <map>
 #include <memory>
@@ -25,6 +26,7 @@
 #include <string>
 
 #include "../../include/hfg.h"
+#include "cholesky.h"
 #include "engine.h"
 #include "la_device.h"
 #include "scf_host.h"
@@ -202,6 +204,26 @@ struct DfEngine {
     HIP_CHECK(hipStreamCreate(&stream));
     bl.reset(new Blas(stream));
     build(cutoff);
+  }
+  // Factors of the pivoted Cholesky decomposition of the AO ERIs
+  // (src/cholesky.rs:44-192, cholesky.hip) in place of the fitted B, for
+  // CD-HF: n_aux = rank = the number of Cholesky vectors.
+  DfEngine(const hfg_basis_view& basis, double threshold, double budget, CholeskyInfo& info) {
+    orb = group_shells(basis);
+    n = basis.n_functions;
+    for (auto& s : orb)
+      if (s.L > 2) throw HfgError{HFG_UNSUPPORTED, "Cholesky factors compiled for orbital shells up to d"};
+    HIP_CHECK(hipStreamCreate(&stream));
+    try {
+      bl.reset(new Blas(stream));
+      cholesky_factors(orb, n, stream, *bl, threshold, budget, d_B, rank, info);
+    } catch (...) {
+      bl.reset();
+      (void)hipStreamDestroy(stream);
+      stream = nullptr;
+      throw;
+    }
+    naux = rank;
   }
   ~DfEngine() {
     bl.reset();
@@ -560,14 +582,17 @@ struct DfEngine {
 // DF-RHF (src/hf_variants/ri_df_hf.rs:160-263): core-Hamiltonian guess, CDIIS
 // from the second iteration, convergence on |dE|, the projected commutator
 // residual and max|dP|, then one final Fock build with the new orbitals.
-static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const double* zxyz, int natom, int charge,
-                  const hfg_dfhf_options* opt, hfg_dfhf_result* res, double* p_out, double* f_out, double* j_out,
-                  double* k_out) {
+// make_engine builds the factors (DF: metric, 3-index and B; CD-HF: the
+// Cholesky vectors) while the one-electron matrices are formed on a second
+// host thread.
+static int df_rhf(const hfg_basis_view* basis, const std::function<DfEngine*()>& make_engine, const double* zxyz,
+                  int natom, int charge, const hfg_dfhf_options* opt, hfg_dfhf_result* res, double* p_out,
+                  double* f_out, double* j_out, double* k_out) {
   using clk = std::chrono::steady_clock;
   const auto t0 = clk::now();
   std::memset(res, 0, sizeof *res);
   if (natom <= 0 || opt->max_iter <= 0 || !(opt->energy_tol > 0) || !(opt->residual_tol > 0) ||
-      !(opt->density_tol > 0) || !(opt->metric_cutoff > 0))
+      !(opt->density_tol > 0))
     throw HfgError{HFG_INVALID_INPUT, "DF-HF requires finite geometry and positive convergence controls"};
   if (opt->k_precision < 0 || opt->k_precision > 2 || !(opt->switch_residual > 0))
     throw HfgError{HFG_INVALID_INPUT, "DF-HF k_precision must be 0, 1 or 2 and switch_residual positive"};
@@ -586,7 +611,7 @@ static int df_rhf(const hfg_basis_view* basis, const hfg_basis_view* auxv, const
   auto one_e = std::async(std::launch::async, [&] { one_electron(sh1, n1, atoms, S, T, V); });
   std::unique_ptr<DfEngine> df_p;
   try {
-    df_p.reset(new DfEngine(*basis, *auxv, opt->metric_cutoff));
+    df_p.reset(make_engine());
   } catch (...) {
     one_e.wait();
     throw;
@@ -883,7 +908,54 @@ int hfg_df_rhf(const hfg_basis_view* basis, const hfg_basis_view* aux, const dou
                double* k_out) {
   if (!basis || !aux || !zxyz || !opt || !res) return dferr(HFG_INVALID_INPUT, "null argument");
   try {
-    return df_rhf(basis, aux, zxyz, natom, charge, opt, res, p_out, f_out, j_out, k_out);
+    if (!(opt->metric_cutoff > 0)) throw HfgError{HFG_INVALID_INPUT, "DF metric cutoff must be finite and positive"};
+    return df_rhf(
+        basis, [&] { return new DfEngine(*basis, *aux, opt->metric_cutoff); }, zxyz, natom, charge, opt, res, p_out,
+        f_out, j_out, k_out);
+  } catch (HfgError& e) {
+    return dferr(e.code, e.msg);
+  }
+}
+
+int hfg_cd_build(const hfg_basis_view* basis, double threshold, double budget_bytes, hfg_df** out, hfg_cd_info* info,
+                 int32_t* pivots, int32_t max_pivots) {
+  if (!basis || !out) return dferr(HFG_INVALID_INPUT, "null argument");
+  try {
+    CholeskyInfo ci;
+    auto* d = new hfg_df;
+    try {
+      d->e.reset(new DfEngine(*basis, threshold, budget_bytes, ci));
+    } catch (...) {
+      delete d;
+      throw;
+    }
+    if (info) {
+      info->rank = ci.rank;
+      info->max_residual = ci.max_residual;
+    }
+    if (pivots)
+      for (int k = 0; k < ci.rank && k < max_pivots; ++k) pivots[k] = ci.pivots[k];
+    *out = d;
+    return HFG_OK;
+  } catch (HfgError& e) {
+    return dferr(e.code, e.msg);
+  }
+}
+
+int hfg_cd_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, double delta,
+               double budget_bytes, const hfg_dfhf_options* opt, hfg_dfhf_result* res, hfg_cd_info* info,
+               double* p_out, double* f_out) {
+  if (!basis || !zxyz || !opt || !res) return dferr(HFG_INVALID_INPUT, "null argument");
+  try {
+    CholeskyInfo ci;
+    const int st = df_rhf(
+        basis, [&] { return new DfEngine(*basis, delta, budget_bytes, ci); }, zxyz, natom, charge, opt, res, p_out,
+        f_out, nullptr, nullptr);
+    if (info) {
+      info->rank = ci.rank;
+      info->max_residual = ci.max_residual;
+    }
+    return st;
   } catch (HfgError& e) {
     return dferr(e.code, e.msg);
   }

@@ -251,6 +251,28 @@ int hfg_batch_profile(hfg_batch* batch, int enable);
 int hfg_batch_class_stats(hfg_batch* batch, int cls, int32_t* l4, double* flops_per_launch, int64_t* quartets,
                           double* ms_total, int64_t* launches);
 
+/* Pivoted Cholesky factors of the AO two-electron integrals
+ * (pivoted_cholesky_eri, src/cholesky.rs:44-192): the pivot is the largest
+ * residual diagonal (ties within 1e-12 relative resolve to the smallest packed
+ * pair index); it stops when the largest residual is at most threshold.  The
+ * factors stay in HBM behind a DF handle (rank = number of vectors), so
+ * hfg_df_jk / hfg_df_jk_ex / hfg_df_info apply to them as to fitted B.
+ * budget_bytes bounds the packed factors as CD_HF_FACTOR_BUDGET does
+ * (src/hf_variants/cholesky_df.rs:22; HFG_INVALID_INPUT beyond it).  pivots
+ * (optional) receives the first max_pivots pivots, packed AO-pair indices. */
+typedef struct {
+  int32_t rank;
+  double max_residual; /* largest remaining residual diagonal (CholeskyInfo) */
+} hfg_cd_info;
+int hfg_cd_build(const hfg_basis_view* basis, double threshold, double budget_bytes, hfg_df** out,
+                 hfg_cd_info* info, int32_t* pivots, int32_t max_pivots);
+/* CD-HF (run_cd_hf_with_options, src/hf_variants/cholesky_df.rs:43-95): the
+ * factors to delta in the factorized RHF driver of hfg_df_rhf (whose
+ * metric_cutoff field is unused here); res->n_aux = the number of vectors. */
+int hfg_cd_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, double delta,
+               double budget_bytes, const hfg_dfhf_options* opt, hfg_dfhf_result* res, hfg_cd_info* info,
+               double* p_out, double* f_out);
+
 /* Served and batched RHF (no reference counterpart: the reference solves one
  * molecule per call of run_scf_in_space_with_convergence, src/scf.rs:410-627,
  * or run_scf_with_initial_density, :630-639).  A server holds `capacity`

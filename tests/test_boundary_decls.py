@@ -131,7 +131,8 @@ def test_ctypes_mirrors_match_the_header():
     pairs = {"hfg_basis_view": hf.BasisView, "hfg_scf_options": hf.ScfOptions, "hfg_scf_result": hf.ScfResult,
              "hfg_dfhf_options": hf.DfhfOptions, "hfg_dfhf_result": hf.DfhfResult,
              "hfg_server_options": hf.ServerOptions, "hfg_step_result": hf.StepResult,
-             "hfg_request_result": hf.RequestResult, "hfg_server_stats_t": hf.ServerStats}
+             "hfg_request_result": hf.RequestResult, "hfg_server_stats_t": hf.ServerStats,
+             "hfg_cd_info": hf.CdInfo}
     assert set(pairs) == set(cs), sorted(set(cs) ^ set(pairs))
     for cname, py in pairs.items():
         fields = cs[cname]

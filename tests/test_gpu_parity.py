@@ -156,6 +156,24 @@ def test_jk_linear_in_density_at_full_size(hf):
     assert (pp * jp).sum() > 0 and (pp * kp).sum() > 0
 
 
+def test_jk_at_full_size_matches_oracle_fixture(hf):
+    """(H2O)16/cc-pVDZ, n = 400: J and K of the seeded density against the
+    oracle's direct J/K (tests/golden/make_h2o16_jk.py, upper triangles), max
+    abs 1e-10 -- the whole quartet set of the full-size workload, every class
+    and every molecule-sized table, not only a small cluster."""
+    fx = np.load(os.path.join(GOLDEN, "h2o16_ccpvdz_jk.npz"))
+    mol = M.water_cluster((4, 2, 2), seed=2)
+    ctx = hf.Context(hf.Basis.from_g94(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz))
+    n = int(fx["n"])
+    assert ctx.n == n == 400
+    c = np.random.default_rng(int(fx["seed"])).standard_normal((n, 40)) * 0.2
+    j, k = ctx.contract_jk(c @ c.T)
+    iu = np.triu_indices(n)
+    ej, ek = np.abs(j[iu] - fx["j_upper"]).max(), np.abs(k[iu] - fx["k_upper"]).max()
+    print("n=%d  J %.2e  K %.2e  (|J| %.2f, |K| %.2f)" % (n, ej, ek, np.abs(j).max(), np.abs(k).max()))
+    assert ej <= 1e-10 and ek <= 1e-10
+
+
 def test_rhf_h2o_sto3g_matches_pyscf_and_oracle(hf):
     mol = M.water_config1()
     text = M.basis_text("sto-3g")
