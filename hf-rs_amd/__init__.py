@@ -59,7 +59,7 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_batch_get_jk", "hfg_batch_jk_each", "hfg_df_jk_ex", "hfg_contract_jk_multi", "hfg_server_create",
             "hfg_server_destroy", "hfg_server_submit", "hfg_server_run", "hfg_server_result", "hfg_server_stats",
             "hfg_scf_rhf_guess", "hfg_eri_mo", "hfg_batch_profile", "hfg_batch_class_stats", "hfg_cd_build",
-            "hfg_cd_rhf"]
+            "hfg_cd_rhf", "hfg_rhf_gradient", "hfg_graph_launches"]
 
 
 class ServerOptions(C.Structure):
@@ -142,6 +142,8 @@ def lib():
         L.hfg_df_jk_ex.argtypes = [vp, vp, i32, vp, vp, i32, C.POINTER(dbl)]
         L.hfg_df_rhf.argtypes = [C.POINTER(BasisView), C.POINTER(BasisView), vp, i32, i32, C.POINTER(DfhfOptions),
                                  C.POINTER(DfhfResult), vp, vp, vp, vp]
+        L.hfg_rhf_gradient.argtypes = [vp, vp, i32, vp, vp, vp]
+        L.hfg_graph_launches.argtypes = [vp, C.POINTER(C.c_int64)]
         L.hfg_cd_build.argtypes = [C.POINTER(BasisView), dbl, dbl, C.POINTER(vp), C.POINTER(CdInfo), vp, i32]
         L.hfg_cd_rhf.argtypes = [C.POINTER(BasisView), vp, i32, i32, dbl, dbl, C.POINTER(DfhfOptions),
                                  C.POINTER(DfhfResult), C.POINTER(CdInfo), vp, vp]
@@ -457,6 +459,22 @@ class Context:
         _check(lib().hfg_contract_jk(self.h, _p(p), _p(j), _p(k)))
         return j, k
 
+    def graph_launches(self):
+        """Fock builds that replayed the captured HIP graph of their launches."""
+        v = C.c_int64()
+        _check(lib().hfg_graph_launches(self.h, C.byref(v)))
+        return v.value
+
+    def rhf_gradient(self, zxyz, p, w):
+        """Nuclear gradient (natom x 3, Eh/bohr) for density p and energy-weighted
+        density w: compute_gradient_from_density (src/gradient.rs:287-512)."""
+        zxyz = _atoms(zxyz)
+        p = _square(p, self.n, "density")
+        w = _square(w, self.n, "energy-weighted density")
+        g = np.zeros((len(zxyz), 3))
+        _check(lib().hfg_rhf_gradient(self.h, _p(zxyz), len(zxyz), _p(p), _p(w), _p(g)))
+        return g
+
     def contract_jk_multi(self, total, spins):
         """J of the total density and K of each spin density (contract_jk(total,
         spins), src/scf.rs:337-364).  A spin density that is the total object
@@ -566,6 +584,18 @@ DF_PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
 CD_HF_FACTOR_BUDGET = 512 * 1024 * 1024  # bytes (src/hf_variants/cholesky_df.rs:22)
+
+
+def rhf_gradient(basis, zxyz, density, fock):
+    """compute_rhf_gradient (src/gradient.rs:272-285) from a converged RHF
+    density and Fock matrix: W = 2 C_occ eps C_occ^T = 1/2 P F P."""
+    p = np.asarray(density, dtype=np.float64)
+    w = 0.5 * p @ np.asarray(fock, dtype=np.float64) @ p
+    ctx = Context(basis)
+    try:
+        return ctx.rhf_gradient(zxyz, p, w)
+    finally:
+        ctx.close()
 
 
 def cholesky(basis, threshold, budget=CD_HF_FACTOR_BUDGET):

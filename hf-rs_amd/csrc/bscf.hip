@@ -34,6 +34,8 @@
 #include <thread>
 #include <atomic>
 #include <array>
+#include <condition_variable>
+#include <mutex>
 
 #include "../../include/hfg.h"
 #include "bscf_layout.h"
@@ -416,8 +418,10 @@ struct Request {
 };
 
 // FREE: empty; ITER: iterating; PRE: one more build after max_iter; FINAL:
-// the final build; NEXT: finished a geometry step, the next one's set-up is due.
-enum Phase { FREE = 0, ITER, PRE, FINAL, NEXT };
+// the final build; NEXT: finished a geometry step, the next one's set-up is
+// due; SETUP: a worker is setting the slot up (the main loop leaves it alone
+// until it collects the set-up).
+enum Phase { FREE = 0, ITER, PRE, FINAL, NEXT, SETUP };
 
 struct Slot {
   Phase phase = FREE;
@@ -498,6 +502,23 @@ struct Server {
   std::unique_ptr<Blas> blas;
   std::vector<Worker> workers;
   size_t budget = 0, resident = 0;
+  // Admission pool: slot set-ups run on the worker threads while the slots
+  // already set up iterate; the main loop collects the finished ones at the
+  // start of each iteration.
+  struct Setup {
+    int si, ri, step, result;
+  };
+  std::mutex pool_mu;
+  std::condition_variable pool_cv, done_cv;
+  std::deque<Setup> pool_jobs;
+  std::vector<Setup> pool_done;
+  int pool_pending = 0;  // submitted and not yet collected (main thread only)
+  bool pool_stop = false;
+  void pool_work(Worker& w);
+  void submit(const std::vector<std::array<int, 3>>& todo);
+  // Moves finished set-ups into the iteration (or frees their slots); with
+  // wait, blocks until at least one has finished.
+  void collect(bool wait);
   // statistics
   int64_t fused_builds = 0, iterations = 0;
   double seconds = 0, seconds_setup = 0, seconds_sync = 0;
@@ -510,7 +531,7 @@ struct Server {
   // 0: failed (recorded in the request), 1: ready, 2: ready but the density
   // must come from the core Hamiltonian (no SAD guess)
   int admit(int slot, int req, int step, const double* guess, Worker& w);
-  void admit_all(const std::vector<std::array<int, 3>>& todo);
+
   void finish_step(int slot);
   void density(const std::vector<int>& L);
   void upload_lists(const std::vector<int>& L, int which);
@@ -637,7 +658,6 @@ int Server::admit(int si, int ri, int step, const double* guess, Worker& w) {
     s.mol.tab = fleet->d_tabs.as<MolTab>() + si;
     h_mols[si] = s.mol;
     HIP_CHECK(hipMemcpyAsync(d_mols.as<BMol>() + si, &h_mols[si], sizeof(BMol), hipMemcpyHostToDevice, stq));
-    s.phase = ITER;
     s.req = ri;
     s.step = step;
     s.it = 0;
@@ -665,42 +685,73 @@ int Server::admit(int si, int ri, int step, const double* guess, Worker& w) {
   R.finished = now();
   if (s.eng) fleet->detach(si);
   s.eng.reset();
-  s.phase = FREE;
   s.req = -1;
   return 0;
 }
 
-// Sets up the listed (slot, request, step) admissions on the workers in
-// parallel; then, on the fleet's stream, the core-Hamiltonian densities of
-// those without a SAD guess.
-void Server::admit_all(const std::vector<std::array<int, 3>>& todo) {
-  if (todo.empty()) return;
-  const double ts = now();
-  dirty = true;
-  std::vector<int> res(todo.size(), 0);
-  std::atomic<size_t> next{0};
-  auto work = [&](Worker& w) {
-    for (;;) {
-      const size_t i = next++;
-      if (i >= todo.size()) break;
-      const int si = todo[i][0], ri = todo[i][1], step = todo[i][2];
-      Slot& s = slots[si];
-      const bool warm = !s.guess.empty() && s.guess.size() == reqs[ri].steps[step].atom.size() * reqs[ri].steps[step].atom.size();
-      res[i] = admit(si, ri, step, warm ? s.guess.data() : nullptr, w);
+// Set-ups run on the worker threads, each with its own stream and BLAS
+// handle; a finished one waits in pool_done for the main loop.
+void Server::pool_work(Worker& w) {
+  for (;;) {
+    Setup job;
+    {
+      std::unique_lock<std::mutex> lk(pool_mu);
+      pool_cv.wait(lk, [&] { return pool_stop || !pool_jobs.empty(); });
+      if (pool_jobs.empty()) return;  // stopping
+      job = pool_jobs.front();
+      pool_jobs.pop_front();
     }
-  };
-  const size_t nt = std::min(workers.size(), todo.size());
-  std::vector<std::thread> th;
-  for (size_t t = 1; t < nt; ++t) th.emplace_back(work, std::ref(workers[t]));
-  work(workers[0]);
-  for (auto& t : th) t.join();
+    Slot& s = slots[job.si];
+    const bool warm = !s.guess.empty() &&
+                      s.guess.size() == reqs[job.ri].steps[job.step].atom.size() * reqs[job.ri].steps[job.step].atom.size();
+    job.result = admit(job.si, job.ri, job.step, warm ? s.guess.data() : nullptr, w);
+    {
+      std::lock_guard<std::mutex> lk(pool_mu);
+      pool_done.push_back(job);
+    }
+    done_cv.notify_all();
+  }
+}
+
+void Server::submit(const std::vector<std::array<int, 3>>& todo) {
+  if (todo.empty()) return;
+  {
+    std::lock_guard<std::mutex> lk(pool_mu);
+    for (const auto& t : todo) {
+      slots[t[0]].phase = SETUP;
+      pool_jobs.push_back(Setup{t[0], t[1], t[2], 0});
+    }
+  }
+  pool_pending += (int)todo.size();
+  pool_cv.notify_all();
+}
+
+void Server::collect(bool wait) {
+  std::vector<Setup> got;
+  {
+    std::unique_lock<std::mutex> lk(pool_mu);
+    if (wait && pool_done.empty() && pool_pending > 0) {
+      const double ts = now();
+      done_cv.wait(lk, [&] { return !pool_done.empty(); });
+      seconds_setup += now() - ts;
+    }
+    got.swap(pool_done);
+  }
+  if (got.empty()) return;
+  pool_pending -= (int)got.size();
+  dirty = true;
   std::vector<int> core;
-  for (size_t i = 0; i < todo.size(); ++i) {
-    Slot& s = slots[todo[i][0]];
+  for (const Setup& j : got) {
+    Slot& s = slots[j.si];
     s.guess.clear();
     s.guess.shrink_to_fit();
-    if (res[i] == 0) resident -= std::min(resident, footprint((int)reqs[todo[i][1]].steps[todo[i][2]].atom.size()));
-    if (res[i] == 2) core.push_back(todo[i][0]);
+    if (j.result == 0) {
+      resident -= std::min(resident, footprint((int)reqs[j.ri].steps[j.step].atom.size()));
+      s.phase = FREE;
+      continue;
+    }
+    s.phase = ITER;
+    if (j.result == 2) core.push_back(j.si);
   }
   if (!core.empty()) {
     // density from H before the first build (src/scf.rs:452-471)
@@ -718,7 +769,6 @@ void Server::admit_all(const std::vector<std::array<int, 3>>& todo) {
     }
     HIP_CHECK(hipStreamSynchronize(fleet->stream));
   }
-  seconds_setup += now() - ts;
 }
 
 // Density of the listed slots from their Fu (k_b_store wrote it): Fo = X^T Fu
@@ -858,6 +908,25 @@ void Server::run() {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival < reqs[b].arrival; });
   queue.assign(order.begin(), order.end());
   std::vector<int> prev_active;
+  // the admission pool for this run; stopped and joined on every way out
+  pool_stop = false;
+  pool_pending = 0;
+  std::vector<std::thread> pool;
+  for (Worker& w : workers) pool.emplace_back([this, &w] { pool_work(w); });
+  struct PoolJoin {
+    Server* S;
+    std::vector<std::thread>& th;
+    ~PoolJoin() {
+      {
+        std::lock_guard<std::mutex> lk(S->pool_mu);
+        S->pool_stop = true;
+        S->pool_jobs.clear();
+      }
+      S->pool_cv.notify_all();
+      for (auto& t : th) t.join();
+      S->pool_done.clear();
+    }
+  } pool_join{this, pool};
   for (;;) {
     // admission: the next geometry steps of requests in flight, then arrived
     // requests into free slots while the footprint fits; all set up at once
@@ -865,22 +934,34 @@ void Server::run() {
     for (int si = 0; si < opt.capacity; ++si)
       if (slots[si].phase == NEXT) todo.push_back({si, slots[si].req, slots[si].step + 1});
     const double tnow = now();
-    for (int si = 0; si < opt.capacity && !queue.empty(); ++si) {
+    // Requests that have arrived are admitted largest first: a large molecule
+    // admitted late would finish last (longest-processing-time order).
+    size_t arrived = 0;
+    while (arrived < queue.size() && reqs[queue[arrived]].arrival <= tnow) ++arrived;
+    std::stable_sort(queue.begin(), queue.begin() + arrived, [&](int a, int b) {
+      return reqs[a].steps[0].atom.size() > reqs[b].steps[0].atom.size();
+    });
+    for (int si = 0; si < opt.capacity && arrived > 0; ++si) {
       if (slots[si].phase != FREE) continue;
       const int ri = queue.front();
-      if (reqs[ri].arrival > tnow) break;
       const size_t fp = footprint((int)reqs[ri].steps[0].atom.size());
       if (resident > 0 && resident + fp > budget) break;
       queue.pop_front();
+      --arrived;
       resident += fp;
-      slots[si].phase = NEXT;  // reserved until its set-up
+      slots[si].phase = SETUP;  // reserved until its set-up is collected
       todo.push_back({si, ri, 0});
     }
-    admit_all(todo);
+    submit(todo);
+    collect(false);
     std::vector<int> act;
     for (int si = 0; si < opt.capacity; ++si)
-      if (slots[si].phase != FREE) act.push_back(si);
+      if (slots[si].phase != FREE && slots[si].phase != SETUP) act.push_back(si);
     if (act.empty()) {
+      if (pool_pending > 0) {
+        collect(true);  // nothing can iterate until a set-up finishes
+        continue;
+      }
       if (queue.empty()) break;
       // idle until the next arrival
       const double wait = reqs[queue.front()].arrival - now();
@@ -918,7 +999,9 @@ void Server::run() {
       prev_active = act;
       dirty = false;
     }
-    bool all_full = true;
+    // the tuner times full builds of a settled set only (a growing set
+    // would favour the later candidates)
+    bool all_full = pool_pending == 0;
     for (int si : act) all_full = all_full && h_ctrl[si].build == 0;
     fleet->build(all_full);
     ++fused_builds;

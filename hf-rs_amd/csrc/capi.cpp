@@ -45,6 +45,8 @@ static int nc_of(const Engine& e, int pair) {
 namespace hfg {
 void eri_mo(Engine& e, const double* C, int nmo, double* out);
 void jk_device_noalloc(Engine& e, const double* dP, double* dJ, double* dK);
+void rhf_gradient(Engine& e, const std::vector<std::array<double, 4>>& atoms, const double* P, const double* W,
+                  double* grad);
 int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int charge, const hfg_scf_options* opt,
             hfg_scf_result* res, double* p_out, double* f_out, double* eps_out, const double* p_guess);
 }
@@ -233,6 +235,21 @@ int hfg_contract_jk_device_noalloc(hfg_ctx* ctx, const double* d_p, double* d_j,
   })
 }
 
+int hfg_rhf_gradient(hfg_ctx* ctx, const double* zxyz, int natom, const double* p, const double* w, double* grad) {
+  if (!ctx || !zxyz || !p || !w || !grad) return set_err(HFG_INVALID_INPUT, "null argument");
+  if (natom <= 0) return set_err(HFG_INVALID_INPUT, "gradient needs at least one atom");
+  GUARD({
+    std::vector<std::array<double, 4>> atoms(natom);
+    for (int a = 0; a < natom; ++a)
+      for (int k = 0; k < 4; ++k) {
+        atoms[a][k] = zxyz[4 * a + k];
+        if (!std::isfinite(atoms[a][k])) return set_err(HFG_INVALID_INPUT, "non-finite geometry");
+      }
+    rhf_gradient(*ctx->e, atoms, p, w, grad);
+    return HFG_OK;
+  })
+}
+
 int hfg_contract_jk(hfg_ctx* ctx, const double* p, double* j, double* k) {
   if (!ctx || !p || !j || !k) return set_err(HFG_INVALID_INPUT, "null argument");
   GUARD({
@@ -338,6 +355,12 @@ int hfg_profile(hfg_ctx* ctx, int enable) {
     ctx->e->set_profile(enable != 0);
     return HFG_OK;
   })
+}
+
+int hfg_graph_launches(const hfg_ctx* ctx, int64_t* launches) {
+  if (!ctx || !launches) return set_err(HFG_INVALID_INPUT, "null argument");
+  *launches = ctx->e->graph_launches;
+  return HFG_OK;
 }
 
 int hfg_class_stats(hfg_ctx* ctx, int cls, int32_t* l4, double* flops, int64_t* quartets, double* ms,

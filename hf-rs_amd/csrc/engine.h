@@ -217,6 +217,23 @@ struct Engine {
   // The engine's launch resources (job counters, fork/join events, Boys table,
   // threshold, profiling events) for launch_jk_set.
   JKLaunch launch_ctx();
+  // HIP graphs of jk_device's launches after its table update (accumulator
+  // fills, job-counter clear, the class launches on both streams and their
+  // fork/join, the symmetrisation), one per set of output pointers (J, K,
+  // H, F), captured on the second build with those pointers and replayed from
+  // then on: every launch argument of those kernels is fixed per engine and
+  // output set (the density and threshold of a build reach the kernels
+  // through the table).  Engines with their own streams only; HFG_GRAPH=0
+  // turns them off.
+  bool use_graph = true;
+  struct JKGraph {
+    const void* key[4];
+    int seen = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<JKGraph> graphs;
+  int64_t graph_launches = 0;
   void sync();
   void set_profile(bool on);
 };

@@ -204,6 +204,8 @@ Engine::Engine(const hfg_basis_view& b, hipStream_t shared) {
     HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     if (const char* env = std::getenv("HFG_SIDE_BLOCKS")) side_blocks = std::max(0, std::atoi(env));
   }
+  if (const char* env = std::getenv("HFG_GRAPH")) use_graph = std::atoi(env) != 0;
+  use_graph = use_graph && own_streams;
   {
     int cus = 0;
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -268,6 +270,10 @@ Engine::Engine(const hfg_basis_view& b, hipStream_t shared) {
 }
 
 Engine::~Engine() {
+  for (auto& g : graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
   for (auto e : events) (void)hipEventDestroy(e);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
@@ -554,13 +560,50 @@ void Engine::jk_device(const double* dP, double* dJ, double* dK, const double* d
   const size_t nn = (size_t)n * n;
   const int tb = 256;
   const size_t nacc = nn * NXCC_SLOTS;
-  hipLaunchKernelGGL(k_fill, dim3((nacc + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(), nacc, 0.0);
-  hipLaunchKernelGGL(k_fill, dim3((nacc + tb - 1) / tb), dim3(tb), 0, stream, d_Kacc.as<double>(), nacc, 0.0);
   hipLaunchKernelGGL(k_tab_set, dim3(1), dim3(64), 0, stream, d_tab.as<MolTab>(), dP,
                      dmax > 0.0 && dmax < 1.0 ? screen / dmax : screen);
-  launch_jk_set(launch_ctx(), d_tab.as<MolTab>(), dir, d_dir.as<int>(), stream, side, side_blocks);
-  hipLaunchKernelGGL(k_symmetrize, dim3((nn + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(),
-                     d_Kacc.as<double>(), dJ, dK, n, dH, dF);
+  auto launches = [&] {
+    hipLaunchKernelGGL(k_fill, dim3((nacc + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(), nacc, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3((nacc + tb - 1) / tb), dim3(tb), 0, stream, d_Kacc.as<double>(), nacc, 0.0);
+    launch_jk_set(launch_ctx(), d_tab.as<MolTab>(), dir, d_dir.as<int>(), stream, side, side_blocks);
+    hipLaunchKernelGGL(k_symmetrize, dim3((nn + tb - 1) / tb), dim3(tb), 0, stream, d_Jacc.as<double>(),
+                       d_Kacc.as<double>(), dJ, dK, n, dH, dF);
+  };
+  JKGraph* g = nullptr;
+  if (use_graph && !profile) {
+    const void* key[4] = {dJ, dK, dH, dF};
+    for (auto& x : graphs)
+      if (std::equal(key, key + 4, x.key)) g = &x;
+    if (!g && graphs.size() < 8) {
+      graphs.emplace_back();
+      g = &graphs.back();
+      std::copy(key, key + 4, g->key);
+    }
+  }
+  if (g && g->exec) {
+    HIP_CHECK(hipGraphLaunch(g->exec, stream));
+    ++graph_launches;
+  } else if (g && g->seen >= 1) {
+    // second build with these outputs: capture it (the first ran eagerly, so
+    // the launch helpers' one-time attribute and occupancy queries are done)
+    HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      launches();
+    } catch (...) {
+      hipGraph_t partial = nullptr;
+      (void)hipStreamEndCapture(stream, &partial);  // leave the stream usable
+      if (partial) (void)hipGraphDestroy(partial);
+      use_graph = false;
+      throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(stream, &g->graph));
+    HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphLaunch(g->exec, stream));
+    ++graph_launches;
+  } else {
+    launches();
+    if (g) ++g->seen;
+  }
   HIP_CHECK(hipGetLastError());
   if (profile) {
     sync();

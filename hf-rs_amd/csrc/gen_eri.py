@@ -498,6 +498,14 @@ def classes():
     return out
 
 
+def grad_classes():
+    """Block classes the nuclear gradient needs beyond classes(): a d shell
+    raised to f in one pair (fp, fd, in kernel orientation) against every
+    orbital pair; the other raised and lowered pairs fall in classes()."""
+    orbital = [c for c in PAIR_CLASSES if c[0] <= LMAX]
+    return [(3, b, lc, ld) for b in (1, 2) for (lc, ld) in orbital]
+
+
 def main(outdir):
     os.makedirs(outdir, exist_ok=True)
     parts = ["// Generated by gen_eri.py -- do not edit.\n"]
@@ -517,6 +525,13 @@ def main(outdir):
             if has_unit_variant(c):
                 assert slice_count(c) == 1
                 f.write(gen_class_gc(*c, S=0, V=1))
+    with open(os.path.join(outdir, "eri_grad.inc"), "w") as f:
+        f.write("// Generated by gen_eri.py -- do not edit.  Block classes of the gradient (grad.hip).\n")
+        for c in grad_classes():
+            src, fl = gen_class(*c)
+            f.write(src)
+            report.append("%s %d" % ("".join(map(str, c)), fl))
+        f.write("#define HFG_FOR_EACH_GRAD_CLASS(X) " + " ".join("X(%d, %d, %d, %d)" % c for c in grad_classes()) + "\n")
     with open(os.path.join(outdir, "eri_classes.txt"), "w") as f:
         f.write("\n".join(report) + "\n")
     # class list for the C++ side, in quartet_class() order

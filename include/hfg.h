@@ -103,6 +103,16 @@ int hfg_contract_jk_device(hfg_ctx* ctx, const double* d_p, double* d_j, double*
  * shares the array: calls that overlap on one GPU race. */
 int hfg_contract_jk_device_noalloc(hfg_ctx* ctx, const double* d_p, double* d_j, double* d_k);
 
+/* Analytic RHF nuclear gradient, compute_gradient_from_density
+ * (src/gradient.rs:287-512): grad (natom x 3, Eh/bohr) of the context's
+ * molecule for density p and energy-weighted density w (n x n each; for a
+ * converged RHF w = 2 C_occ eps C_occ^T = 1/2 p F p); zxyz holds natom rows of
+ * (Z, x, y, z) in bohr, atom indices as in the basis.  The two-electron
+ * derivative integrals and their contraction run on the device; the
+ * one-electron derivative terms and the nuclear repulsion on the host. */
+int hfg_rhf_gradient(hfg_ctx* ctx, const double* zxyz, int natom, const double* p, const double* w,
+                     double* grad);
+
 /* Resident-buffer form used by throughput measurements: the density is
  * uploaded once, J and K stay in HBM until fetched.  (Same arithmetic as
  * hfg_contract_jk; no reference counterpart: hf-rs has no device memory.) */
@@ -119,6 +129,10 @@ int hfg_get_jk(hfg_ctx* ctx, double* j, double* k);
  * class's surviving quartets of Ka*Kb*Kc*Kd*flops(la,lb,lc,ld)), quartets per
  * launch, total measured milliseconds and number of launches. */
 int hfg_profile(hfg_ctx* ctx, int enable);
+/* Fock builds of the context that replayed a captured HIP graph of their
+ * launches (no reference counterpart; HFG_GRAPH=0 at context creation turns
+ * the capture off). */
+int hfg_graph_launches(const hfg_ctx* ctx, int64_t* launches);
 int hfg_class_stats(hfg_ctx* ctx, int cls, int32_t* l4, double* flops_per_launch, int64_t* quartets,
                     double* ms_total, int64_t* launches);
 /* Algorithmic FLOPs of one Fock build (sum over classes) and per primitive
@@ -300,6 +314,10 @@ typedef struct {
 } hfg_request_result;
 typedef struct {
   int64_t fused_builds, iterations;
+  /* seconds: the run; seconds_setup: of it, waiting for slot set-ups (engine,
+   * one-electron matrices, orthogonaliser, guess: they run on worker threads
+   * while the slots already set up iterate) with no slot able to iterate;
+   * seconds_sync: waiting for the per-iteration results */
   double seconds, seconds_setup, seconds_sync, hbm_budget_bytes;
   /* the class-launch configuration the autotuner chose (-1: still tuning)
    * out of jk_candidates */

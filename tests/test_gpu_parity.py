@@ -363,3 +363,28 @@ def test_global_atomic_single_stream_path_matches_oracle(hf, monkeypatch):
         ej, ek = np.abs(j - jr).max(), np.abs(k - kr).max()
         print("global-atomic %s %s n=%d  J %.2e  K %.2e" % (mol.name, basis, ctx.n, ej, ek))
         assert ej <= 1e-10 and ek <= 1e-10
+
+
+def test_fock_build_graph_replays_match_eager_builds(hf, monkeypatch):
+    """The one-molecule Fock build captured as a HIP graph on its second call
+    and replayed after: every replay (with eager work and synchronisations of
+    the program in between) against the eager build of the same density, for
+    five densities; (H2O)2/cc-pVDZ with both streams."""
+    mol = M.water_cluster((2, 1, 1), seed=2)
+    b = hf.Basis.from_g94(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz)
+    monkeypatch.setenv("HFG_GRAPH", "0")
+    eager = hf.Context(b)
+    monkeypatch.setenv("HFG_GRAPH", "1")
+    graph = hf.Context(b)
+    monkeypatch.delenv("HFG_GRAPH")
+    for i in range(5):
+        p = sym_density(b.n, 100 + i)
+        j0, k0 = eager.contract_jk(p)
+        j1, k1 = graph.contract_jk(p)
+        jm, km = graph.contract_jk_multi(p, [p])  # the multi-density entry, through the same build
+        assert np.abs(j1 - j0).max() < 1e-12 and np.abs(k1 - k0).max() < 1e-12, i
+        assert np.abs(jm - j0).max() < 1e-12 and np.abs(km[0] - k0).max() < 1e-12, i
+    assert eager.graph_launches() == 0
+    assert graph.graph_launches() >= 4
+    eager.close()
+    graph.close()
