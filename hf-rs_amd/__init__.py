@@ -59,7 +59,7 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_batch_get_jk", "hfg_batch_jk_each", "hfg_df_jk_ex", "hfg_contract_jk_multi", "hfg_server_create",
             "hfg_server_destroy", "hfg_server_submit", "hfg_server_run", "hfg_server_result", "hfg_server_stats",
             "hfg_scf_rhf_guess", "hfg_eri_mo", "hfg_batch_profile", "hfg_batch_class_stats", "hfg_cd_build",
-            "hfg_cd_rhf", "hfg_rhf_gradient", "hfg_graph_launches"]
+            "hfg_cd_rhf", "hfg_rhf_gradient", "hfg_graph_launches", "hfg_one_electron", "hfg_sad_density"]
 
 
 class ServerOptions(C.Structure):
@@ -144,6 +144,8 @@ def lib():
                                  C.POINTER(DfhfResult), vp, vp, vp, vp]
         L.hfg_rhf_gradient.argtypes = [vp, vp, i32, vp, vp, vp]
         L.hfg_graph_launches.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.hfg_one_electron.argtypes = [vp, vp, i32, vp, vp, vp]
+        L.hfg_sad_density.argtypes = [vp, vp, i32, i32, vp]
         L.hfg_cd_build.argtypes = [C.POINTER(BasisView), dbl, dbl, C.POINTER(vp), C.POINTER(CdInfo), vp, i32]
         L.hfg_cd_rhf.argtypes = [C.POINTER(BasisView), vp, i32, i32, dbl, dbl, C.POINTER(DfhfOptions),
                                  C.POINTER(DfhfResult), C.POINTER(CdInfo), vp, vp]
@@ -464,6 +466,23 @@ class Context:
         v = C.c_int64()
         _check(lib().hfg_graph_launches(self.h, C.byref(v)))
         return v.value
+
+    def one_electron(self, zxyz):
+        """S, T, V (n x n each) of the context's basis for the nuclei zxyz,
+        computed on the device: compute_one_electron_integrals
+        (src/scf.rs:246-271)."""
+        zxyz = _atoms(zxyz)
+        s, t, v = (np.zeros((self.n, self.n)) for _ in range(3))
+        _check(lib().hfg_one_electron(self.h, _p(zxyz), len(zxyz), _p(s), _p(t), _p(v)))
+        return s, t, v
+
+    def sad_density(self, zxyz, charge=0):
+        """The SAD initial density (compute_sad_density, src/sad.rs:217-259),
+        superposed and scaled on the device."""
+        zxyz = _atoms(zxyz)
+        p = np.zeros((self.n, self.n))
+        _check(lib().hfg_sad_density(self.h, _p(zxyz), len(zxyz), int(charge), _p(p)))
+        return p
 
     def rhf_gradient(self, zxyz, p, w):
         """Nuclear gradient (natom x 3, Eh/bohr) for density p and energy-weighted

@@ -601,27 +601,20 @@ int Server::admit(int si, int ri, int step, const double* guess, Worker& w) {
   try {
     const RhfInputs in = rhf_inputs(bv, st.zxyz.data(), R.natom, R.charge);
     const int n = bv.n_functions;
-    HMat S, T, V;
-    const std::vector<Shell> sh1 = group_shells(bv);
-    auto one_e = std::async(std::launch::async, [&] { one_electron(sh1, n, in.atoms, S, T, V); });
     if (s.eng) fleet->detach(si);
     s.eng.reset();
     hipStream_t stq = w.stream;
-    try {
-      s.eng.reset(new Engine(bv, stq));
-    } catch (...) {
-      one_e.wait();
-      throw;
-    }
-    one_e.get();
+    s.eng.reset(new Engine(bv, stq));
     s.n = n;
     s.nocc = in.nocc;
     s.enuc = in.enuc;
     const size_t nn = (size_t)n * n;
-    // S on the device for the orthogonaliser (in the slot's buffer below)
-    Dev dS;
+    // S and H = T + V on the device, copied into the slot's buffer below once
+    // the orthogonaliser has fixed its size
+    Dev dS, dH;
     dS.alloc(nn);
-    upload(dS.p, S.data(), nn, stq);
+    dH.alloc(nn);
+    one_electron_device(s.eng->shells, n, in.atoms, dS.p, nullptr, nullptr, dH.p, stq);
     HMat X;
     s.m = orthogonalizer(*w.blas, stq, s.eng->shells, n, dS.p, R.opt.spherical != 0, X);
     if (s.nocc > s.m) throw HfgError{HFG_INVALID_INPUT, "more occupied orbitals than retained overlap rank"};
@@ -632,10 +625,8 @@ int Server::admit(int si, int ri, int step, const double* guess, Worker& w) {
       s.cap = need;
     }
     layout(s);
-    HMat H(nn);
-    for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
-    upload(s.mol.S, S.data(), nn, stq);
-    upload(s.mol.H, H.data(), nn, stq);
+    HIP_CHECK(hipMemcpyAsync(s.mol.S, dS.p, nn * 8, hipMemcpyDeviceToDevice, stq));
+    HIP_CHECK(hipMemcpyAsync(s.mol.H, dH.p, nn * 8, hipMemcpyDeviceToDevice, stq));
     upload(s.mol.X, X.data(), (size_t)n * s.m, stq);
     // DIIS histories start at zero (scf.cpp clears them the same way)
     for (double* h : {s.mol.cF, s.mol.aF, s.mol.aP})
@@ -646,10 +637,8 @@ int Server::admit(int si, int ri, int step, const double* guess, Worker& w) {
     if (guess) {
       upload(s.mol.P, guess, nn, stq);
     } else {
-      HMat P0;
       try {
-        P0 = sad_guess(s.eng->shells, n, in.atoms, R.charge, S);
-        upload(s.mol.P, P0.data(), nn, stq);
+        sad_guess_device(s.eng->shells, n, in.atoms, R.charge, s.mol.S, s.mol.P, *w.blas, stq);
       } catch (HfgError&) {
         core_guess = true;  // src/scf.rs:452-471: core Hamiltonian when SAD is unavailable
       }

@@ -9,6 +9,7 @@
 #include "../../include/hfg.h"
 #include "engine.h"
 #include "orient.h"
+#include "rhf.h"
 
 using namespace hfg;
 
@@ -246,6 +247,55 @@ int hfg_rhf_gradient(hfg_ctx* ctx, const double* zxyz, int natom, const double* 
         if (!std::isfinite(atoms[a][k])) return set_err(HFG_INVALID_INPUT, "non-finite geometry");
       }
     rhf_gradient(*ctx->e, atoms, p, w, grad);
+    return HFG_OK;
+  })
+}
+
+// Nuclei (Z, x, y, z) of a call, checked finite.
+static std::vector<std::array<double, 4>> nuclei(const double* zxyz, int natom) {
+  std::vector<std::array<double, 4>> atoms(natom);
+  for (int a = 0; a < natom; ++a)
+    for (int k = 0; k < 4; ++k) {
+      atoms[a][k] = zxyz[4 * a + k];
+      if (!std::isfinite(atoms[a][k])) throw HfgError{HFG_INVALID_INPUT, "non-finite geometry"};
+    }
+  return atoms;
+}
+
+int hfg_one_electron(hfg_ctx* ctx, const double* zxyz, int natom, double* s, double* t, double* v) {
+  if (!ctx || !zxyz || !s || !t || !v) return set_err(HFG_INVALID_INPUT, "null argument");
+  if (natom <= 0) return set_err(HFG_INVALID_INPUT, "one-electron matrices need at least one nucleus");
+  GUARD({
+    Engine& e = *ctx->e;
+    const auto atoms = nuclei(zxyz, natom);
+    const size_t nn = (size_t)e.n * e.n;
+    DevBuf d;
+    d.alloc(3 * nn * sizeof(double));
+    double* dS = d.as<double>();
+    one_electron_device(e.shells, e.n, atoms, dS, dS + nn, dS + 2 * nn, nullptr, e.stream);
+    HIP_CHECK(hipMemcpyAsync(s, dS, nn * 8, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(t, dS + nn, nn * 8, hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipMemcpyAsync(v, dS + 2 * nn, nn * 8, hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+    return HFG_OK;
+  })
+}
+
+int hfg_sad_density(hfg_ctx* ctx, const double* zxyz, int natom, int charge, double* p) {
+  if (!ctx || !zxyz || !p) return set_err(HFG_INVALID_INPUT, "null argument");
+  if (natom <= 0) return set_err(HFG_INVALID_INPUT, "SAD needs at least one atom");
+  GUARD({
+    Engine& e = *ctx->e;
+    const auto atoms = nuclei(zxyz, natom);
+    const size_t nn = (size_t)e.n * e.n;
+    DevBuf d;
+    d.alloc(2 * nn * sizeof(double));
+    double* dS = d.as<double>();
+    one_electron_device(e.shells, e.n, atoms, dS, nullptr, nullptr, nullptr, e.stream);
+    Blas bl(e.stream);
+    sad_guess_device(e.shells, e.n, atoms, charge, dS, dS + nn, bl, e.stream);
+    HIP_CHECK(hipMemcpyAsync(p, dS + nn, nn * 8, hipMemcpyDeviceToHost, e.stream));
+    e.sync();
     return HFG_OK;
   })
 }

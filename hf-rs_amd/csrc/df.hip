@@ -29,6 +29,7 @@
 #include "cholesky.h"
 #include "engine.h"
 #include "la_device.h"
+#include "rhf.h"
 #include "scf_host.h"
 
 namespace hfg {
@@ -604,21 +605,11 @@ static int df_rhf(const hfg_basis_view* basis, const std::function<DfEngine*()>&
   }
   if (!(el > 0) || std::fabs(el - std::round(el)) > 1e-8 || ((long)std::lround(el)) % 2)
     throw HfgError{HFG_INVALID_INPUT, "DF-RHF requires a positive even electron count"};
-  // One-electron matrices (host) on a second thread during the DF build.
-  std::vector<double> S, T, V;
   const std::vector<Shell> sh1 = group_shells(*basis);
-  const int n1 = basis->n_functions;
-  auto one_e = std::async(std::launch::async, [&] { one_electron(sh1, n1, atoms, S, T, V); });
-  std::unique_ptr<DfEngine> df_p;
-  try {
-    df_p.reset(make_engine());
-  } catch (...) {
-    one_e.wait();
-    throw;
-  }
+  std::unique_ptr<DfEngine> df_p(make_engine());
   DfEngine& df = *df_p;
-  one_e.get();
   const int n = df.n, no = (int)std::lround(el) / 2;
+  if (n != basis->n_functions) throw HfgError{HFG_INVALID_INPUT, "DF tensor and basis differ in size"};
   hipStream_t st = df.stream;
   Blas& bl = *df.bl;
   const size_t nn = (size_t)n * n;
@@ -628,12 +619,9 @@ static int df_rhf(const hfg_basis_view* basis, const std::function<DfEngine*()>&
       const double dx = atoms[i][1] - atoms[j][1], dy = atoms[i][2] - atoms[j][2], dz = atoms[i][3] - atoms[j][3];
       enuc += atoms[i][0] * atoms[j][0] / std::sqrt(dx * dx + dy * dy + dz * dz);
     }
-  std::vector<double> H(nn);
-  for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
   Dev dS, dH, dX, dTmp, dTmp2, dF, dP, dPn, dJ, dK, dC, dCo, dCn, dW, dErr, dFu;
   for (Dev* d : {&dS, &dH, &dTmp, &dTmp2, &dF, &dP, &dPn, &dJ, &dK, &dErr, &dFu}) d->alloc(nn);
-  upload(dS.p, S.data(), nn, st);
-  upload(dH.p, H.data(), nn, st);
+  one_electron_device(sh1, n, atoms, dS.p, nullptr, nullptr, dH.p, st);  // S, H = T + V
   // canonical orthogonaliser of S (cutoff 1e-8, Cartesian)
   Eigh eig;
   int m;

@@ -21,7 +21,7 @@ namespace hfg {
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void k_block(BlockArgs a) {
   constexpr int NT = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
-  __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
+  __shared__ double boys[BoysSlice<false>::size];
   load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.ntask) return;

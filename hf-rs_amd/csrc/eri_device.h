@@ -65,23 +65,62 @@ constexpr double BOYS_INV_STEP = 8.0;
 constexpr double BOYS_TMAX = 50.0;
 constexpr int BOYS_ROWS = 402;
 
-// Per-class slice of the table, as the kernels keep it in LDS: row r holds
-// exp(-T_r) followed by the Taylor coefficients F_{M+k}(T_r)/k!, k = 0..8
-// (BOYS_SLICE doubles), so that each Horner step is one FMA.
-constexpr int BOYS_SLICE = BOYS_TAYLOR + 1;
+// Per-class slice of the table, as the kernels keep it in LDS: rows of
+// exp(-T_r) followed by the Taylor coefficients F_{M+k}(T_r)/k!, so that each
+// Horner step is one FMA.  Two layouts, picked per kernel at compile time:
+//   full     every table row (T_r = r/8, 402 rows), nine terms, |dT| <= 1/16,
+//            remainder below (1/16)^9/9! = 4e-17 of F_M; 10 doubles a row, 32 KB;
+//   compact  every other row (T_r = r/4, 201 rows), eleven terms, |dT| <= 1/8,
+//            remainder below (1/8)^11/11! = 3e-18; 12 doubles a row, 19 KB.
+// The slice is the largest LDS item of a J/K workgroup: the compact one leaves
+// room for more workgroups per CU at two more FMAs per evaluation.
+template <bool C>
+struct BoysSlice {
+  static constexpr int taylor = C ? 11 : BOYS_TAYLOR;
+  static constexpr int cols = taylor + 1;
+  static constexpr int step = C ? 2 : 1;  // table rows per slice row
+  static constexpr int rows = C ? (BOYS_ROWS + 1) / 2 : BOYS_ROWS;
+  static constexpr int size = rows * cols;
+  static constexpr double inv_step = BOYS_INV_STEP / step;
+  static constexpr int nexp = C ? 12 : 10;  // terms of e^{-dT}: remainder < 2e-19
+};
+// Measured on the def2-SVP batch and on (H2O)16/cc-pVDZ: the one-member
+// (segmented-basis) variants of the classes above (ss|ss) run faster with the
+// compact layout (they are LDS-bound); the general-contraction kernels, which
+// evaluate more primitive quartets per digested block, with the full one.
+constexpr bool boys_compact(int V, int L) { return V == 1 && L > 0; }
 
-template <int M>
+template <int M, bool C = false>
 HFG_DEV void boys_slice_row(const double* __restrict__ tab, int r, double* __restrict__ row) {
-  row[0] = tab[r * BOYS_ROW];
+  using B = BoysSlice<C>;
+  static_assert(M + B::taylor <= BOYS_ORDERS, "Boys table holds too few orders for this slice");
+  const int g = B::step * r;  // table row of T = r / inv_step
+  row[0] = tab[g * BOYS_ROW];
   double inv_fact = 1.0;
-  for (int k = 0; k < BOYS_TAYLOR; ++k) {
+  for (int k = 0; k < B::taylor; ++k) {
     if (k > 0) inv_fact /= k;
-    row[1 + k] = tab[r * BOYS_ROW + 1 + M + k] * inv_fact;
+    row[1 + k] = tab[g * BOYS_ROW + 1 + M + k] * inv_fact;
   }
 }
 
-template <int M, bool SELECT = false>
+// 1/k!, k = 0..12: the series of e^{-dT} around a slice row.
+constexpr double BOYS_EXP_C[13] = {1.0,
+                                   1.0,
+                                   1.0 / 2,
+                                   1.0 / 6,
+                                   1.0 / 24,
+                                   1.0 / 120,
+                                   1.0 / 720,
+                                   1.0 / 5040,
+                                   1.0 / 40320,
+                                   1.0 / 362880,
+                                   1.0 / 3628800,
+                                   1.0 / 39916800,
+                                   1.0 / 479001600};
+
+template <int M, bool SELECT = false, bool C = false>
 HFG_DEV void boys_eval(double T, const double* __restrict__ slice, double* __restrict__ F) {
+  using B = BoysSlice<C>;
   if constexpr (SELECT && M <= 4) {
     // Both branches below, evaluated without control flow and selected per
     // lane: the loop body stays one basic block, so that the scheduler can
@@ -89,20 +128,18 @@ HFG_DEV void boys_eval(double T, const double* __restrict__ slice, double* __res
     // loop is unrolled; it costs registers elsewhere).
     const bool near = T < BOYS_TMAX;
     const double Tn = near ? T : 0.0;
-    const int r = (int)(Tn * BOYS_INV_STEP + 0.5);
-    const double x = r * (1.0 / BOYS_INV_STEP) - Tn;
-    const double* row = slice + r * BOYS_SLICE;
-    double acc = row[BOYS_TAYLOR];
+    const int r = (int)(Tn * B::inv_step + 0.5);
+    const double x = r * (1.0 / B::inv_step) - Tn;
+    const double* row = slice + r * B::cols;
+    double acc = row[B::taylor];
 #pragma unroll
-    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = fma(acc, x, row[k]);
+    for (int k = B::taylor - 1; k >= 1; --k) acc = fma(acc, x, row[k]);
     double Fn[M + 1];
     Fn[M] = acc;
     if (M > 0) {
-      constexpr double c[10] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040,
-                                1.0 / 40320, 1.0 / 362880};
-      double e = c[9];
+      double e = BOYS_EXP_C[B::nexp - 1];
 #pragma unroll
-      for (int k = 8; k >= 0; --k) e = fma(e, x, c[k]);
+      for (int k = B::nexp - 2; k >= 0; --k) e = fma(e, x, BOYS_EXP_C[k]);
       const double emt = row[0] * e;
       const double t2 = 2.0 * Tn;
 #pragma unroll
@@ -120,21 +157,19 @@ HFG_DEV void boys_eval(double T, const double* __restrict__ slice, double* __res
     return;
   }
   if (T < BOYS_TMAX) {
-    const int r = (int)(T * BOYS_INV_STEP + 0.5);
-    const double x = r * (1.0 / BOYS_INV_STEP) - T;  // -dT, |dT| <= 1/16
-    const double* row = slice + r * BOYS_SLICE;
+    const int r = (int)(T * B::inv_step + 0.5);
+    const double x = r * (1.0 / B::inv_step) - T;  // -dT
+    const double* row = slice + r * B::cols;
     // F_M(T) = sum_k [F_{M+k}(T_r)/k!] (-dT)^k
-    double acc = row[BOYS_TAYLOR];
+    double acc = row[B::taylor];
 #pragma unroll
-    for (int k = BOYS_TAYLOR - 1; k >= 1; --k) acc = fma(acc, x, row[k]);
+    for (int k = B::taylor - 1; k >= 1; --k) acc = fma(acc, x, row[k]);
     F[M] = acc;
     if (M > 0) {
-      // e^{-dT} = sum_k x^k/k!, 10 terms
-      constexpr double c[10] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040,
-                                1.0 / 40320, 1.0 / 362880};
-      double e = c[9];
+      // e^{-dT} = sum_k x^k/k!
+      double e = BOYS_EXP_C[B::nexp - 1];
 #pragma unroll
-      for (int k = 8; k >= 0; --k) e = fma(e, x, c[k]);
+      for (int k = B::nexp - 2; k >= 0; --k) e = fma(e, x, BOYS_EXP_C[k]);
       const double emt = row[0] * e;
       const double t2 = 2.0 * T;
 #pragma unroll
@@ -200,9 +235,10 @@ constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 
 #ifndef HFG_HOST_CHECK
 // Boys slice of order M into LDS by the whole workgroup (layout above).
-template <int M>
+template <int M, bool C = false>
 __device__ __forceinline__ void load_boys_slice(const double* __restrict__ g, double* __restrict__ lds) {
-  for (int r = threadIdx.x; r < BOYS_ROWS; r += blockDim.x) boys_slice_row<M>(g, r, lds + r * BOYS_SLICE);
+  using B = BoysSlice<C>;
+  for (int r = threadIdx.x; r < B::rows; r += blockDim.x) boys_slice_row<M, C>(g, r, lds + r * B::cols);
   __syncthreads();
 }
 #endif

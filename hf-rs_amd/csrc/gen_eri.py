@@ -449,7 +449,8 @@ def gen_class_gc(la, lb, lc, ld, S=0, V=0):
     out.append("      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
     out.append("      const double pref = TWO_PI_2_5 * Kp * Kq * rs;")
     out.append("      double Fm[%d];" % (M + 1))
-    out.append("      boys_eval<%d, %s>(T, boys, Fm);" % (M, "true" if (la, lb, lc, ld) in UNROLL_BRA else "false"))
+    out.append("      boys_eval<%d, %s, %s>(T, boys, Fm);" % (M, "true" if (la, lb, lc, ld) in UNROLL_BRA else "false",
+                                                      "true" if V == 1 and M > 0 else "false"))  # boys_compact(V, M)
     for m in range(M + 1):
         out.append("      const double F%d = pref * Fm[%d];" % (m, m))
     out.append("      " + "; ".join(bra_loads("rn", "n_")) + ";")

@@ -30,14 +30,11 @@
 #include "engine.h"
 #include "eri_device.h"
 #include "la_device.h"
+#include "rhf.h"
 #include "scf_host.h"
 
 namespace hfg {
 #include "gen/eri_grad.inc"
-
-void one_electron_block(const Shell& A, const Shell& B, const std::vector<std::array<double, 4>>& atoms,
-                        std::vector<double>& s, std::vector<double>& t, std::vector<double>& v,
-                        std::vector<std::vector<double>>* v_atom);
 
 namespace {
 
@@ -45,7 +42,7 @@ namespace {
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void k_gblock(BlockArgs a) {
   constexpr int NT = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
-  __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
+  __shared__ double boys[BoysSlice<false>::size];
   load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.ntask) return;
@@ -268,94 +265,7 @@ void rhf_gradient(Engine& e, const std::vector<std::array<double, 4>>& atoms, co
       for (int k = 0; k < 3; ++k) g[3 * a + k] -= f * d[k];
     }
   const std::vector<Shell>& sh = e.shells;
-  const int ns = (int)sh.size();
   const std::vector<double>& sc = e.ao_scale;
-  auto cidx = [](int L, int x, int y) {
-    // index of (x, y, L-x-y) in hf-rs order (d: xx, yy, zz, xy, xz, yz)
-    if (L == 2) {
-      const int z = L - x - y;
-      if (x == 2) return 0;
-      if (y == 2) return 1;
-      if (z == 2) return 2;
-      if (x == 1 && y == 1) return 3;
-      if (x == 1) return 4;
-      return 5;
-    }
-    int k = 0;
-    for (int lx = L; lx >= 0; --lx)
-      for (int ly = L - lx; ly >= 0; --ly, ++k)
-        if (lx == x && ly == y) return k;
-    return -1;
-  };
-  auto exps = [](int L, int i, int ex[3]) {
-    if (L == 2) {
-      static const int d[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
-      for (int c = 0; c < 3; ++c) ex[c] = d[i][c];
-      return;
-    }
-    int k = 0;
-    for (int lx = L; lx >= 0; --lx)
-      for (int ly = L - lx; ly >= 0; --ly, ++k)
-        if (k == i) {
-          ex[0] = lx;
-          ex[1] = ly;
-          ex[2] = L - lx - ly;
-          return;
-        }
-  };
-  // ---- one-electron terms, every ordered shell pair (A, B): derivatives of
-  // the A functions (src/gradient.rs:411-490)
-  std::vector<std::vector<double>> g1(ns, std::vector<double>((size_t)3 * na, 0.0));
-#pragma omp parallel for schedule(dynamic)
-  for (int A = 0; A < ns; ++A) {
-    std::vector<double>& out = g1[A];
-    const Shell& sa = sh[A];
-    const Shell ra = raised(sa);
-    const int nA = ncart(sa.L);
-    std::vector<double> s_r, t_r, v_r, s_l, t_l, v_l;
-    std::vector<std::vector<double>> vc_r, vc_l;
-    for (int B = 0; B < ns; ++B) {
-      const Shell& sb = sh[B];
-      const int nB = ncart(sb.L);
-      // V of all nuclei and of each (the operator derivative needs each)
-      one_electron_block(ra, sb, atoms, s_r, t_r, v_r, &vc_r);
-      if (sa.L > 0) one_electron_block(lowered(sa), sb, atoms, s_l, t_l, v_l, &vc_l);
-      for (int i = 0; i < nA; ++i) {
-        int ex[3];
-        exps(sa.L, i, ex);
-        const int mu = sa.ao + i;
-        for (int j = 0; j < nB; ++j) {
-          const int nu = sb.ao + j;
-          const double f = sc[mu] * sc[nu];
-          const double p = P[(size_t)mu * n + nu], w = W[(size_t)mu * n + nu];
-          for (int t = 0; t < 3; ++t) {
-            int up[3] = {ex[0], ex[1], ex[2]};
-            ++up[t];
-            const int ir = cidx(sa.L + 1, up[0], up[1]);
-            double ds = s_r[(size_t)ir * nB + j], dt = t_r[(size_t)ir * nB + j], dv = v_r[(size_t)ir * nB + j];
-            int il = -1;
-            if (ex[t] > 0) {
-              int dn[3] = {ex[0], ex[1], ex[2]};
-              --dn[t];
-              il = cidx(sa.L - 1, dn[0], dn[1]);
-              ds -= ex[t] * s_l[(size_t)il * nB + j];
-              dt -= ex[t] * t_l[(size_t)il * nB + j];
-              dv -= ex[t] * v_l[(size_t)il * nB + j];
-            }
-            out[3 * sa.atom + t] += 2.0 * f * (p * (dt + dv) - w * ds);
-            for (int c = 0; c < na; ++c) {
-              double vc = vc_r[c][(size_t)ir * nB + j];
-              if (il >= 0) vc -= ex[t] * vc_l[c][(size_t)il * nB + j];
-              out[3 * c + t] -= 2.0 * f * p * vc;
-            }
-          }
-        }
-      }
-    }
-  }
-  for (int A = 0; A < ns; ++A)
-    for (size_t x = 0; x < g.size(); ++x) g[x] += g1[A][x];
-
   // ---- two-electron part on the device
   hipStream_t st = e.stream;
   const size_t npair = e.pairs.size();
@@ -412,6 +322,13 @@ void rhf_gradient(Engine& e, const std::vector<std::array<double, 4>>& atoms, co
   HIP_CHECK(hipMemcpyAsync(d_scale.p, sc.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
   d_grad.alloc((size_t)3 * na * 8);
   hipLaunchKernelGGL(k_zero, dim3((3 * na + 255) / 256), dim3(256), 0, st, d_grad.as<double>(), (size_t)3 * na);
+  // ---- one-electron terms on the device (onee_dev.hip)
+  {
+    DevBuf d_W;
+    d_W.alloc((size_t)n * n * 8);
+    HIP_CHECK(hipMemcpyAsync(d_W.p, W, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
+    onee_gradient_device(sh, n, atoms, d_P.as<double>(), d_W.as<double>(), d_grad.as<double>(), st);
+  }
   const double* b = d_pp.as<double>();
   const PrimPairs view{b, b + npp, b + 2 * npp, b + 3 * npp, b + 4 * npp, b + 5 * npp, b + 6 * npp, b + 7 * npp,
                        b + 8 * npp};

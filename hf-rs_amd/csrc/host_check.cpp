@@ -24,15 +24,25 @@ extern "C" int hfgc_boys(int m, double t, double* out) {
   return 0;
 }
 
-// Templated per-class evaluation (the kernels' path) for M = 0..10.
-extern "C" int hfgc_boys_class(int m, double t, double* out) {
+template <int M, bool C>
+static void boys_class(double t, double* out) {
   static std::vector<double> tab = boys_table();
-  std::vector<double> slice(BOYS_ROWS * BOYS_SLICE);
+  using B = BoysSlice<C>;
+  std::vector<double> slice(B::size);
+  for (int r = 0; r < B::rows; ++r) boys_slice_row<M, C>(tab.data(), r, &slice[r * B::cols]);
+  boys_eval<M, false, C>(t, slice.data(), out);
+}
+
+// Templated per-class evaluation (the kernels' path) for M = 0..10, from the
+// full (compact = 0) or the compact (compact = 1) LDS slice.
+extern "C" int hfgc_boys_class(int m, int compact, double t, double* out) {
   switch (m) {
-#define B(M)                                                                                   \
-  case M:                                                                                      \
-    for (int r = 0; r < BOYS_ROWS; ++r) boys_slice_row<M>(tab.data(), r, &slice[r * BOYS_SLICE]); \
-    boys_eval<M>(t, slice.data(), out);                                                        \
+#define B(M)                            \
+  case M:                               \
+    if (compact)                        \
+      boys_class<M, true>(t, out);      \
+    else                                \
+      boys_class<M, false>(t, out);     \
     return 0;
     B(0) B(1) B(2) B(3) B(4) B(5) B(6) B(7) B(8) B(9) B(10)
 #undef B
@@ -43,13 +53,13 @@ extern "C" int hfgc_boys_class(int m, double t, double* out) {
 static int run_class(int cls, const PrimPairs& bp, int nb, const PrimPairs& kp, int nk, const double* AB,
                      const double* CD, double* blk) {
   static std::vector<double> tab = boys_table();
-  std::vector<double> slice(BOYS_ROWS * BOYS_SLICE);
+  std::vector<double> slice(BoysSlice<false>::size);
   switch (cls) {
 #define X(a, b_, c, d)                                                                              \
   case (a * (a + 1) / 2 + b_ > 5 ? 3 + a : a * (a + 1) / 2 + b_) *                                 \
                ((a * (a + 1) / 2 + b_ > 5 ? 3 + a : a * (a + 1) / 2 + b_) + 1) / 2 +              \
            (c * (c + 1) / 2 + d > 5 ? 3 + c : c * (c + 1) / 2 + d):                                \
-    for (int r = 0; r < BOYS_ROWS; ++r) boys_slice_row<a + b_ + c + d>(tab.data(), r, &slice[r * BOYS_SLICE]); \
+    for (int r = 0; r < BoysSlice<false>::rows; ++r) boys_slice_row<a + b_ + c + d>(tab.data(), r, &slice[r * BoysSlice<false>::cols]); \
     eri_block<a, b_, c, d>(bp, 0, nb, kp, 0, nk, 1, AB, CD, slice.data(), blk);                   \
     return 0;
     HFG_FOR_EACH_CLASS_ALL(X)
@@ -157,11 +167,11 @@ extern "C" int hfgc_shell_quartet(const hfg_basis_view* b, int sa, int sb, int s
       AB[k] = sh[o.k[0]].c[k] - sh[o.k[1]].c[k];
       CD[k] = sh[o.k[2]].c[k] - sh[o.k[3]].c[k];
     }
-    std::vector<double> blk(2048), slice(BOYS_ROWS * BOYS_SLICE);
+    std::vector<double> blk(2048), slice(BoysSlice<false>::size);
     switch (o.cls) {
 #define X(a, b_, c, d)                                                                              \
   case (a * (a + 1) / 2 + b_) * ((a * (a + 1) / 2 + b_) + 1) / 2 + (c * (c + 1) / 2 + d):          \
-    for (int r = 0; r < BOYS_ROWS; ++r) boys_slice_row<a + b_ + c + d>(tab.data(), r, &slice[r * BOYS_SLICE]); \
+    for (int r = 0; r < BoysSlice<false>::rows; ++r) boys_slice_row<a + b_ + c + d>(tab.data(), r, &slice[r * BoysSlice<false>::cols]); \
     eri_block<a, b_, c, d>(bp, 0, (int)bra.size(), kp, 0, (int)ket.size(), 1, AB, CD, slice.data(), blk.data()); \
     break;
       HFG_FOR_EACH_CLASS_ALL(X)

@@ -34,11 +34,12 @@ const std::vector<double>& boys_tab() {
   return tab;
 }
 
-template <int M>
+template <int M, bool C = false>
 const double* slice() {
   static std::vector<double> s = [] {
-    std::vector<double> v(BOYS_ROWS * BOYS_SLICE);
-    for (int r = 0; r < BOYS_ROWS; ++r) boys_slice_row<M>(boys_tab().data(), r, &v[r * BOYS_SLICE]);
+    using B = BoysSlice<C>;
+    std::vector<double> v(B::size);
+    for (int r = 0; r < B::rows; ++r) boys_slice_row<M, C>(boys_tab().data(), r, &v[r * B::cols]);
     return v;
   }();
   return s.data();
@@ -105,7 +106,8 @@ void replay_class_v(Replay& r) {
       const ViewDev& vk = P.views[ket];
       const int j = ket - g.start;
       eri_gc_all<LA, LB, LC, LD, CLS, V>(P.vb.data(), vb.p0, vb.p1, kp, P.kcoef.data() + g.cbase + j, g.base + j,
-                                      g.npp, g.size, vb.AB, vk.AB, slice<LA + LB + LC + LD>(), I.data());
+                                      g.npp, g.size, vb.AB, vk.AB,
+                                      slice<LA + LB + LC + LD, boys_compact(V, LA + LB + LC + LD)>(), I.data());
       double w = 1.0;
       if (vb.same) w *= 0.5;
       if (vk.same) w *= 0.5;

@@ -486,8 +486,9 @@ __device__ __forceinline__ void jk_task_slice(const MolView& a, double screen0, 
 // copies).
 template <int LA, int LB, int LC, int LD, int V>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
-  __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
-  load_boys_slice<LA + LB + LC + LD>(a.boys, boys);
+  constexpr bool BC = boys_compact(V, LA + LB + LC + LD);
+  __shared__ double boys[BoysSlice<BC>::size];
+  load_boys_slice<LA + LB + LC + LD, BC>(a.boys, boys);
   constexpr int NS = slice_count<LA, LB, LC, LD>();
   constexpr int CLS = class_index<LA, LB, LC, LD>();
   const int lane = threadIdx.x & 63;
@@ -517,12 +518,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
   constexpr int RA = gc_cap(CLS, V, 0), RB = gc_cap(CLS, V, 1), CA = RA * RB;
   constexpr int NR = RA * NA + RB * NB;
   constexpr int NS = slice_count<LA, LB, LC, LD>();
-  __shared__ double boys[BOYS_ROWS * BOYS_SLICE];
+  constexpr bool BC = boys_compact(V, LA + LB + LC + LD);
+  __shared__ double boys[BoysSlice<BC>::size];
   __shared__ int job_sh;
   extern __shared__ double rows[];
   // a.nmax is the largest molecule's size: it bounds every job's rows
   for (int e = threadIdx.x; e < NR * a.nmax + CA * NAB; e += blockDim.x) rows[e] = 0.0;
-  load_boys_slice<LA + LB + LC + LD>(a.boys, boys);  // ends with a barrier
+  load_boys_slice<LA + LB + LC + LD, BC>(a.boys, boys);  // ends with a barrier
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) job_sh = atomicAdd(a.job_counter, 1);
@@ -590,7 +592,7 @@ void launch_jk(const JKArgs& a, int max_blocks, hipStream_t s) {
     cached_dev = dev;
     cached_lds = 0;
   }
-  if (a.use_rows && a.njob > 0 && lds + sizeof(double) * BOYS_ROWS * BOYS_SLICE + 64 <= (size_t)lds_limit) {
+  if (a.use_rows && a.njob > 0 && lds + sizeof(double) * BoysSlice<boys_compact(V, LA + LB + LC + LD)>::size + 64 <= (size_t)lds_limit) {
     if (lds != cached_lds) {
       int per_cu = 0;
       HIP_CHECK(hipFuncSetAttribute((const void*)k_jk_rows<LA, LB, LC, LD, V>,

@@ -1,5 +1,6 @@
-// onee.cpp -- one-electron matrices S, T, V and the real spherical transform
-// for the SCF driver (host side, OpenMP over shell pairs).
+// onee.cpp -- host one-electron matrices S, T, V for the atomic SAD solves
+// (a few dozen functions each; the molecules' matrices are built on the
+// device, onee_dev.hip) and the real spherical transform.
 //
 // Obara-Saika recurrences: 1-D overlaps S_ij with S_{i+1,j} = X_PA S_ij +
 // (i S_{i-1,j} + j S_{i,j-1})/(2p); kinetic t_ij = b(2j+1) S_ij - 2b^2 S_{i,j+2}
@@ -36,8 +37,7 @@ static std::vector<std::array<int, 3>> comps(int L) {
 
 // Shell-pair block of S, T, V in (component a, component b) order, unscaled.
 static void pair_block(const Shell& A, const Shell& B, const std::vector<std::array<double, 4>>& atoms,
-                       std::vector<double>& s, std::vector<double>& t, std::vector<double>& v,
-                       std::vector<std::vector<double>>* v_atom = nullptr) {
+                       std::vector<double>& s, std::vector<double>& t, std::vector<double>& v) {
   const int la = A.L, lb = B.L, Lt = la + lb;
   auto ca = comps(la), cb = comps(lb);
   const int na = (int)ca.size(), nb = (int)cb.size();
@@ -52,8 +52,6 @@ static void pair_block(const Shell& A, const Shell& B, const std::vector<std::ar
   }
   const int D = Lt + 1;  // index stride for [x][y][z]
   std::vector<double> ve((size_t)D * D * D, 0.0);  // contracted V_[e|0]
-  // per nucleus when asked for (the gradient's operator derivative)
-  std::vector<std::vector<double>> vea(v_atom ? atoms.size() : 0, std::vector<double>((size_t)D * D * D, 0.0));
   std::vector<double> th((size_t)(Lt + 1) * D * D * D);
   double F[64];
   for (int ka = 0; ka < A.K; ++ka)
@@ -131,14 +129,11 @@ static void pair_block(const Shell& A, const Shell& B, const std::vector<std::ar
         for (int l = la; l <= Lt; ++l)
           for (int x = l; x >= 0; --x)
             for (int y = l - x; y >= 0; --y) {
-              const size_t k = ((size_t)x * D + y) * D + (l - x - y);
-              const double val = I(0, x, y, l - x - y);
-              ve[k] += val;
-              if (v_atom) vea[ia][k] += val;
+              ve[((size_t)x * D + y) * D + (l - x - y)] += I(0, x, y, l - x - y);
             }
       }
     }
-  // horizontal recurrence on the contracted V (of all nuclei, or of one)
+  // horizontal recurrence on the contracted V
   const std::vector<double>* src = &ve;
   std::map<std::array<int, 6>, double> memo;
   std::function<double(std::array<int, 3>, std::array<int, 3>)> hrr = [&](std::array<int, 3> ea,
@@ -157,26 +152,6 @@ static void pair_block(const Shell& A, const Shell& B, const std::vector<std::ar
   };
   for (int x = 0; x < na; ++x)
     for (int y = 0; y < nb; ++y) v[x * nb + y] = hrr(ca[x], cb[y]);
-  if (v_atom) {
-    v_atom->assign(atoms.size(), std::vector<double>());
-    for (size_t ia = 0; ia < atoms.size(); ++ia) {
-      src = &vea[ia];
-      memo.clear();
-      std::vector<double>& o = (*v_atom)[ia];
-      o.assign((size_t)na * nb, 0.0);
-      for (int x = 0; x < na; ++x)
-        for (int y = 0; y < nb; ++y) o[x * nb + y] = hrr(ca[x], cb[y]);
-    }
-  }
-}
-
-// The unscaled S, T, V blocks of shells A, B (any L up to 3, as the raised
-// shells of the gradient need), components in hf-rs order; v_atom (if not
-// null) receives V of each nucleus alone.
-void one_electron_block(const Shell& A, const Shell& B, const std::vector<std::array<double, 4>>& atoms,
-                        std::vector<double>& s, std::vector<double>& t, std::vector<double>& v,
-                        std::vector<std::vector<double>>* v_atom) {
-  pair_block(A, B, atoms, s, t, v, v_atom);
 }
 
 void one_electron(const std::vector<Shell>& sh, int n, const std::vector<std::array<double, 4>>& atoms,

@@ -274,42 +274,29 @@ static std::string shell_key(const std::vector<Shell>& sh, double z) {
   return os.str();
 }
 
-HMat sad_guess(const std::vector<Shell>& sh, int n, const std::vector<std::array<double, 4>>& atoms, int charge,
-               const HMat& S) {
+HMat atomic_density(const std::vector<Shell>& sh, int atom, double z, std::vector<int>& idx) {
   static std::mutex mu;
   static std::map<std::string, AtomicDensity> cache;  // src/sad.rs:136-214
-  HMat P((size_t)n * n, 0.0);
-  double target = -charge;
-  for (auto& a : atoms) target += a[0];
-  for (size_t a = 0; a < atoms.size(); ++a) {
-    std::vector<Shell> loc;
-    for (auto& s : sh)
-      if (s.atom == (int)a) loc.push_back(s);
-    if (loc.empty()) throw HfgError{HFG_INVALID_INPUT, "atom without basis functions"};
-    const std::string key = shell_key(loc, atoms[a][0]);
-    AtomicDensity ad;
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      auto it = cache.find(key);
-      if (it != cache.end()) ad = it->second;
-    }
-    if (ad.p.empty()) {
-      ad = atomic_hf(loc, atoms[a][0]);
-      std::lock_guard<std::mutex> lk(mu);
-      cache[key] = ad;
-    }
-    std::vector<int> idx;
-    for (auto& s : loc)
-      for (size_t c = 0; c < s.scale.size(); ++c) idx.push_back(s.ao + (int)c);
-    const int m = (int)idx.size();
-    for (int i = 0; i < m; ++i)
-      for (int j = 0; j < m; ++j) P[(size_t)idx[i] * n + idx[j]] = ad.p[(size_t)i * m + j];
+  std::vector<Shell> loc;
+  for (auto& s : sh)
+    if (s.atom == atom) loc.push_back(s);
+  if (loc.empty()) throw HfgError{HFG_INVALID_INPUT, "atom without basis functions"};
+  const std::string key = shell_key(loc, z);
+  AtomicDensity ad;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) ad = it->second;
   }
-  double el = 0;
-  for (size_t i = 0; i < P.size(); ++i) el += P[i] * S[i];
-  if (!(el > 1e-12)) throw HfgError{HFG_INTEGRAL, "zero SAD population"};
-  for (double& x : P) x *= target / el;
-  return P;
+  if (ad.p.empty()) {
+    ad = atomic_hf(loc, z);
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = ad;
+  }
+  idx.clear();
+  for (auto& s : loc)
+    for (size_t c = 0; c < s.scale.size(); ++c) idx.push_back(s.ao + (int)c);
+  return ad.p;
 }
 
 RhfInputs rhf_inputs(const hfg_basis_view& basis, const double* zxyz, int natom, int charge) {
@@ -396,32 +383,20 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
     t_phase = now;
     return d;
   };
-  // The one-electron matrices (host) are computed on a second thread while
-  // the engine is set up (Schwarz bounds on the GPU, J/K plan on the host).
-  HMat S, T, V;
-  const std::vector<Shell> sh1 = group_shells(*basis);
-  auto one_e = std::async(std::launch::async, [&] { one_electron(sh1, n, atoms, S, T, V); });
-  std::unique_ptr<Engine> eng_p;
-  try {
-    eng_p.reset(new Engine(*basis));
-  } catch (...) {
-    one_e.wait();
-    throw;
-  }
+  std::unique_ptr<Engine> eng_p(new Engine(*basis));
   Engine& eng = *eng_p;
   res->seconds_setup = lap();
   hipStream_t st = eng.stream;
   Blas bl(st);
   const size_t nn = (size_t)n * n;
   const double enuc = in.enuc;
-  one_e.get();  // the remaining wait, if any, is the one-electron phase
-  HMat H(nn);
-  for (size_t i = 0; i < nn; ++i) H[i] = T[i] + V[i];
-  res->seconds_one_electron = lap();
 
+  // S and H = T + V on the device (onee_dev.hip)
   Dev dS, dH, dX, dTmp, dTmp2, dF, dP, dPprev, dErr, dW, dC, dFu;
   dS.alloc(nn);
   dH.alloc(nn);
+  one_electron_device(eng.shells, n, atoms, dS.p, nullptr, nullptr, dH.p, st);
+  res->seconds_one_electron = lap();
   dTmp.alloc(nn);
   dTmp2.alloc(nn);
   dF.alloc(nn);
@@ -431,8 +406,6 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   dW.alloc(n);
   dC.alloc(nn);
   dFu.alloc(nn);
-  upload(dS.p, S.data(), nn, st);
-  upload(dH.p, H.data(), nn, st);
   Eigh eig;
   int m = 0;
   {
@@ -531,20 +504,16 @@ int scf_rhf(const hfg_basis_view* basis, const double* zxyz, int natom, int char
   // A given density replaces the guess (run_scf_with_initial_density,
   // src/scf.rs:630-639).
   bool have_sad = true;
-  HMat P0;
   if (p_guess) {
-    P0.assign(p_guess, p_guess + nn);
+    upload(dP.p, p_guess, nn, st);
   } else {
     try {
-      P0 = sad_guess(eng.shells, n, atoms, charge, S);
+      sad_guess_device(eng.shells, n, atoms, charge, dS.p, dP.p, bl, st);
     } catch (HfgError&) {
       have_sad = false;
     }
   }
-  if (have_sad)
-    upload(dP.p, P0.data(), nn, st);
-  else
-    diag_density(dH.p, dP.p, false);
+  if (!have_sad) diag_density(dH.p, dP.p, false);
   HIP_CHECK(hipStreamSynchronize(st));
   res->seconds_guess = lap();
 

@@ -103,6 +103,18 @@ int hfg_contract_jk_device(hfg_ctx* ctx, const double* d_p, double* d_j, double*
  * shares the array: calls that overlap on one GPU race. */
 int hfg_contract_jk_device_noalloc(hfg_ctx* ctx, const double* d_p, double* d_j, double* d_k);
 
+/* S, T and V (n x n each, row-major, Cartesian, the basis's component
+ * normalisation) of the context's basis for natom nuclei (Z, x, y, z; bohr),
+ * computed on the device.  Replaces compute_one_electron_integrals
+ * (src/scf.rs:246-271; H = T + V).  Shells up to d, as the context. */
+int hfg_one_electron(hfg_ctx* ctx, const double* zxyz, int natom, double* s, double* t, double* v);
+/* The SAD initial density (n x n) of the context's molecule: the cached
+ * fractional-occupation atomic densities, superposed and scaled to the
+ * electron count with the device overlap.  Replaces compute_sad_density
+ * (src/sad.rs:217-259).  HFG_INVALID_INPUT when the basis names an atom
+ * beyond natom or the electron count is not positive. */
+int hfg_sad_density(hfg_ctx* ctx, const double* zxyz, int natom, int charge, double* p);
+
 /* Analytic RHF nuclear gradient, compute_gradient_from_density
  * (src/gradient.rs:287-512): grad (natom x 3, Eh/bohr) of the context's
  * molecule for density p and energy-weighted density w (n x n each; for a

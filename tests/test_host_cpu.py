@@ -116,19 +116,40 @@ def test_generated_recurrences_match_oracle_on_contracted_shells():
     assert worst < 1e-12, worst
 
 
-def test_kernel_boys_path_against_mpmath_fixture():
-    """The kernels' per-class evaluation (LDS slice, T >= 50 branch) for M = 0..10."""
+@pytest.mark.parametrize("compact", [0, 1])
+def test_kernel_boys_path_against_mpmath_fixture(compact):
+    """The kernels' per-class evaluation (full or compact LDS slice, T >= 50
+    branch) for M = 0..10."""
     L = cpucheck()
-    L.hfgc_boys_class.argtypes = [C.c_int, C.c_double, C.c_void_p]
+    L.hfgc_boys_class.argtypes = [C.c_int, C.c_int, C.c_double, C.c_void_p]
     worst = 0.0
     for row in json.load(open(os.path.join(GOLDEN, "boys.json")))["rows"]:
         for M in range(11):
             out = np.zeros(M + 1)
-            assert L.hfgc_boys_class(M, row["t"], out.ctypes.data) == 0
+            assert L.hfgc_boys_class(M, compact, row["t"], out.ctypes.data) == 0
             for m in range(M + 1):
                 ref = float(row["f"][m])
                 worst = max(worst, abs(out[m] - ref) / ref)
     assert worst < 5e-15, worst
+
+
+@pytest.mark.parametrize("compact", [0, 1])
+def test_kernel_boys_path_between_grid_rows(compact):
+    """Both slice layouts on a dense T grid (every eighth between the slice
+    rows, where the Taylor step is longest, up to and across T = 50) against
+    the oracle's Boys function, pinned to mpmath at 1e-14 elsewhere in this
+    suite: 1e-14 relative, F_0..F_M for M = 0..10."""
+    L = cpucheck()
+    L.hfgc_boys_class.argtypes = [C.c_int, C.c_int, C.c_double, C.c_void_p]
+    ts = np.concatenate([np.arange(0.0, 52.0, 1.0 / 64), [49.999, 50.0, 50.001, 75.0, 200.0]])
+    worst = 0.0
+    for t in ts:
+        ref = O.boys(10, float(t))
+        for M in (0, 3, 6, 10):
+            out = np.zeros(M + 1)
+            assert L.hfgc_boys_class(M, compact, float(t), out.ctypes.data) == 0
+            worst = max(worst, float(np.max(np.abs(out - ref[:M + 1]) / ref[:M + 1])))
+    assert worst < 1e-14, worst
 
 
 @pytest.mark.parametrize("basis,cells", [("sto-3g", (2, 1, 1)), ("6-31g*", (2, 1, 1)), ("cc-pvdz", (2, 1, 1)),

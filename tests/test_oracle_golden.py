@@ -369,3 +369,64 @@ def test_rhf_gradient_matches_pyscf(name, basis):
     g = O.rhf_gradient(b, mol.zxyz, p, 0.5 * p @ f @ p)
     assert np.abs(g - np.array(ref["gradient"])).max() < 1e-6
     assert np.abs(g.sum(axis=0)).max() < 1e-9
+
+
+# --- one-electron matrices: closed forms for s and p Gaussians -----------------
+# (Szabo and Ostlund, Modern Quantum Chemistry, appendix A: overlap, kinetic
+# and nuclear-attraction integrals of normalised s Gaussians; the p function
+# x_A exp(-a r_A^2) has norm 2 sqrt(a) (2a/pi)^(3/4) and kinetic energy 5a/2.)
+TWO_CENTRE_SP = """H     0
+S   1   1.00
+      0.80000000             1.00000000
+P   1   1.00
+      0.50000000             1.00000000
+****
+"""
+
+
+def sp_closed_forms(xyz):
+    """S, T, V of the functions (s_A, p_A, s_B, p_B) of two protons at xyz."""
+    a, b = 0.8, 0.5
+    A, B = np.asarray(xyz[0], float), np.asarray(xyz[1], float)
+
+    def ns(x):
+        return (2 * x / math.pi) ** 0.75
+
+    def f0(t):
+        return 1.0 if t < 1e-14 else 0.5 * math.sqrt(math.pi / t) * math.erf(math.sqrt(t))
+
+    def ss(x, X, y, Y):  # (S, T, V) of normalised s functions on X and Y
+        p, mu = x + y, x * y / (x + y)
+        r2 = float(((X - Y) ** 2).sum())
+        P = (x * X + y * Y) / p
+        s = ns(x) * ns(y) * (math.pi / p) ** 1.5 * math.exp(-mu * r2)
+        t = mu * (3 - 2 * mu * r2) * s
+        v = sum(-ns(x) * ns(y) * 2 * math.pi / p * math.exp(-mu * r2) * f0(p * float(((P - C) ** 2).sum()))
+                for C in (A, B))
+        return s, t, v
+
+    s_aa = ss(a, A, a, A)
+    s_ab = ss(a, A, a, B)
+    # <p_x on A (exponent b) | s on B (exponent a)> = N_p N_s (P - A)_x (pi/p)^(3/2) exp(-mu R^2)
+    p, mu = a + b, a * b / (a + b)
+    r2 = float(((A - B) ** 2).sum())
+    P = (b * A + a * B) / p
+    pxs = 2 * math.sqrt(b) * ns(b) * ns(a) * (P - A)[0] * (math.pi / p) ** 1.5 * math.exp(-mu * r2)
+    return {"s_aa": s_aa, "s_ab": s_ab, "p_x_A_s_B": pxs, "t_pp": 2.5 * b}
+
+
+def test_oracle_one_electron_matches_closed_forms():
+    xyz = [[0.1, -0.2, 0.3], [1.0, 0.4, -0.5]]
+    B = O.Basis.build(TWO_CENTRE_SP, ["H", "H"], xyz)
+    zxyz = np.array([[1.0] + xyz[0], [1.0] + xyz[1]])
+    s, t, v = B.one_electron(zxyz)
+    ref = sp_closed_forms(xyz)
+    # functions: s_A, px_A, py_A, pz_A, s_B, px_B, py_B, pz_B
+    assert np.abs(np.diag(s) - 1.0).max() < 1e-14
+    for k, (sv, tv, vv) in (("s_aa", ref["s_aa"]), ("s_ab", ref["s_ab"])):
+        j = 0 if k == "s_aa" else 4
+        assert abs(s[0, j] - sv) < 1e-14 and abs(t[0, j] - tv) < 1e-13 and abs(v[0, j] - vv) < 1e-13
+    assert abs(s[1, 4] - ref["p_x_A_s_B"]) < 1e-14
+    assert abs(t[1, 1] - ref["t_pp"]) < 1e-13 and abs(t[2, 2] - ref["t_pp"]) < 1e-13
+    for m in (s, t, v):
+        assert np.abs(m - m.T).max() < 1e-15
