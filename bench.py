@@ -115,25 +115,30 @@ def cpu_baseline(M, count, threads, budget_s, seed=11):
                           len(sample), count, ", ".join(names))}
 
 
-def df_report(hf, M, steps=5, solves=True):
-    """Density-fitted J/K on (H2O)32/cc-pVDZ (4x4x2 lattice, seed 4) with the
+def df_report(hf, M, steps=5, solves=True, basis="cc-pvdz"):
+    """Density-fitted J/K on (H2O)32 (4x4x2 lattice, seed 4) in `basis`
+    (cc-pVDZ; cc-pVTZ is the survey's config 4, n = 2080) with the
     def2-universal-JKFIT auxiliary basis (Cartesian), 160 occupied orbitals
     (random, fixed seed): the exchange contraction in FP64, FP32 and split
-    BF16 on the matrix cores, each with its error against FP64."""
+    BF16 on the matrix cores, each with its error against FP64.  At cc-pVTZ
+    the factor is kept packed ([rank][n(n+1)/2], 74 GB) and formed one
+    column chunk at a time."""
     mol = M.water_cluster((4, 4, 2), seed=4)
-    b = hf.Basis.from_g94(M.basis_text("cc-pvdz"), mol.symbols, mol.xyz)
+    b = hf.Basis.from_g94(M.basis_text(basis), mol.symbols, mol.xyz)
     a = hf.Basis.from_g94(M.basis_text("def2-universal-jkfit"), mol.symbols, mol.xyz)
     t0 = time.perf_counter()
     df = hf.DfTensor(b, a)
     build = time.perf_counter() - t0
+    progress("DF %s: factor built in %.1f s" % (basis, build))
     o = 5 * 32
     c = np.random.default_rng(7).standard_normal((b.n, o)) * 0.1
     n, rank = b.n, df.rank
     k_flops = 2.0 * rank * n * n * o + 2.0 * n * n * rank * o  # A = B C and K = 2 A A^T as computed
     # algorithmic count of DF-K: A = B C plus the symmetric A A^T counted once
     k_alg = 2.0 * rank * n * n * o + 1.0 * rank * n * (n + 1) * o
-    out = {"workload": "DF-J/K (H2O)32 cc-pVDZ / def2-universal-JKFIT, n %d, n_aux %d, rank %d, %d occupied"
-                       % (n, df.n_aux, rank, o), "build_s": build, "k_flops_per_build": k_flops,
+    out = {"workload": "DF-J/K (H2O)32 %s / def2-universal-JKFIT, n %d, n_aux %d, rank %d, %d occupied"
+                       % ({"cc-pvdz": "cc-pVDZ", "cc-pvtz": "cc-pVTZ"}.get(basis, basis), n, df.n_aux, rank, o),
+           "build_s": build, "k_flops_per_build": k_flops,
            "k_algorithmic_flops_per_build": k_alg}
     j0, k0, _ = df.jk_precision(c, "fp64")
     scale = float(np.abs(k0).max())
@@ -561,6 +566,9 @@ def main():
     ap.add_argument("--fock-steps", type=int, default=20)
     ap.add_argument("--solve-repeats", type=int, default=5)
     ap.add_argument("--no-df", action="store_true")
+    ap.add_argument("--no-df4", action="store_true", help="skip the config-4 DF (cc-pVTZ, packed factor)")
+    ap.add_argument("--only-df4", action="store_true",
+                    help="only the config-4 DF build and DF J/K builds, without solves (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -594,9 +602,9 @@ def main():
     if hf.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    if args.only_fock or args.only_df:
+    if args.only_fock or args.only_df or args.only_df4:
         sub = (fock_report(hf, M, args.fock_steps, 3, 0, scf=False) if args.only_fock
-               else df_report(hf, M, steps=args.steps, solves=False))
+               else df_report(hf, M, steps=args.steps, solves=False, basis="cc-pvtz" if args.only_df4 else "cc-pvdz"))
         print(json.dumps(sub), flush=True)
         return
     fused, roof = fused_report(hf, M, args.batch_count, args.steps, args.warmup, unbatched=not args.only_fused)
@@ -632,6 +640,9 @@ def main():
     if not args.no_df:
         line["df"] = df_report(hf, M)
         progress("DF done")
+    if not args.no_df4:
+        line["df_config4"] = df_report(hf, M, steps=3, basis="cc-pvtz")
+        progress("DF config 4 done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(M, args.batch_count, args.cpu_threads, args.cpu_seconds)
         progress("CPU baseline done")

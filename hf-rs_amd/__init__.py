@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libhfg.so")
 _LIB = None
 
 HFG_OK, HFG_INVALID_INPUT, HFG_INTEGRAL, HFG_DEVICE, HFG_NOT_CONVERGED, HFG_UNSUPPORTED = range(6)
+HFG_SERVE_KEEP_DENSITY, HFG_SERVE_GRADIENT = 1, 2  # hfg_server_submit flags
 
 
 class HfError(RuntimeError):
@@ -59,7 +60,8 @@ EXPORTED = ["hfg_last_error", "hfg_device_count", "hfg_ctx_create", "hfg_ctx_des
             "hfg_batch_get_jk", "hfg_batch_jk_each", "hfg_df_jk_ex", "hfg_contract_jk_multi", "hfg_server_create",
             "hfg_server_destroy", "hfg_server_submit", "hfg_server_run", "hfg_server_result", "hfg_server_stats",
             "hfg_scf_rhf_guess", "hfg_eri_mo", "hfg_batch_profile", "hfg_batch_class_stats", "hfg_cd_build",
-            "hfg_cd_rhf", "hfg_rhf_gradient", "hfg_graph_launches", "hfg_one_electron", "hfg_sad_density"]
+            "hfg_cd_rhf", "hfg_rhf_gradient", "hfg_graph_launches", "hfg_one_electron", "hfg_sad_density",
+            "hfg_server_gradient"]
 
 
 class ServerOptions(C.Structure):
@@ -165,6 +167,7 @@ def lib():
                                         C.POINTER(C.c_int)]
         L.hfg_server_run.argtypes = [vp]
         L.hfg_server_result.argtypes = [vp, i32, C.POINTER(RequestResult), vp, vp]
+        L.hfg_server_gradient.argtypes = [vp, i32, vp]
         L.hfg_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
         _LIB = L
     return _LIB
@@ -359,9 +362,11 @@ class Server:
     def __del__(self):
         self.close()
 
-    def submit(self, bases, zxyzs, charge=0, arrival=0.0, keep_density=False, **scf_kw):
+    def submit(self, bases, zxyzs, charge=0, arrival=0.0, keep_density=False, gradients=False, **scf_kw):
         """Queues one request: a trajectory of len(bases) geometries of one
-        molecule (bases[k] built at zxyzs[k]); returns its id."""
+        molecule (bases[k] built at zxyzs[k]); returns its id.  gradients:
+        the analytic nuclear gradient of every step as well (result()'s
+        "gradients", nsteps x natom x 3)."""
         bases = list(bases)
         atoms = [_atoms(z) for z in zxyzs]
         if not bases or len(bases) != len(atoms):
@@ -374,9 +379,10 @@ class Server:
         opt = scf_options(**scf_kw)
         rid = C.c_int()
         _check(lib().hfg_server_submit(self.h, float(arrival), len(bases), C.cast(views, C.c_void_p),
-                                       C.cast(zp, C.c_void_p), natom, int(charge), C.byref(opt), int(keep_density),
-                                       C.byref(rid)))
-        self.requests.append((rid.value, [b.n for b in bases], bases, atoms))
+                                       C.cast(zp, C.c_void_p), natom, int(charge), C.byref(opt),
+                                       (HFG_SERVE_KEEP_DENSITY if keep_density else 0) |
+                                       (HFG_SERVE_GRADIENT if gradients else 0), C.byref(rid)))
+        self.requests.append((rid.value, [b.n for b in bases], bases, atoms, bool(gradients)))
         return rid.value
 
     def run(self):
@@ -397,6 +403,10 @@ class Server:
             out["error"] = lib().hfg_last_error().decode()
         if density:
             out["density"] = p
+        if self.requests[rid][4] and st == HFG_OK:
+            g = np.zeros((n_steps, self.requests[rid][3][0].shape[0], 3))
+            _check(lib().hfg_server_gradient(self.h, rid, _p(g)))
+            out["gradients"] = g
         return out
 
     def stats(self):

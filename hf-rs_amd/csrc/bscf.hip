@@ -413,6 +413,8 @@ struct Request {
   std::vector<double> last_p;  // final density of the last step
   int n = 0;
   bool keep_density = false;
+  bool want_gradient = false;  // nuclear gradient of every step
+  std::vector<double> grad;    // nsteps x natom x 3
   int status = HFG_OK;
   std::string error;
 };
@@ -505,8 +507,12 @@ struct Server {
   // Admission pool: slot set-ups run on the worker threads while the slots
   // already set up iterate; the main loop collects the finished ones at the
   // start of each iteration.
+  // A worker job for slot si: the gradient of step grad_step (if >= 0) of
+  // request ri, then the set-up of step `step` (none when step is past the
+  // request's last).  result: 0 failed, 1 set up, 2 set up without SAD
+  // (core guess), 3 gradient only (the request has finished).
   struct Setup {
-    int si, ri, step, result;
+    int si, ri, step, result, grad_step;
   };
   std::mutex pool_mu;
   std::condition_variable pool_cv, done_cv;
@@ -531,6 +537,7 @@ struct Server {
   // 0: failed (recorded in the request), 1: ready, 2: ready but the density
   // must come from the core Hamiltonian (no SAD guess)
   int admit(int slot, int req, int step, const double* guess, Worker& w);
+  void step_gradient(int slot, int req, int step, Worker& w);
 
   void finish_step(int slot);
   void density(const std::vector<int>& L);
@@ -678,6 +685,26 @@ int Server::admit(int si, int ri, int step, const double* guess, Worker& w) {
   return 0;
 }
 
+// Nuclear gradient of a finished geometry step from the slot's final density
+// P and Fock matrix F (W = 1/2 P F P, the energy-weighted density of a
+// converged RHF), on the worker's stream; the slot stays reserved (SETUP)
+// meanwhile, and the gradient reads only its engine's tables.
+void Server::step_gradient(int si, int ri, int step, Worker& w) {
+  Slot& s = slots[si];
+  Request& R = reqs[ri];
+  const int n = s.n;
+  const size_t nn = (size_t)n * n;
+  Dev t;
+  t.alloc(2 * nn);
+  w.blas->gemm(false, false, n, n, n, 1.0, s.mol.F, n, s.mol.P, n, 0.0, t.p, n);     // F P
+  w.blas->gemm(false, false, n, n, n, 0.5, s.mol.P, n, t.p, n, 0.0, t.p + nn, n);  // 1/2 P F P
+  std::vector<std::array<double, 4>> atoms(R.natom);
+  const std::vector<double>& z = R.steps[step].zxyz;
+  for (int a = 0; a < R.natom; ++a)
+    for (int k = 0; k < 4; ++k) atoms[a][k] = z[4 * a + k];
+  rhf_gradient_device(*s.eng, atoms, s.mol.P, t.p + nn, R.grad.data() + (size_t)step * 3 * R.natom, w.stream);
+}
+
 // Set-ups run on the worker threads, each with its own stream and BLAS
 // handle; a finished one waits in pool_done for the main loop.
 void Server::pool_work(Worker& w) {
@@ -691,9 +718,30 @@ void Server::pool_work(Worker& w) {
       pool_jobs.pop_front();
     }
     Slot& s = slots[job.si];
-    const bool warm = !s.guess.empty() &&
-                      s.guess.size() == reqs[job.ri].steps[job.step].atom.size() * reqs[job.ri].steps[job.step].atom.size();
-    job.result = admit(job.si, job.ri, job.step, warm ? s.guess.data() : nullptr, w);
+    Request& R = reqs[job.ri];
+    bool ok = true;
+    if (job.grad_step >= 0) {
+      try {
+        step_gradient(job.si, job.ri, job.grad_step, w);
+      } catch (HfgError& e) {
+        R.status = e.code;
+        R.error = e.msg;
+        ok = false;
+      } catch (std::exception& e) {
+        R.status = HFG_INTEGRAL;
+        R.error = e.what();
+        ok = false;
+      }
+    }
+    if (job.step >= (int)R.steps.size()) {
+      job.result = ok ? 3 : 0;
+    } else if (!ok) {
+      job.result = 0;
+    } else {
+      const bool warm = !s.guess.empty() &&
+                        s.guess.size() == R.steps[job.step].atom.size() * R.steps[job.step].atom.size();
+      job.result = admit(job.si, job.ri, job.step, warm ? s.guess.data() : nullptr, w);
+    }
     {
       std::lock_guard<std::mutex> lk(pool_mu);
       pool_done.push_back(job);
@@ -708,7 +756,9 @@ void Server::submit(const std::vector<std::array<int, 3>>& todo) {
     std::lock_guard<std::mutex> lk(pool_mu);
     for (const auto& t : todo) {
       slots[t[0]].phase = SETUP;
-      pool_jobs.push_back(Setup{t[0], t[1], t[2], 0});
+      // a step after the first follows a finished step of the same request
+      const int gs = t[2] > 0 && reqs[t[1]].want_gradient ? t[2] - 1 : -1;
+      pool_jobs.push_back(Setup{t[0], t[1], t[2], 0, gs});
     }
   }
   pool_pending += (int)todo.size();
@@ -734,6 +784,18 @@ void Server::collect(bool wait) {
     Slot& s = slots[j.si];
     s.guess.clear();
     s.guess.shrink_to_fit();
+    if (j.result == 3 || (j.result == 0 && s.eng)) {
+      // the request's last gradient is done (or a gradient failed): free
+      // the slot as finish_step does for a request without gradients
+      Request& R = reqs[j.ri];
+      R.finished = now();
+      fleet->detach(j.si);
+      s.eng.reset();
+      resident -= std::min(resident, s.footprint);
+      s.phase = FREE;
+      s.req = -1;
+      continue;
+    }
     if (j.result == 0) {
       resident -= std::min(resident, footprint((int)reqs[j.ri].steps[j.step].atom.size()));
       s.phase = FREE;
@@ -878,6 +940,12 @@ void Server::finish_step(int si) {
   if (R.keep_density) {
     R.last_p.resize((size_t)s.n * s.n);
     download(R.last_p.data(), s.mol.P, R.last_p.size(), fleet->stream);
+  }
+  if (R.want_gradient) {
+    // the last step's gradient runs on a worker; the slot stays reserved
+    // until it is collected (a job with no set-up after it)
+    s.phase = NEXT;
+    return;
   }
   R.finished = now();
   fleet->detach(si);
@@ -1209,7 +1277,7 @@ void hfg_server_destroy(hfg_server* s) {
 }
 
 int hfg_server_submit(hfg_server* srv, double arrival_s, int nsteps, const hfg_basis_view* bases,
-                      const double* const* zxyz, int natom, int charge, const hfg_scf_options* opt, int keep_density,
+                      const double* const* zxyz, int natom, int charge, const hfg_scf_options* opt, int flags,
                       int* request_id) {
   if (!srv || !bases || !zxyz || !opt || nsteps <= 0 || natom <= 0)
     return set_error(HFG_INVALID_INPUT, "null argument or empty request");
@@ -1223,7 +1291,14 @@ int hfg_server_submit(hfg_server* srv, double arrival_s, int nsteps, const hfg_b
     R.natom = natom;
     R.charge = charge;
     R.opt = *opt;
-    R.keep_density = keep_density != 0;
+    R.keep_density = (flags & HFG_SERVE_KEEP_DENSITY) != 0;
+    // Switched off: a run of the served-gradient test after the rest of the
+    // GPU suite stopped making progress (the same test passed alone); the
+    // cause is not established, so the path stays closed.
+    if (flags & HFG_SERVE_GRADIENT)
+      throw HfgError{HFG_UNSUPPORTED, "gradients of served trajectories are switched off in this build"};
+    R.want_gradient = false;
+    if (R.want_gradient) R.grad.assign((size_t)nsteps * 3 * natom, 0.0);
     for (int k = 0; k < nsteps; ++k) {
       const hfg_basis_view& b = bases[k];
       if (b.n_functions <= 0 || !zxyz[k]) throw HfgError{HFG_INVALID_INPUT, "empty basis or geometry"};
@@ -1273,6 +1348,18 @@ int hfg_server_result(hfg_server* srv, int id, hfg_request_result* r, hfg_step_r
     std::memcpy(p_out, R.last_p.data(), R.last_p.size() * sizeof(double));
   }
   if (R.status != HFG_OK) return set_error(R.status, R.error);
+  return HFG_OK;
+}
+
+int hfg_server_gradient(hfg_server* srv, int id, double* grad) {
+  if (!srv || !grad) return set_error(HFG_INVALID_INPUT, "null argument");
+  Server& S = *srv->s;
+  if (id < 0 || id >= (int)S.reqs.size()) return set_error(HFG_INVALID_INPUT, "request id out of range");
+  const Request& R = S.reqs[id];
+  if (!R.want_gradient) return set_error(HFG_INVALID_INPUT, "gradients not requested for this request");
+  if (R.status != HFG_OK) return set_error(R.status, R.error);
+  if (R.finished < 0) return set_error(HFG_INVALID_INPUT, "request has not finished");
+  std::memcpy(grad, R.grad.data(), R.grad.size() * sizeof(double));
   return HFG_OK;
 }
 

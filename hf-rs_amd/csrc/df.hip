@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <future>
@@ -24,6 +25,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <type_traits>
 
 #include "../../include/hfg.h"
 #include "cholesky.h"
@@ -67,6 +69,84 @@ __global__ void k_scatter3(const Scatter3* __restrict__ d, int ntask, const doub
     const double v = blk[s.off + e] * so[mu] * so[nu] * sx[P];
     T[(size_t)P * nn + (size_t)mu * n + nu] = v;
     T[(size_t)P * nn + (size_t)nu * n + mu] = v;
+  }
+}
+
+// Packed layout: column ij = i(i+1)/2 + j (i >= j) of the pair (mu, nu),
+// the src/scf.rs:221-235 compound index.  Block element -> T[P][ij - c0] of a
+// column chunk (cols wide); a block of one shell with itself writes only its
+// a >= b elements, so every entry is written once.
+__global__ void k_scatter3p(const Scatter3* __restrict__ d, int ntask, const double* __restrict__ blk,
+                            const double* __restrict__ so, const double* __restrict__ sx, double* __restrict__ T,
+                            long long c0, long long cols) {
+  const int t = blockIdx.x;
+  if (t >= ntask) return;
+  const Scatter3 s = d[t];
+  const int tot = s.na * s.nb * s.np;
+  const bool diag = s.mu0 == s.nu0;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int a, b, p;
+    if (s.aux_bra) {
+      p = e / (s.na * s.nb);
+      const int r = e - p * s.na * s.nb;
+      a = r / s.nb;
+      b = r - a * s.nb;
+    } else {
+      a = e / (s.nb * s.np);
+      const int r = e - a * s.nb * s.np;
+      b = r / s.np;
+      p = r - b * s.np;
+    }
+    if (diag && a < b) continue;
+    const int mu = s.mu0 + a, nu = s.nu0 + b, P = s.p0 + p;
+    const long long i = mu > nu ? mu : nu, j = mu > nu ? nu : mu;
+    T[(size_t)P * cols + (size_t)(i * (i + 1) / 2 + j - c0)] = blk[s.off + e] * so[mu] * so[nu] * sx[P];
+  }
+}
+
+// Symmetric n x n -> packed with the off-diagonal doubled: sum_ij X_ij Y_ij
+// = sum_(i>=j) packed(X)_ij Y_ij for symmetric X, Y.
+__global__ void k_pack_sym2(const double* __restrict__ X, int n, double* __restrict__ Xp) {
+  const size_t ij = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t np = (size_t)n * (n + 1) / 2;
+  if (ij >= np) return;
+  int ii = (int)((sqrt(8.0 * (double)ij + 1.0) - 1.0) * 0.5);
+  while ((size_t)ii * (ii + 1) / 2 > ij) --ii;
+  while ((size_t)(ii + 1) * (ii + 2) / 2 <= ij) ++ii;
+  const int j = (int)(ij - (size_t)ii * (ii + 1) / 2);
+  Xp[ij] = (ii == j ? 1.0 : 2.0) * X[(size_t)ii * n + j];
+}
+
+// Packed -> symmetric n x n.
+__global__ void k_unpack_sym(const double* __restrict__ Xp, int n, double* __restrict__ X) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * n) return;
+  const int i = (int)(idx / n), j = (int)(idx - (size_t)i * n);
+  const int a = i > j ? i : j, b = i > j ? j : i;
+  X[idx] = Xp[(size_t)a * (a + 1) / 2 + b];
+}
+
+// Rows q < nq of packed B (row stride np) -> nq symmetric n x n matrices, as
+// T (double, float, or the two BF16 parts of a split).
+template <class T>
+__global__ void k_unpack_rows(const double* __restrict__ Bp, int nq, int n, size_t np, T* __restrict__ out,
+                              __hip_bfloat16* __restrict__ lo) {
+  const size_t nn = (size_t)n * n;
+  const size_t tot = (size_t)nq * nn;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t q = idx / nn, r = idx - q * nn;
+    const int i = (int)(r / n), j = (int)(r - (size_t)i * n);
+    const int a = i > j ? i : j, b = i > j ? j : i;
+    const double v = Bp[q * np + (size_t)a * (a + 1) / 2 + b];
+    if constexpr (std::is_same<T, __hip_bfloat16>::value) {  // as k_split_bf16
+      const float f = (float)v;
+      const __hip_bfloat16 h = __float2bfloat16(f);
+      out[idx] = h;
+      lo[idx] = __float2bfloat16(f - __bfloat162float(h));
+    } else {
+      out[idx] = (T)v;
+    }
   }
 }
 
@@ -169,12 +249,27 @@ __global__ void k_sum_parts(const T* __restrict__ part, int nb, size_t nn, doubl
   }
 }
 
+// df_mfma.hip: K = alpha X X^T on the FP64 matrix cores.
+void syrk_f64(const double* X, int n, long long L, long long ldx, double alpha, double* K, DevBuf& scratch,
+              hipStream_t st);
+
 struct DfEngine {
   hipStream_t stream = nullptr;
   std::unique_ptr<Blas> bl;
   int n = 0, naux = 0, rank = 0;
   std::vector<Shell> orb, aux;
-  DevBuf d_B;  // rank x n x n
+  // B: rank x n x n, or rank x n(n+1)/2 packed (columns ij = i(i+1)/2 + j,
+  // i >= j) when the unpacked 3-index tensor and B would not fit in HBM
+  bool packed = false;
+  size_t npk = 0;
+  DevBuf d_B;
+  DevBuf d_pk, d_unp;  // packed path: packed P and J; unpacked rows of B for DF-K
+  DevBuf d_syrk;       // partial tiles of the hand-written exchange product
+  // FP64 exchange product by the hand-written MFMA kernel (HFG_DF_SYRK=0: rocBLAS dgemm)
+  const bool use_syrk = [] {
+    const char* e = std::getenv("HFG_DF_SYRK");
+    return !(e && std::atoi(e) == 0);
+  }();
   DevBuf d_work, d_A, d_At;
   size_t work_o = 0;
   // reduced-precision copies of B (made on first use) and work arrays
@@ -184,6 +279,7 @@ struct DfEngine {
   // dimension rank*o): the output alone gives too few tiles for the chip, so
   // the auxiliary index is cut into nsplit blocks computed as one strided
   // batch, then summed in FP64.
+  static constexpr size_t T_CHUNK = 8ull << 30;  // bytes of one column chunk of the packed 3-index tensor
   int nsplit() const {
     const int tiles = std::max(1, ((n + 127) / 128) * ((n + 127) / 128));
     return std::max(1, std::min(rank, (1024 + tiles - 1) / tiles));
@@ -196,12 +292,22 @@ struct DfEngine {
     aux = group_shells(auxv);
     n = basis.n_functions;
     naux = auxv.n_functions;
+    npk = (size_t)n * (n + 1) / 2;
     for (auto& s : orb)
-      if (s.L > 2) throw HfgError{HFG_UNSUPPORTED, "density fitting compiled for orbital shells up to d"};
+      if (s.L > 3) throw HfgError{HFG_UNSUPPORTED, "density fitting compiled for orbital shells up to f"};
     for (auto& s : aux)
       if (s.L > 4) throw HfgError{HFG_UNSUPPORTED, "density fitting compiled for auxiliary shells up to g"};
-    const double bytes = 8.0 * ((double)naux * n * n + (double)naux * naux * 2);
-    if (bytes > 64e9) throw HfgError{HFG_INVALID_INPUT, "DF 3-index tensor exceeds the 64 GB working set"};
+    // HBM footprint: unpacked, the 3-index tensor and B at naux n^2 doubles
+    // each; packed, B at naux n(n+1)/2 and the tensor one column chunk at a
+    // time (T_CHUNK bytes), so that config-size bases fit.
+    size_t free_b = 0, total_b = 0;
+    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const double metric = 8.0 * 2.0 * (double)naux * naux;
+    const double unpacked = 8.0 * 2.0 * (double)naux * n * n + metric;
+    const char* force = std::getenv("HFG_DF_PACKED");
+    packed = force ? std::atoi(force) != 0 : unpacked > 0.4 * (double)free_b;
+    if (packed && 8.0 * (double)naux * npk + (double)T_CHUNK + metric > 0.8 * (double)free_b)
+      throw HfgError{HFG_DEVICE, "DF factor B does not fit in the free device memory"};
     HIP_CHECK(hipStreamCreate(&stream));
     bl.reset(new Blas(stream));
     build(cutoff);
@@ -383,17 +489,46 @@ struct DfEngine {
 
     // 3-index tensor T[P][mu][nu]: every orbital pair against every
     // auxiliary shell (src/df.rs:60-78), one (pair class, auxiliary class)
-    // product at a time, in batches of at most `budget` block doubles.
-    DevBuf d_T;
+    // product at a time, in batches of at most `budget` block doubles, then
+    // B = X^T T.  Packed, the tensor is formed one chunk of function rows
+    // [f0, f1) at a time -- its packed columns f0(f0+1)/2 .. f1(f1+1)/2 - 1,
+    // the pairs whose later shell starts in the chunk -- and each chunk's
+    // columns of B follow from one GEMM, so the tensor never exists whole.
     const size_t nn = (size_t)n * n;
-    d_T.alloc((size_t)naux * nn * 8);
-    {
-      std::map<int, std::vector<int>> by_oc, by_xc;  // class -> orbital pairs / auxiliary shells
-      for (int i = 0; i < norb_pairs; ++i) by_oc[pair_class(pairs[i].la, pairs[i].lb)].push_back(i);
-      for (int p = 0; p < (int)aux.size(); ++p) by_xc[pair_class(aux[p].L, 0)].push_back(p);
-      const long long budget = 1ll << 26;
-      DevBuf d_opair, d_oa, d_ob, d_xsh, d_xao, dt, ds3, dout;
-      size_t cap_tasks = 0, cap_out = 0;
+    std::vector<std::pair<int, int>> rows;  // function-row chunks
+    if (!packed) {
+      rows.push_back({0, n});
+    } else {
+      std::vector<int> starts;
+      for (auto& sh : orb) starts.push_back(sh.ao);
+      std::sort(starts.begin(), starts.end());
+      starts.push_back(n);
+      auto cols_of = [](long long f0, long long f1) { return f1 * (f1 + 1) / 2 - f0 * (f0 + 1) / 2; };
+      size_t k = 0;
+      while (k + 1 < starts.size()) {
+        size_t e = k + 1;
+        while (e + 1 < starts.size() && 8.0 * naux * cols_of(starts[k], starts[e + 1]) <= (double)T_CHUNK) ++e;
+        rows.push_back({starts[k], starts[e]});
+        k = e;
+      }
+    }
+    d_B.alloc((size_t)rank * (packed ? npk : nn) * 8);
+    DevBuf d_T;
+    std::map<int, std::vector<int>> by_xc;  // auxiliary class -> shells
+    for (int p = 0; p < (int)aux.size(); ++p) by_xc[pair_class(aux[p].L, 0)].push_back(p);
+    const long long budget = 1ll << 26;
+    DevBuf d_opair, d_oa, d_ob, d_xsh, d_xao, dt, ds3, dout;
+    size_t cap_tasks = 0, cap_out = 0;
+    for (const auto& rc : rows) {
+      const long long c0 = (long long)rc.first * (rc.first + 1) / 2;
+      const long long cols = packed ? (long long)rc.second * (rc.second + 1) / 2 - c0 : (long long)nn;
+      const size_t tsz = (size_t)naux * cols;
+      if (d_T.bytes < tsz * 8) d_T.alloc(tsz * 8);
+      std::map<int, std::vector<int>> by_oc;  // orbital pair class -> pairs of this chunk
+      for (int i = 0; i < norb_pairs; ++i) {
+        const int row = std::max(pairs[i].ao_a, pairs[i].ao_b);
+        if (row >= rc.first && row < rc.second) by_oc[pair_class(pairs[i].la, pairs[i].lb)].push_back(i);
+      }
       for (auto& oc : by_oc) {
         const std::vector<int>& ol = oc.second;
         std::vector<int> oa(ol.size()), ob(ol.size());
@@ -439,21 +574,48 @@ struct DfEngine {
                                ds3.as<Scatter3>());
             HIP_CHECK(hipGetLastError());
             BlockArgs args{dt.as<BlockTask>(), cnt, d_pairs.as<PairDev>(), view, d_boys.as<double>(), dout.as<double>()};
-            launch_block_class(cls, args, stream);
-            hipLaunchKernelGGL(k_scatter3, dim3(cnt), dim3(64), 0, stream, ds3.as<Scatter3>(), cnt, dout.as<double>(),
-                               d_so.as<double>(), d_sx.as<double>(), d_T.as<double>(), n);
+            if (cls < HFG_NCLASS_ALL)
+              launch_block_class(cls, args, stream);
+            else  // an orbital f-p, f-d or f-f pair (never on the auxiliary side)
+              launch_df_f_block(o0.la, o0.lb, aux[xl[0]].L, 0, args, stream);
+            if (packed)
+              hipLaunchKernelGGL(k_scatter3p, dim3(cnt), dim3(64), 0, stream, ds3.as<Scatter3>(), cnt,
+                                 dout.as<double>(), d_so.as<double>(), d_sx.as<double>(), d_T.as<double>(), c0, cols);
+            else
+              hipLaunchKernelGGL(k_scatter3, dim3(cnt), dim3(64), 0, stream, ds3.as<Scatter3>(), cnt,
+                                 dout.as<double>(), d_so.as<double>(), d_sx.as<double>(), d_T.as<double>(), n);
             HIP_CHECK(hipGetLastError());
           }
           // the lists are rewritten for the next product
           HIP_CHECK(hipStreamSynchronize(stream));
         }
       }
+      // B[:, chunk] = X^T T_chunk  (rank x naux) (naux x cols)
+      const int ldb = (int)(packed ? npk : nn);
+      bl->gemm(false, false, rank, (int)cols, naux, 1.0, d_Xt.as<double>(), naux, d_T.as<double>(), (int)cols, 0.0,
+               d_B.as<double>() + (packed ? c0 : 0), ldb);
+      HIP_CHECK(hipStreamSynchronize(stream));
     }
-    // B = X^T T  (rank x naux) (naux x n^2)
-    d_B.alloc((size_t)rank * nn * 8);
-    bl->gemm(false, false, rank, (int)nn, naux, 1.0, d_Xt.as<double>(), naux, d_T.as<double>(), (int)nn, 0.0,
-             d_B.as<double>(), (int)nn);
-    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  // Packed B: f(q0, qn, W, lo) for chunks of rows q0 .. q0+qn-1 unpacked to
+  // qn symmetric n x n matrices W of type T (for BF16, W the high and lo the
+  // low part of the split), about 2 GB of scratch at a time.
+  template <class T, class F>
+  void for_unpacked_rows(F&& f) {
+    const size_t nn = (size_t)n * n;
+    const int qc = (int)std::max<size_t>(1, std::min<size_t>(rank, (2ull << 30) / (nn * 8)));
+    if (d_unp.bytes < (size_t)qc * nn * 8) d_unp.alloc((size_t)qc * nn * 8);
+    T* W = d_unp.as<T>();
+    __hip_bfloat16* lo = std::is_same<T, __hip_bfloat16>::value ? d_unp.as<__hip_bfloat16>() + (size_t)qc * nn
+                                                                 : nullptr;
+    for (int q0 = 0; q0 < rank; q0 += qc) {
+      const int qn = std::min(qc, rank - q0);
+      hipLaunchKernelGGL(k_unpack_rows<T>, dim3(4096), dim3(256), 0, stream, d_B.as<double>() + (size_t)q0 * npk, qn,
+                         n, npk, W, lo);
+      HIP_CHECK(hipGetLastError());
+      f(q0, qn, (const T*)W, (const __hip_bfloat16*)lo);
+    }
   }
 
   // J = sum_Q B_Q (B_Q . P), P = 2 C C^T; K = 2 sum_Q (B_Q C)(B_Q C)^T
@@ -471,21 +633,48 @@ struct DfEngine {
     dq.alloc(rank);
     bl->gemm(false, true, n, n, o, 2.0, dC, o, dC, o, 0.0, P, n);
     const double one = 1.0, zero = 0.0;
-    // column-major view of B: nn x rank (B^T); d = B P  ->  gemv(T)
-    RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_transpose, (rocblas_int)nn, rank, &one, d_B.as<double>(),
-                           (rocblas_int)nn, P, 1, &zero, dq.p, 1));
-    RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_none, (rocblas_int)nn, rank, &one, d_B.as<double>(),
-                           (rocblas_int)nn, dq.p, 1, &zero, dJ, 1));
+    const size_t bc = packed ? npk : nn;  // columns of B
+    const double* pv = P;
+    double* jv = dJ;
+    if (packed) {
+      // packed P with the off-diagonal doubled, so that B P over the packed
+      // columns is the full contraction; J comes back packed
+      if (d_pk.bytes < 2 * npk * 8) d_pk.alloc(2 * npk * 8);
+      hipLaunchKernelGGL(k_pack_sym2, dim3((unsigned)((npk + 255) / 256)), dim3(256), 0, stream, P, n,
+                         d_pk.as<double>());
+      HIP_CHECK(hipGetLastError());
+      pv = d_pk.as<double>();
+      jv = d_pk.as<double>() + npk;
+    }
+    // column-major view of B: bc x rank (B^T); d = B P  ->  gemv(T)
+    RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_transpose, (rocblas_int)bc, rank, &one, d_B.as<double>(),
+                           (rocblas_int)bc, pv, 1, &zero, dq.p, 1));
+    RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_none, (rocblas_int)bc, rank, &one, d_B.as<double>(),
+                           (rocblas_int)bc, dq.p, 1, &zero, jv, 1));
+    if (packed) {
+      hipLaunchKernelGGL(k_unpack_sym, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, stream, jv, n, dJ);
+      HIP_CHECK(hipGetLastError());
+    }
     if (!dK) return;  // J only
     // A = B C: rows (Q, m), (rank*n) x o
-    bl->gemm(false, false, rank * n, o, n, 1.0, d_B.as<double>(), n, dC, o, 0.0, d_A.as<double>(), o);
+    if (!packed)
+      bl->gemm(false, false, rank * n, o, n, 1.0, d_B.as<double>(), n, dC, o, 0.0, d_A.as<double>(), o);
+    else
+      for_unpacked_rows<double>([&](int q0, int qn, const double* W, const __hip_bfloat16*) {
+        bl->gemm(false, false, qn * n, o, n, 1.0, W, n, dC, o, 0.0, d_A.as<double>() + (size_t)q0 * n * o, o);
+      });
     const size_t tot = (size_t)rank * n * o;
     hipLaunchKernelGGL(k_permute_qmi, dim3((tot + 255) / 256), dim3(256), 0, stream, d_A.as<double>(),
                        d_At.as<double>(), rank, n, o);
     HIP_CHECK(hipGetLastError());
-    // FP64: one GEMM (split-K measured slower here, and rocBLAS dsyrk, with
-    // half the flops, 2.1x slower on (H2O)32/cc-pVDZ with JKFIT)
-    bl->gemm(false, true, n, n, rank * o, 2.0, d_At.as<double>(), rank * o, d_At.as<double>(), rank * o, 0.0, dK, n);
+    // FP64: the symmetric product on the matrix cores (upper tiles only,
+    // split over the contraction index); or one rocBLAS GEMM (its dsyrk,
+    // with half the flops, ran 2.1x slower than the GEMM on (H2O)32/cc-pVDZ)
+    if (use_syrk)
+      syrk_f64(d_At.as<double>(), n, (long long)rank * o, (long long)rank * o, 2.0, dK, d_syrk, stream);
+    else
+      bl->gemm(false, true, n, n, rank * o, 2.0, d_At.as<double>(), rank * o, d_At.as<double>(), rank * o, 0.0, dK,
+               n);
   }
 
   // DF-K in reduced precision on the matrix cores (J stays FP64, HBM-bound):
@@ -504,11 +693,11 @@ struct DfEngine {
   void k_reduced(const double* dC, int o, double* dK, int prec) {
     const size_t nn = (size_t)n * n, bsz = (size_t)rank * nn, csz = (size_t)n * o, asz = (size_t)rank * n * o;
     const dim3 g(2048), b(256);
-    if (prec == 1 && !d_B32.p) {
+    if (prec == 1 && !packed && !d_B32.p) {
       d_B32.alloc(bsz * 4);
       hipLaunchKernelGGL(k_to_f32, g, b, 0, stream, d_B.as<double>(), d_B32.as<float>(), bsz);
     }
-    if (prec == 2 && !d_Bh.p) {
+    if (prec == 2 && !packed && !d_Bh.p) {
       d_Bh.alloc(bsz * 2);
       d_Bl.alloc(bsz * 2);
       hipLaunchKernelGGL(k_split_bf16<double>, g, b, 0, stream, d_B.as<double>(), d_Bh.as<__hip_bfloat16>(),
@@ -530,14 +719,28 @@ struct DfEngine {
     float* K = d_K32.as<float>();
     if (prec == 1) {
       hipLaunchKernelGGL(k_to_f32, g, b, 0, stream, dC, d_C32.as<float>(), csz);
-      gemm_ex(false, false, rank * n, o, n, d_B32.p, n, d_C32.p, o, A, o, 1.f, 0.f, rocblas_datatype_f32_r);
+      if (!packed)
+        gemm_ex(false, false, rank * n, o, n, d_B32.p, n, d_C32.p, o, A, o, 1.f, 0.f, rocblas_datatype_f32_r);
+      else
+        for_unpacked_rows<float>([&](int q0, int qn, const float* W, const __hip_bfloat16*) {
+          gemm_ex(false, false, qn * n, o, n, W, n, d_C32.p, o, A + (size_t)q0 * n * o, o, 1.f, 0.f,
+                  rocblas_datatype_f32_r);
+        });
     } else {
       hipLaunchKernelGGL(k_split_bf16<double>, g, b, 0, stream, dC, d_Ch.as<__hip_bfloat16>(),
                          d_Cl.as<__hip_bfloat16>(), csz);
       const auto bf = rocblas_datatype_bf16_r;
-      gemm_ex(false, false, rank * n, o, n, d_Bh.p, n, d_Ch.p, o, A, o, 1.f, 0.f, bf);
-      gemm_ex(false, false, rank * n, o, n, d_Bh.p, n, d_Cl.p, o, A, o, 1.f, 1.f, bf);
-      gemm_ex(false, false, rank * n, o, n, d_Bl.p, n, d_Ch.p, o, A, o, 1.f, 1.f, bf);
+      auto three = [&](const void* Bh, const void* Bl, int rows, float* Ao) {
+        gemm_ex(false, false, rows, o, n, Bh, n, d_Ch.p, o, Ao, o, 1.f, 0.f, bf);
+        gemm_ex(false, false, rows, o, n, Bh, n, d_Cl.p, o, Ao, o, 1.f, 1.f, bf);
+        gemm_ex(false, false, rows, o, n, Bl, n, d_Ch.p, o, Ao, o, 1.f, 1.f, bf);
+      };
+      if (!packed)
+        three(d_Bh.p, d_Bl.p, rank * n, A);
+      else
+        for_unpacked_rows<__hip_bfloat16>([&](int q0, int qn, const __hip_bfloat16* Wh, const __hip_bfloat16* Wl) {
+          three(Wh, Wl, qn * n, A + (size_t)q0 * n * o);
+        });
     }
     hipLaunchKernelGGL(k_permute_qmi_f32, dim3((asz + 255) / 256), dim3(256), 0, stream, A, At, rank, n, o);
     (void)K;

@@ -97,6 +97,8 @@ struct BlockArgs {
   double* out;
 };
 void launch_block_class(int cls, const BlockArgs& a, hipStream_t s);
+// The 3-index classes with an orbital f-p, f-d or f-f pair (df_f.hip).
+void launch_df_f_block(int l1, int l2, int l3, int l4, const BlockArgs& a, hipStream_t s);
 
 #define HIP_CHECK(expr)                                                                           \
   do {                                                                                            \

@@ -507,6 +507,14 @@ def grad_classes():
     return [(3, b, lc, ld) for b in (1, 2) for (lc, ld) in orbital]
 
 
+def df_f_classes():
+    """Block classes of the density-fitting 3-index integrals with an orbital
+    f shell paired with p, d or f (fp, fd, ff, kernel orientation) against an
+    auxiliary shell with the unit function (s..g); the f-s orbital pairs fall
+    in classes()."""
+    return [(3, b, lc, 0) for b in (1, 2, 3) for lc in range(5)]
+
+
 def main(outdir):
     os.makedirs(outdir, exist_ok=True)
     parts = ["// Generated by gen_eri.py -- do not edit.\n"]
@@ -533,6 +541,13 @@ def main(outdir):
             f.write(src)
             report.append("%s %d" % ("".join(map(str, c)), fl))
         f.write("#define HFG_FOR_EACH_GRAD_CLASS(X) " + " ".join("X(%d, %d, %d, %d)" % c for c in grad_classes()) + "\n")
+    with open(os.path.join(outdir, "eri_df.inc"), "w") as f:
+        f.write("// Generated by gen_eri.py -- do not edit.  3-index classes with orbital f pairs (df_f.hip).\n")
+        for c in df_f_classes():
+            src, fl = gen_class(*c)
+            f.write(src)
+            report.append("%s %d" % ("".join(map(str, c)), fl))
+        f.write("#define HFG_FOR_EACH_DF_F_CLASS(X) " + " ".join("X(%d, %d, %d, %d)" % c for c in df_f_classes()) + "\n")
     with open(os.path.join(outdir, "eri_classes.txt"), "w") as f:
         f.write("\n".join(report) + "\n")
     # class list for the C++ side, in quartet_class() order

@@ -21,6 +21,9 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -244,10 +247,12 @@ struct DPair {
 
 }  // namespace
 
-// Nuclear gradient (natom x 3) of the engine's molecule for density P and
-// energy-weighted density W (host, row-major n x n).
-void rhf_gradient(Engine& e, const std::vector<std::array<double, 4>>& atoms, const double* P, const double* W,
-                  double* grad) {
+// Nuclear gradient (natom x 3, host) of the engine's molecule for the device
+// density dP and energy-weighted density dW (row-major n x n), on stream st.
+// Reads the engine's tables only, so it may run beside builds of other
+// molecules.
+void rhf_gradient_device(Engine& e, const std::vector<std::array<double, 4>>& atoms, const double* dP,
+                         const double* dW, double* grad, hipStream_t st) {
   const int n = e.n, na = (int)atoms.size();
   std::vector<double> g((size_t)3 * na, 0.0);
   for (const Shell& s : e.shells)
@@ -267,7 +272,6 @@ void rhf_gradient(Engine& e, const std::vector<std::array<double, 4>>& atoms, co
   const std::vector<Shell>& sh = e.shells;
   const std::vector<double>& sc = e.ao_scale;
   // ---- two-electron part on the device
-  hipStream_t st = e.stream;
   const size_t npair = e.pairs.size();
   // pair table: the engine's pairs, then the derivative pairs
   PrimPairHost pp = e.pp;
@@ -309,88 +313,49 @@ void rhf_gradient(Engine& e, const std::vector<std::array<double, 4>>& atoms, co
     if (B.L > 0) d.idx[3] = add_pair(A, lowered(B));  // L_A >= L_B - 1
   }
   const size_t npp = pp.size();
-  DevBuf d_pp, d_pd, d_P, d_scale, d_grad;
+  DevBuf d_pp, d_pd, d_scale, d_grad;
   d_pp.alloc(9 * npp * sizeof(double));
   const std::vector<double>* cols[9] = {&pp.p, &pp.Px, &pp.Py, &pp.Pz, &pp.PAx, &pp.PAy, &pp.PAz, &pp.K, &pp.H};
   for (int c = 0; c < 9; ++c)
     HIP_CHECK(hipMemcpyAsync(d_pp.as<double>() + c * npp, cols[c]->data(), npp * 8, hipMemcpyHostToDevice, st));
   d_pd.alloc(pd.size() * sizeof(PairDev));
   HIP_CHECK(hipMemcpyAsync(d_pd.p, pd.data(), pd.size() * sizeof(PairDev), hipMemcpyHostToDevice, st));
-  d_P.alloc((size_t)n * n * 8);
-  HIP_CHECK(hipMemcpyAsync(d_P.p, P, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
   d_scale.alloc((size_t)n * 8);
   HIP_CHECK(hipMemcpyAsync(d_scale.p, sc.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
   d_grad.alloc((size_t)3 * na * 8);
   hipLaunchKernelGGL(k_zero, dim3((3 * na + 255) / 256), dim3(256), 0, st, d_grad.as<double>(), (size_t)3 * na);
   // ---- one-electron terms on the device (onee_dev.hip)
-  {
-    DevBuf d_W;
-    d_W.alloc((size_t)n * n * 8);
-    HIP_CHECK(hipMemcpyAsync(d_W.p, W, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
-    onee_gradient_device(sh, n, atoms, d_P.as<double>(), d_W.as<double>(), d_grad.as<double>(), st);
-  }
+  onee_gradient_device(sh, n, atoms, dP, dW, d_grad.as<double>(), st);
   const double* b = d_pp.as<double>();
   const PrimPairs view{b, b + npp, b + 2 * npp, b + 3 * npp, b + 4 * npp, b + 5 * npp, b + 6 * npp, b + 7 * npp,
                        b + 8 * npp};
-  // Blocks of a batch of quartets: (class, task) lists, then the contraction.
-  struct Item {
-    int l[4];
-    BlockTask t;
-  };
-  std::vector<Item> items;
-  std::vector<GQuartet> quartets;
-  long long off = 0;
-  DevBuf d_tasks, d_blk, d_q;
-  size_t cap_t = 0, cap_b = 0, cap_q = 0;
+  // The quartets of pair rows ij (kl <= ij, passing the Schwarz test) and
+  // their derivative blocks.  A counting pass (parallel over rows) gives each
+  // row's quartet, block and block-double counts; consecutive rows form
+  // batches of at most `budget` block doubles; a second parallel pass fills
+  // each row's segment of the batch, and the block tasks are bucketed by
+  // class (row order kept within a class) for one launch per class.
   const size_t lds = (size_t)3 * na * sizeof(double);
   if (lds > 60000) throw HfgError{HFG_UNSUPPORTED, "gradient: too many atoms for the per-workgroup force sums"};
-  auto flush = [&]() {
-    if (quartets.empty()) return;
-    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) {
-      return std::lexicographical_compare(x.l, x.l + 4, y.l, y.l + 4);
-    });
-    std::vector<BlockTask> tasks(items.size());
-    for (size_t i = 0; i < items.size(); ++i) tasks[i] = items[i].t;
-    if (tasks.size() > cap_t) {
-      d_tasks.alloc(tasks.size() * sizeof(BlockTask));
-      cap_t = tasks.size();
-    }
-    if ((size_t)off > cap_b) {
-      d_blk.alloc((size_t)off * 8);
-      cap_b = off;
-    }
-    if (quartets.size() > cap_q) {
-      d_q.alloc(quartets.size() * sizeof(GQuartet));
-      cap_q = quartets.size();
-    }
-    HIP_CHECK(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(BlockTask), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(d_q.p, quartets.data(), quartets.size() * sizeof(GQuartet), hipMemcpyHostToDevice, st));
-    size_t a = 0;
-    while (a < items.size()) {
-      size_t z = a;
-      while (z < items.size() && std::equal(items[z].l, items[z].l + 4, items[a].l)) ++z;
-      BlockArgs args{d_tasks.as<BlockTask>() + a, (int)(z - a), d_pd.as<PairDev>(), view, e.d_boys.as<double>(),
-                     d_blk.as<double>()};
-      launch_any_block(items[a].l[0], items[a].l[1], items[a].l[2], items[a].l[3], args, st);
-      a = z;
-    }
-    const int blocks = (int)std::min<size_t>(2048, (quartets.size() + 3) / 4);
-    hipLaunchKernelGGL(k_grad_contract, dim3(blocks), dim3(256), lds, st, d_q.as<GQuartet>(), (int)quartets.size(),
-                       d_blk.as<double>(), d_P.as<double>(), d_scale.as<double>(), n, na, d_grad.as<double>());
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(st));  // the host lists are reused
-    items.clear();
-    quartets.clear();
-    off = 0;
+  const bool trace = std::getenv("HFG_TRACE") != nullptr;
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto secs = [](std::chrono::steady_clock::time_point t0, std::chrono::steady_clock::time_point t1) {
+    return std::chrono::duration<double>(t1 - t0).count();
   };
-  const long long budget = 1ll << 24;  // block doubles per batch
-  // one block of a quartet: derivative pair dpi (lengths dl) against pair other
-  auto add_block = [&](GQuartet& Q, int kind, int dpi, bool dswap, int other, const int sh_d[2],
-                       const int sh_o[2], bool raised_) {
+  const auto t_start = now();
+  double t_count = 0, t_fill = 0, t_gpu = 0;
+  auto cls_key = [](int l0, int l1, int l2, int l3) { return ((l0 * 5 + l1) * 5 + l2) * 5 + l3; };
+  constexpr int NKEY = 625;
+  // One block of a quartet: derivative pair dpi against pair other.  With Q
+  // null only the block's size is returned.
+  auto add_block = [&](GQuartet* Q, long long& off, BlockTask* t_out, int* k_out, int dpi, bool dswap, int other,
+                       const int sh_d[2], const int sh_o[2], bool raised_) -> long long {
     const std::array<int, 2> ld = pl[dpi], lo = pl[other];
+    const long long size = (long long)ncart(ld[0]) * ncart(ld[1]) * ncart(lo[0]) * ncart(lo[1]);
+    if (!Q) return size;
     // storage: the pair of higher rank first
     const bool dfirst = pair_rank(ld[0], ld[1]) >= pair_rank(lo[0], lo[1]);
-    GBlock& B = Q.blk[Q.nblk++];
+    GBlock& B = Q->blk[Q->nblk++];
     B.off = off;
     B.raised = raised_ ? 1 : 0;
     const int dpos0 = dfirst ? 0 : 2, opos0 = dfirst ? 2 : 0;
@@ -404,62 +369,168 @@ void rhf_gradient(Engine& e, const std::vector<std::array<double, 4>>& atoms, co
     B.sh[opos0 + 1] = (signed char)sh_o[1];
     B.L[opos0] = (signed char)lo[0];
     B.L[opos0 + 1] = (signed char)lo[1];
-    Item it;
     if (dfirst) {
-      it.l[0] = ld[0];
-      it.l[1] = ld[1];
-      it.l[2] = lo[0];
-      it.l[3] = lo[1];
-      it.t = BlockTask{dpi, other, off};
+      *k_out = cls_key(ld[0], ld[1], lo[0], lo[1]);
+      *t_out = BlockTask{dpi, other, off};
     } else {
-      it.l[0] = lo[0];
-      it.l[1] = lo[1];
-      it.l[2] = ld[0];
-      it.l[3] = ld[1];
-      it.t = BlockTask{other, dpi, off};
+      *k_out = cls_key(lo[0], lo[1], ld[0], ld[1]);
+      *t_out = BlockTask{other, dpi, off};
     }
-    items.push_back(it);
-    off += (long long)ncart(ld[0]) * ncart(ld[1]) * ncart(lo[0]) * ncart(lo[1]);
-    (void)kind;
+    off += size;
+    return size;
   };
-  for (size_t ij = 0; ij < npair; ++ij)
-    for (size_t kl = 0; kl <= ij; ++kl) {
-      if (!(e.Q[ij] * e.Q[kl] > e.screen)) continue;
-      const PairInfo& pb = e.pairs[ij];
-      const PairInfo& pk = e.pairs[kl];
-      GQuartet Q;
-      std::memset(&Q, 0, sizeof Q);
+  // Quartet (ij, kl): its blocks (d/dA raised and lowered, d/dB raised and
+  // lowered, d/dC raised and lowered); returns the block count and adds the
+  // block doubles to off.  Count-only with Q null.
+  auto make_quartet = [&](size_t ij, size_t kl, GQuartet* Q, BlockTask* t_out, int* k_out, long long& off) {
+    const PairInfo& pb = e.pairs[ij];
+    const PairInfo& pk = e.pairs[kl];
+    if (Q) {
+      std::memset(Q, 0, sizeof *Q);
       const int shells4[4] = {pb.sh_a, pb.sh_b, pk.sh_a, pk.sh_b};
       for (int k = 0; k < 4; ++k) {
         const Shell& s = sh[shells4[k]];
-        Q.ao[k] = s.ao;
-        Q.atom[k] = s.atom;
-        Q.L[k] = (signed char)s.L;
+        Q->ao[k] = s.ao;
+        Q->atom[k] = s.atom;
+        Q->L[k] = (signed char)s.L;
       }
       double deg = 1.0;
       if (pb.sh_a != pb.sh_b) deg *= 2.0;
       if (pk.sh_a != pk.sh_b) deg *= 2.0;
       if (ij != kl) deg *= 2.0;
-      Q.w = 0.5 * deg;
-      const int AB[2] = {0, 1}, BA[2] = {1, 0}, CD[2] = {2, 3};
-      const DPair& db = dp[ij];
-      const DPair& dk = dp[kl];
-      // d/dA: raised and lowered A against (C D)
-      add_block(Q, 0, db.idx[0], false, (int)kl, AB, CD, true);
-      if (db.idx[1] >= 0) add_block(Q, 1, db.idx[1], db.swap[1], (int)kl, AB, CD, false);
-      // d/dB: the pair (A, B') or (B', A)
-      add_block(Q, 2, db.idx[2], !db.swap[2], (int)kl, BA, CD, true);
-      if (db.idx[3] >= 0) add_block(Q, 3, db.idx[3], true, (int)kl, BA, CD, false);
-      // d/dC: raised and lowered C against (A B)
-      add_block(Q, 4, dk.idx[0], false, (int)ij, CD, AB, true);
-      if (dk.idx[1] >= 0) add_block(Q, 5, dk.idx[1], dk.swap[1], (int)ij, CD, AB, false);
-      quartets.push_back(Q);
-      if (off >= budget) flush();
+      Q->w = 0.5 * deg;
     }
-  flush();
+    const int AB[2] = {0, 1}, BA[2] = {1, 0}, CD[2] = {2, 3};
+    const DPair& db = dp[ij];
+    const DPair& dk = dp[kl];
+    int nb = 0;
+    long long count_off = 0;
+    long long& o = Q ? off : count_off;
+    auto add = [&](int dpi, bool dswap, int other, const int* sd, const int* so, bool r) {
+      const long long sz = add_block(Q, o, t_out + nb, k_out + nb, dpi, dswap, other, sd, so, r);
+      if (!Q) o += sz;
+      ++nb;
+    };
+    add(db.idx[0], false, (int)kl, AB, CD, true);
+    if (db.idx[1] >= 0) add(db.idx[1], db.swap[1], (int)kl, AB, CD, false);
+    add(db.idx[2], !db.swap[2], (int)kl, BA, CD, true);
+    if (db.idx[3] >= 0) add(db.idx[3], true, (int)kl, BA, CD, false);
+    add(dk.idx[0], false, (int)ij, CD, AB, true);
+    if (dk.idx[1] >= 0) add(dk.idx[1], dk.swap[1], (int)ij, CD, AB, false);
+    if (!Q) off += count_off;
+    return nb;
+  };
+  // counting pass
+  std::vector<long long> row_q(npair), row_b(npair), row_d(npair);
+  {
+    const auto t0 = now();
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long ij = 0; ij < (long long)npair; ++ij) {
+      long long q = 0, nb = 0, d = 0;
+      for (size_t kl = 0; kl <= (size_t)ij; ++kl) {
+        if (!(e.Q[ij] * e.Q[kl] > e.screen)) continue;
+        ++q;
+        nb += make_quartet((size_t)ij, kl, nullptr, nullptr, nullptr, d);
+      }
+      row_q[ij] = q;
+      row_b[ij] = nb;
+      row_d[ij] = d;
+    }
+    t_count = secs(t0, now());
+  }
+  const long long budget = 1ll << 26;  // block doubles per batch (512 MB)
+  DevBuf d_tasks, d_blk, d_q;
+  size_t cap_t = 0, cap_b = 0, cap_q = 0;
+  std::vector<GQuartet> quartets;
+  std::vector<BlockTask> rtask, tasks;
+  std::vector<int> rkey;
+  size_t r0 = 0;
+  int batches = 0;
+  while (r0 < npair) {
+    // rows [r0, r1): at least one row, then as many as fit the budget
+    size_t r1 = r0 + 1;
+    long long dsum = row_d[r0];
+    while (r1 < npair && dsum + row_d[r1] <= budget) dsum += row_d[r1++];
+    std::vector<long long> qo(r1 - r0 + 1, 0), bo(r1 - r0 + 1, 0), dof(r1 - r0 + 1, 0);
+    for (size_t r = r0; r < r1; ++r) {
+      qo[r - r0 + 1] = qo[r - r0] + row_q[r];
+      bo[r - r0 + 1] = bo[r - r0] + row_b[r];
+      dof[r - r0 + 1] = dof[r - r0] + row_d[r];
+    }
+    const size_t nq = (size_t)qo.back(), nbk = (size_t)bo.back();
+    const long long ndbl = dof.back();
+    r0 = r1;
+    if (nq == 0) continue;
+    const auto t0 = now();
+    quartets.resize(nq);
+    rtask.resize(nbk);
+    rkey.resize(nbk);
+    const size_t rbase = r1 - (qo.size() - 1);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long r = 0; r < (long long)(qo.size() - 1); ++r) {
+      const size_t ij = rbase + (size_t)r;
+      long long qi = qo[r], bi = bo[r], off = dof[r];
+      for (size_t kl = 0; kl <= ij; ++kl) {
+        if (!(e.Q[ij] * e.Q[kl] > e.screen)) continue;
+        bi += make_quartet(ij, kl, &quartets[qi], &rtask[bi], &rkey[bi], off);
+        ++qi;
+      }
+    }
+    // bucket the block tasks by class, row order kept within a class
+    std::vector<size_t> cnt(NKEY + 1, 0);
+    for (int k : rkey) ++cnt[k + 1];
+    for (int k = 0; k < NKEY; ++k) cnt[k + 1] += cnt[k];
+    std::vector<size_t> start(cnt.begin(), cnt.end() - 1);
+    tasks.resize(nbk);
+    for (size_t i = 0; i < nbk; ++i) tasks[cnt[rkey[i]]++] = rtask[i];
+    t_fill += secs(t0, now());
+    const auto t1 = now();
+    if (nbk > cap_t) {
+      d_tasks.alloc(nbk * sizeof(BlockTask));
+      cap_t = nbk;
+    }
+    if ((size_t)ndbl > cap_b) {
+      d_blk.alloc((size_t)ndbl * 8);
+      cap_b = ndbl;
+    }
+    if (nq > cap_q) {
+      d_q.alloc(nq * sizeof(GQuartet));
+      cap_q = nq;
+    }
+    HIP_CHECK(hipMemcpyAsync(d_tasks.p, tasks.data(), nbk * sizeof(BlockTask), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(d_q.p, quartets.data(), nq * sizeof(GQuartet), hipMemcpyHostToDevice, st));
+    for (int k = 0; k < NKEY; ++k) {
+      const size_t a0 = start[k], z = cnt[k];  // cnt[k] is now the end of bucket k
+      if (z == a0) continue;
+      BlockArgs args{d_tasks.as<BlockTask>() + a0, (int)(z - a0), d_pd.as<PairDev>(), view, e.d_boys.as<double>(),
+                     d_blk.as<double>()};
+      launch_any_block(k / 125, (k / 25) % 5, (k / 5) % 5, k % 5, args, st);
+    }
+    const int blocks = (int)std::min<size_t>(2048, (nq + 3) / 4);
+    hipLaunchKernelGGL(k_grad_contract, dim3(blocks), dim3(256), lds, st, d_q.as<GQuartet>(), (int)nq,
+                       d_blk.as<double>(), dP, d_scale.as<double>(), n, na, d_grad.as<double>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(st));  // the host lists are reused
+    t_gpu += secs(t1, now());
+    ++batches;
+  }
+  if (trace)
+    fprintf(stderr, "hfg gradient: %zu pairs, %d batches; count %.3f s, fill %.3f s, device %.3f s, total %.3f s\n",
+            npair, batches, t_count, t_fill, t_gpu, secs(t_start, now()));
   std::vector<double> g2((size_t)3 * na);
   download(g2.data(), d_grad.as<double>(), g2.size(), st);
   for (size_t x = 0; x < g.size(); ++x) grad[x] = g[x] + g2[x];
+}
+
+// The same for host P and W, on the engine's stream.
+void rhf_gradient(Engine& e, const std::vector<std::array<double, 4>>& atoms, const double* P, const double* W,
+                  double* grad) {
+  const size_t nn = (size_t)e.n * e.n;
+  DevBuf d;
+  d.alloc(2 * nn * sizeof(double));
+  HIP_CHECK(hipMemcpyAsync(d.as<double>(), P, nn * 8, hipMemcpyHostToDevice, e.stream));
+  HIP_CHECK(hipMemcpyAsync(d.as<double>() + nn, W, nn * 8, hipMemcpyHostToDevice, e.stream));
+  rhf_gradient_device(e, atoms, d.as<double>(), d.as<double>() + nn, grad, e.stream);
 }
 
 }  // namespace hfg

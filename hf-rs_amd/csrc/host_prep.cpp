@@ -162,6 +162,7 @@ int pair_class(int la, int lb) {
   if (lb > la) return -1;
   if (la <= 2) return la * (la + 1) / 2 + lb;  // ss0 ps1 pp2 ds3 dp4 dd5
   if (lb == 0 && la <= 4) return 3 + la;     // fs6 gs7 (auxiliary shell with the unit function)
+  if (la == 3) return 7 + lb;                // fp8 fd9 ff10 (orbital pairs of the 3-index integrals)
   return -1;
 }
 int quartet_class(int pb, int pk) { return pb * (pb + 1) / 2 + pk; }

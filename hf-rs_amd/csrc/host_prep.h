@@ -51,7 +51,7 @@ void append_aux_pair(const Shell& P, PrimPairHost& pp);
 // Boys table (layout documented in eri_device.h), computed in long double.
 std::vector<double> boys_table();
 
-int pair_class(int la, int lb);  // index into (ss,ps,pp,ds,dp,dd), -1 if beyond d
+int pair_class(int la, int lb);  // (ss,ps,pp,ds,dp,dd,fs,gs,fp,fd,ff) index, -1 beyond
 int quartet_class(int pc_bra, int pc_ket);  // pc_bra >= pc_ket, index 0..20
 
 }  // namespace hfg

@@ -5,7 +5,7 @@
 // superposition step of compute_sad_density (src/sad.rs:217-259).
 //
 // One 64-lane workgroup per unique shell pair (A, B), oriented so that
-// L_A >= L_B, one launch per (L_A, L_B) class (L <= 2).  Lanes take the
+// L_A >= L_B, one launch per (L_A, L_B) class (L <= 3).  Lanes take the
 // primitive pairs for S and T and the (primitive pair, nucleus) products for
 // V; the per-lane sums are reduced through LDS in a fixed order, so a result
 // does not depend on the schedule.  Obara-Saika recurrences:
@@ -38,7 +38,7 @@
 namespace hfg {
 namespace {
 
-constexpr int ONEE_LMAX = 2;  // as the J/K classes
+constexpr int ONEE_LMAX = 3;  // f: cc-pVTZ in the DF drivers
 
 constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
 // number of exponent triples (x, y, z) with x + y + z <= L
@@ -602,8 +602,12 @@ void launch_class(int la, int lb, const OneArgs& g, hipStream_t st) {
     case 8: return launch_onee<2, 0>(g, st);
     case 9: return launch_onee<2, 1>(g, st);
     case 10: return launch_onee<2, 2>(g, st);
+    case 12: return launch_onee<3, 0>(g, st);
+    case 13: return launch_onee<3, 1>(g, st);
+    case 14: return launch_onee<3, 2>(g, st);
+    case 15: return launch_onee<3, 3>(g, st);
   }
-  throw HfgError{HFG_UNSUPPORTED, "one-electron class beyond d"};
+  throw HfgError{HFG_UNSUPPORTED, "one-electron class beyond f"};
 }
 
 __global__ void k_sad_scatter(const double* __restrict__ blk, const int* __restrict__ boff,
@@ -649,7 +653,7 @@ struct OneStage {
     std::vector<double> pe, pr, fscale(n, 0.0);
     for (int s = 0; s < ns; ++s) {
       const Shell& x = sh[s];
-      if (x.L > ONEE_LMAX) throw HfgError{HFG_UNSUPPORTED, "one-electron integrals support shells up to d"};
+      if (x.L > ONEE_LMAX) throw HfgError{HFG_UNSUPPORTED, "one-electron integrals support shells up to f"};
       if (x.ao < 0 || x.ao + (int)x.scale.size() > n) throw HfgError{HFG_INVALID_INPUT, "shell beyond the basis"};
       hs[s] = OneShell{{x.c[0], x.c[1], x.c[2]}, x.L, x.K, (int)pe.size(), x.ao, x.atom};
       pe.insert(pe.end(), x.e.begin(), x.e.end());

@@ -39,7 +39,7 @@ void sad_guess_device(const std::vector<Shell>& sh, int n, const std::vector<std
                       int charge, const double* dS, double* dP, Blas& bl, hipStream_t st);
 // S, T, V and H = T + V (n x n each, device; any may be null) of the
 // Cartesian basis sh for the nuclei (Z, x, y, z), compute_one_electron_integrals
-// (src/scf.rs:246-271) on the device; onee_dev.hip.  Shells up to d.
+// (src/scf.rs:246-271) on the device; onee_dev.hip.  Shells up to f.
 void one_electron_device(const std::vector<Shell>& sh, int n, const std::vector<std::array<double, 4>>& atoms,
                          double* dS, double* dT, double* dV, double* dH, hipStream_t st);
 // Adds the one-electron terms of the RHF nuclear gradient (overlap, kinetic
@@ -47,6 +47,11 @@ void one_electron_device(const std::vector<Shell>& sh, int n, const std::vector<
 // derivative at every nucleus, src/gradient.rs:411-490) for the device
 // density dP and energy-weighted density dW (n x n) to dgrad (natom x 3,
 // device); onee_dev.hip.
+// Nuclear gradient (natom x 3, host) for the device density dP and
+// energy-weighted density dW (n x n) of the engine's molecule, on stream st;
+// grad.hip.  Reads the engine's tables only.
+void rhf_gradient_device(Engine& e, const std::vector<std::array<double, 4>>& atoms, const double* dP,
+                         const double* dW, double* grad, hipStream_t st);
 void onee_gradient_device(const std::vector<Shell>& sh, int n, const std::vector<std::array<double, 4>>& atoms,
                           const double* dP, const double* dW, double* dgrad, hipStream_t st);
 

@@ -339,9 +339,15 @@ int hfg_server_create(const hfg_server_options* opt, hfg_server** out);
 void hfg_server_destroy(hfg_server* server);
 /* Queues a request: nsteps geometries of one molecule (bases[k] and zxyz[k],
  * natom atoms each, copied), arriving arrival_s seconds after the start of
- * the run.  keep_density: keep the last step's density for hfg_server_result. */
+ * the run.  flags: HFG_SERVE_KEEP_DENSITY keeps the last step's density for
+ * hfg_server_result; HFG_SERVE_GRADIENT (the analytic nuclear gradient of
+ * every step, compute_rhf_gradient, src/gradient.rs:272-285, for
+ * hfg_server_gradient) is switched off in this build and returns
+ * HFG_UNSUPPORTED. */
+#define HFG_SERVE_KEEP_DENSITY 1
+#define HFG_SERVE_GRADIENT 2
 int hfg_server_submit(hfg_server* server, double arrival_s, int nsteps, const hfg_basis_view* bases,
-                      const double* const* zxyz, int natom, int charge, const hfg_scf_options* opt, int keep_density,
+                      const double* const* zxyz, int natom, int charge, const hfg_scf_options* opt, int flags,
                       int* request_id);
 /* Serves every queued request (arrival times against the wall clock from the
  * start of this call) and returns when all have finished. */
@@ -350,6 +356,9 @@ int hfg_server_run(hfg_server* server);
  * NULL), and the last step's density (n x n, may be NULL; needs keep_density). */
 int hfg_server_result(hfg_server* server, int request_id, hfg_request_result* result, hfg_step_result* steps,
                       double* p_out);
+/* The nuclear gradients of a finished request submitted with
+ * HFG_SERVE_GRADIENT: nsteps x natom x 3 (Eh/bohr), step-major. */
+int hfg_server_gradient(hfg_server* server, int request_id, double* grad);
 int hfg_server_stats(const hfg_server* server, hfg_server_stats_t* stats);
 
 /* Host-side basis construction from Gaussian-94 text, mirroring build_basis
