@@ -126,27 +126,53 @@ __global__ void k_unpack_sym(const double* __restrict__ Xp, int n, double* __res
   X[idx] = Xp[(size_t)a * (a + 1) / 2 + b];
 }
 
-// Rows q < nq of packed B (row stride np) -> nq symmetric n x n matrices, as
-// T (double, float, or the two BF16 parts of a split).
+// Rows q < nq of packed B (row stride np) -> nq symmetric n x n matrices of
+// type T (double, float, or the two BF16 parts of a split, hi and lo), by
+// 64 x 64 tiles through LDS, so that both the packed reads (along j of a row
+// i >= j) and the two writes (row i of the lower tile, row j of its
+// transpose) are coalesced.  grid (lower tiles, nq), 256 threads.
 template <class T>
-__global__ void k_unpack_rows(const double* __restrict__ Bp, int nq, int n, size_t np, T* __restrict__ out,
-                              __hip_bfloat16* __restrict__ lo) {
+__global__ __launch_bounds__(256) void k_unpack_tiles(const double* __restrict__ Bp, int n, size_t np, int nt,
+                                                      T* __restrict__ out, __hip_bfloat16* __restrict__ lo) {
+  __shared__ double tile[64 * 65];
+  int ti = 0, rem = blockIdx.x;  // lower tile (ti >= tj), row-major over the triangle
+  while (rem > ti) {
+    rem -= ti + 1;
+    ++ti;
+  }
+  const int tj = rem;
+  const int q = blockIdx.y;
+  const double* B = Bp + (size_t)q * np;
   const size_t nn = (size_t)n * n;
-  const size_t tot = (size_t)nq * nn;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
-       idx += (size_t)gridDim.x * blockDim.x) {
-    const size_t q = idx / nn, r = idx - q * nn;
-    const int i = (int)(r / n), j = (int)(r - (size_t)i * n);
-    const int a = i > j ? i : j, b = i > j ? j : i;
-    const double v = Bp[q * np + (size_t)a * (a + 1) / 2 + b];
+  T* O = out + (size_t)q * nn;
+  __hip_bfloat16* Lo = lo ? lo + (size_t)q * nn : nullptr;
+  const int i0 = ti * 64, j0 = tj * 64, t = threadIdx.x;
+  const bool diag = ti == tj;
+  auto put = [&](size_t at, double v) {
     if constexpr (std::is_same<T, __hip_bfloat16>::value) {  // as k_split_bf16
       const float f = (float)v;
       const __hip_bfloat16 h = __float2bfloat16(f);
-      out[idx] = h;
-      lo[idx] = __float2bfloat16(f - __bfloat162float(h));
+      O[at] = h;
+      Lo[at] = __float2bfloat16(f - __bfloat162float(h));
     } else {
-      out[idx] = (T)v;
+      O[at] = (T)v;
     }
+  };
+  for (int r = 0; r < 16; ++r) {
+    const int li = (t >> 6) + 4 * r, lj = t & 63;
+    const int i = i0 + li, j = j0 + lj;
+    double v = 0.0;
+    if (i < n && j < n && (!diag || j <= i)) {
+      v = B[(size_t)i * (i + 1) / 2 + j];
+      put((size_t)i * n + j, v);
+    }
+    tile[li * 65 + lj] = v;
+  }
+  __syncthreads();
+  for (int r = 0; r < 16; ++r) {
+    const int lj = (t >> 6) + 4 * r, li = t & 63;  // out row j, along i
+    const int i = i0 + li, j = j0 + lj;
+    if (i < n && j < n && (!diag || i > j)) put((size_t)j * n + i, tile[li * 65 + lj]);
   }
 }
 
@@ -611,8 +637,9 @@ struct DfEngine {
                                                                  : nullptr;
     for (int q0 = 0; q0 < rank; q0 += qc) {
       const int qn = std::min(qc, rank - q0);
-      hipLaunchKernelGGL(k_unpack_rows<T>, dim3(4096), dim3(256), 0, stream, d_B.as<double>() + (size_t)q0 * npk, qn,
-                         n, npk, W, lo);
+      const int nt = (n + 63) / 64;
+      hipLaunchKernelGGL(k_unpack_tiles<T>, dim3((unsigned)(nt * (nt + 1) / 2), (unsigned)qn), dim3(256), 0, stream,
+                         d_B.as<double>() + (size_t)q0 * npk, n, npk, nt, W, lo);
       HIP_CHECK(hipGetLastError());
       f(q0, qn, (const T*)W, (const __hip_bfloat16*)lo);
     }
