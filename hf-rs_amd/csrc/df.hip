@@ -275,9 +275,12 @@ __global__ void k_sum_parts(const T* __restrict__ part, int nb, size_t nn, doubl
   }
 }
 
-// df_mfma.hip: K = alpha X X^T on the FP64 matrix cores.
+// df_mfma.hip: K = alpha X X^T, and A = B C from the packed factor, on the
+// FP64 matrix cores.
 void syrk_f64(const double* X, int n, long long L, long long ldx, double alpha, double* K, DevBuf& scratch,
               hipStream_t st);
+void bc_packed_f64(const double* Bp, size_t np, int rank, int n, const double* C, int o, double* At,
+                   long long ldt, double alpha, double* gamma, DevBuf& scratch, hipStream_t st);
 
 struct DfEngine {
   hipStream_t stream = nullptr;
@@ -291,9 +294,16 @@ struct DfEngine {
   DevBuf d_B;
   DevBuf d_pk, d_unp;  // packed path: packed P and J; unpacked rows of B for DF-K
   DevBuf d_syrk;       // partial tiles of the hand-written exchange product
+  DevBuf d_bcp;        // tile partials of B_Q . P from the packed A = B C
   // FP64 exchange product by the hand-written MFMA kernel (HFG_DF_SYRK=0: rocBLAS dgemm)
   const bool use_syrk = [] {
     const char* e = std::getenv("HFG_DF_SYRK");
+    return !(e && std::atoi(e) == 0);
+  }();
+  // packed factor, FP64: A = B C by the hand-written kernel straight from the
+  // packed rows (HFG_DF_BCPACK=0: unpacked chunks and rocBLAS dgemm)
+  const bool use_bcpack = [] {
+    const char* e = std::getenv("HFG_DF_BCPACK");
     return !(e && std::atoi(e) == 0);
   }();
   DevBuf d_work, d_A, d_At;
@@ -649,12 +659,16 @@ struct DfEngine {
   // (src/df.rs:143-173).  dC: n x o device; outputs device n x n.
   void jk(const double* dC, int o, double* dJ, double* dK) {
     const size_t nn = (size_t)n * n;
+    // packed factor with K: A = B C by the packed kernel straight into the
+    // exchange layout, B P taken from its tiles (B_Q . P = 2 sum A_Q . C)
+    const bool fused = packed && use_bcpack && dK;
     if ((size_t)o != work_o) {
       d_work.alloc(std::max(nn, (size_t)rank) * 8 + 8);
-      d_A.alloc((size_t)rank * n * o * 8);
+      d_A.free();
       d_At.alloc((size_t)rank * n * o * 8);
       work_o = o;
     }
+    if (dK && !fused && d_A.bytes < (size_t)rank * n * o * 8) d_A.alloc((size_t)rank * n * o * 8);
     double* P = d_work.as<double>();
     Dev dq;
     dq.alloc(rank);
@@ -674,8 +688,12 @@ struct DfEngine {
       jv = d_pk.as<double>() + npk;
     }
     // column-major view of B: bc x rank (B^T); d = B P  ->  gemv(T)
-    RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_transpose, (rocblas_int)bc, rank, &one, d_B.as<double>(),
-                           (rocblas_int)bc, pv, 1, &zero, dq.p, 1));
+    if (fused)
+      bc_packed_f64(d_B.as<double>(), npk, rank, n, dC, o, d_At.as<double>(), (long long)rank * o, 2.0, dq.p, d_bcp,
+                    stream);
+    else
+      RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_transpose, (rocblas_int)bc, rank, &one, d_B.as<double>(),
+                             (rocblas_int)bc, pv, 1, &zero, dq.p, 1));
     RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_none, (rocblas_int)bc, rank, &one, d_B.as<double>(),
                            (rocblas_int)bc, dq.p, 1, &zero, jv, 1));
     if (packed) {
@@ -683,17 +701,19 @@ struct DfEngine {
       HIP_CHECK(hipGetLastError());
     }
     if (!dK) return;  // J only
-    // A = B C: rows (Q, m), (rank*n) x o
-    if (!packed)
-      bl->gemm(false, false, rank * n, o, n, 1.0, d_B.as<double>(), n, dC, o, 0.0, d_A.as<double>(), o);
-    else
-      for_unpacked_rows<double>([&](int q0, int qn, const double* W, const __hip_bfloat16*) {
-        bl->gemm(false, false, qn * n, o, n, 1.0, W, n, dC, o, 0.0, d_A.as<double>() + (size_t)q0 * n * o, o);
-      });
-    const size_t tot = (size_t)rank * n * o;
-    hipLaunchKernelGGL(k_permute_qmi, dim3((tot + 255) / 256), dim3(256), 0, stream, d_A.as<double>(),
-                       d_At.as<double>(), rank, n, o);
-    HIP_CHECK(hipGetLastError());
+    if (!fused) {
+      // A = B C: rows (Q, m), (rank*n) x o, then permuted to At[m][Q o + i]
+      if (!packed)
+        bl->gemm(false, false, rank * n, o, n, 1.0, d_B.as<double>(), n, dC, o, 0.0, d_A.as<double>(), o);
+      else
+        for_unpacked_rows<double>([&](int q0, int qn, const double* W, const __hip_bfloat16*) {
+          bl->gemm(false, false, qn * n, o, n, 1.0, W, n, dC, o, 0.0, d_A.as<double>() + (size_t)q0 * n * o, o);
+        });
+      const size_t tot = (size_t)rank * n * o;
+      hipLaunchKernelGGL(k_permute_qmi, dim3((tot + 255) / 256), dim3(256), 0, stream, d_A.as<double>(),
+                         d_At.as<double>(), rank, n, o);
+      HIP_CHECK(hipGetLastError());
+    }
     // FP64: the symmetric product on the matrix cores (upper tiles only,
     // split over the contraction index); or one rocBLAS GEMM (its dsyrk,
     // with half the flops, ran 2.1x slower than the GEMM on (H2O)32/cc-pVDZ)

@@ -136,7 +136,197 @@ __global__ void k_syrk_reduce(const double* __restrict__ part, int nslice, int n
   }
 }
 
+// A_Q = B_Q C for the rows Q = q0 .. q0 + gridDim.z - 1 of the packed factor
+// (B_Q symmetric n x n, lower triangle row-major, packed row stride np; C
+// n x o row-major), written as At[m][Q o + i] (the exchange product's
+// operand, row stride ldt), and with part, the tile's sum of A_Q . C
+// (B_Q . P for P = C C^T without another pass over B).  One workgroup of NT threads
+// per 128 x 160 tile of A_Q (all 160 columns of a 160-orbital C in one tile,
+// so that B is read once).  Per 16-wide contraction step the B panel comes
+// from the packed rows -- along packed row k where the panel lies above the
+// diagonal (k > every m of the tile), along packed row m below it, by
+// (max, min) where the diagonal crosses it -- consecutive lanes reading
+// consecutive doubles; the next panel is loaded into registers while the
+// current one is multiplied out of the other LDS buffer, one barrier a step.
+// Waves (NT / 128) x 2, each 128 / (NT / 128) rows x 80 columns of 16 x 16
+// MFMA blocks.
+constexpr int AM = 128, AN = 160, AK = 16;
+constexpr int APB = AK + 1, APC = AN + 1;
+
+template <int NT>
+struct BcRegs {
+  static constexpr int LR = NT / 16;         // lanes along a panel row
+  static constexpr int NB = AM * AK / NT;    // B values per thread
+  static constexpr int NC = AN * AK / NT;    // C values per thread
+  double b[NB], c[NC];
+};
+
+template <int NT>
+__device__ __forceinline__ void bc_load(const double* __restrict__ B, int n, const double* __restrict__ C, int o,
+                                        int m0, int i0, int k0, int t, BcRegs<NT>& r) {
+  constexpr int LR = BcRegs<NT>::LR, NB = BcRegs<NT>::NB, NC = BcRegs<NT>::NC;
+  if (k0 >= m0 + AM) {
+    const int k = k0 + t / LR;
+    const size_t base = (size_t)k * (k + 1) / 2;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int m = m0 + t % LR + LR * c;
+      r.b[c] = (k < n && m < n) ? B[base + m] : 0.0;
+    }
+  } else if (k0 + AK <= m0) {
+    const int k = k0 + (t & 15);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int m = m0 + (t >> 4) + LR * c;
+      r.b[c] = (k < n && m < n) ? B[(size_t)m * (m + 1) / 2 + k] : 0.0;
+    }
+  } else {
+    const int k = k0 + (t & 15);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int m = m0 + (t >> 4) + LR * c;
+      const size_t hi = m > k ? m : k, lo = m > k ? k : m;
+      r.b[c] = (k < n && m < n) ? B[hi * (hi + 1) / 2 + lo] : 0.0;
+    }
+  }
+  const int k = k0 + t / LR;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int i = i0 + t % LR + LR * c;
+    r.c[c] = (k < n && i < o) ? C[(size_t)k * o + i] : 0.0;
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void bc_store(int m0, int k0, int t, const BcRegs<NT>& r, double* Bs, double* Cs) {
+  constexpr int LR = BcRegs<NT>::LR, NB = BcRegs<NT>::NB, NC = BcRegs<NT>::NC;
+  if (k0 >= m0 + AM) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) Bs[(t % LR + LR * c) * APB + t / LR] = r.b[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) Bs[((t >> 4) + LR * c) * APB + (t & 15)] = r.b[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) Cs[(t / LR) * APC + t % LR + LR * c] = r.c[c];
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_bc_packed_f64(const double* __restrict__ Bp, size_t np, int n,
+                                                      const double* __restrict__ C, int o,
+                                                      double* __restrict__ At, long long ldt,
+                                                      double* __restrict__ part) {
+  constexpr int WR = AM / (NT / 128), BA = WR / 16;  // rows per wave, MFMA blocks along them
+  __shared__ double Bs[2][AM * APB], Cs[2][AK * APC];
+  const int m0 = blockIdx.x * AM, i0 = blockIdx.y * AN, q = blockIdx.z;
+  const double* B = Bp + (size_t)q * np;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  d4 acc[BA][5];
+#pragma unroll
+  for (int a = 0; a < BA; ++a)
+#pragma unroll
+    for (int b = 0; b < 5; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  // buffer k0 / AK & 1 holds panel k0; the registers hold panel k0 + AK.  A
+  // step stores the registers into the other buffer (whose readers all passed
+  // the previous barrier), loads the panel after, multiplies, and meets the
+  // other waves at one barrier.
+  BcRegs<NT> r;
+  bc_load<NT>(B, n, C, o, m0, i0, 0, t, r);
+  bc_store<NT>(m0, 0, t, r, Bs[0], Cs[0]);
+  if (AK < n) bc_load<NT>(B, n, C, o, m0, i0, AK, t, r);
+  __syncthreads();
+  for (int k0 = 0, cur = 0; k0 < n; k0 += AK, cur ^= 1) {
+    if (k0 + AK < n) {
+      bc_store<NT>(m0, k0 + AK, t, r, Bs[cur ^ 1], Cs[cur ^ 1]);
+      if (k0 + 2 * AK < n) bc_load<NT>(B, n, C, o, m0, i0, k0 + 2 * AK, t, r);
+    }
+    const double* bs = Bs[cur];
+    const double* cs = Cs[cur];
+#pragma unroll
+    for (int kk = 0; kk < AK; kk += 4) {
+      const int kq = kk + (lane >> 4), rq = lane & 15;
+      double av[BA], bv[5];
+#pragma unroll
+      for (int a = 0; a < BA; ++a) av[a] = bs[(WR * wm + 16 * a + rq) * APB + kq];
+#pragma unroll
+      for (int b = 0; b < 5; ++b) bv[b] = cs[kq * APC + 80 * wn + 16 * b + rq];
+#pragma unroll
+      for (int a = 0; a < BA; ++a)
+#pragma unroll
+        for (int b = 0; b < 5; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // A_Q[m][i] -> At[m][Q o + i]; with part, this tile's sum of A_Q . C
+  double* Aq = At + (size_t)q * o;
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < BA; ++a)
+#pragma unroll
+    for (int b = 0; b < 5; ++b)
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int m = m0 + WR * wm + 16 * a + (lane >> 4) + 4 * x;
+        const int i = i0 + 80 * wn + 16 * b + (lane & 15);
+        if (m < n && i < o) {
+          Aq[(size_t)m * ldt + i] = acc[a][b][x];
+          if (part) s += acc[a][b][x] * C[(size_t)m * o + i];
+        }
+      }
+  if (part) {  // fixed-order block sum
+    __shared__ double red[NT];
+    red[t] = s;
+    __syncthreads();
+    for (int h = NT / 2; h > 0; h >>= 1) {
+      if (t < h) red[t] += red[t + h];
+      __syncthreads();
+    }
+    if (t == 0) part[(size_t)q * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+  }
+}
+
+// gamma[Q] = alpha * (sum of the np tile partials of row Q, in order)
+__global__ void k_part_reduce(const double* __restrict__ part, int np, int rank, double alpha,
+                              double* __restrict__ gamma) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= rank) return;
+  double s = 0.0;
+  for (int j = 0; j < np; ++j) s += part[(size_t)q * np + j];
+  gamma[q] = alpha * s;
+}
+
 }  // namespace
+
+// A = B C for every row of the packed factor (rank rows of stride np), into
+// the layout of the exchange product: At[m][Q o + i] (row stride ldt >=
+// rank o).  With gamma, also gamma[Q] = alpha sum_{m,i} A_Q[m][i] C[m][i]
+// (= tr(B_Q P) for P = alpha C C^T), reduced in a fixed order through
+// scratch.  On stream st.
+void bc_packed_f64(const double* Bp, size_t np, int rank, int n, const double* C, int o, double* At,
+                   long long ldt, double alpha, double* gamma, DevBuf& scratch, hipStream_t st) {
+  const unsigned gx = (unsigned)((n + AM - 1) / AM), gy = (unsigned)((o + AN - 1) / AN);
+  double* part = nullptr;
+  if (gamma) {
+    const size_t need = (size_t)rank * gx * gy * 8;
+    if (scratch.bytes < need) scratch.alloc(need);
+    part = scratch.as<double>();
+  }
+  for (int q0 = 0; q0 < rank; q0 += 65535) {
+    const int nq = std::min(65535, rank - q0);
+    double* pq = part ? part + (size_t)q0 * gx * gy : nullptr;
+    // 512 threads (2 waves a SIMD) measured 5 % faster than 256 (1 wave a
+    // SIMD, 160 accumulator registers) on (H2O)32 / cc-pVTZ
+    hipLaunchKernelGGL(k_bc_packed_f64<512>, dim3(gx, gy, (unsigned)nq), dim3(512), 0, st, Bp + (size_t)q0 * np, np, n,
+                       C, o, At + (size_t)q0 * o, ldt, pq);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (gamma) {
+    hipLaunchKernelGGL(k_part_reduce, dim3((unsigned)((rank + 255) / 256)), dim3(256), 0, st, part, (int)(gx * gy),
+                       rank, alpha, gamma);
+    HIP_CHECK(hipGetLastError());
+  }
+}
 
 // Contraction slices for an n x n output and contraction length L: enough
 // workgroups (about 2048) for the chip, at least 64 of L each, at most 64.
