@@ -281,6 +281,8 @@ void syrk_f64(const double* X, int n, long long L, long long ldx, double alpha, 
               hipStream_t st);
 void bc_packed_f64(const double* Bp, size_t np, int rank, int n, const double* C, int o, double* At,
                    long long ldt, double alpha, double* gamma, DevBuf& scratch, hipStream_t st);
+void bc_packed_bf16x3(const double* Bp, size_t np, int rank, int n, const double* C, int o, unsigned short* Ah,
+                      unsigned short* Al, long long ldt, DevBuf& ct, hipStream_t st);
 
 struct DfEngine {
   hipStream_t stream = nullptr;
@@ -295,13 +297,15 @@ struct DfEngine {
   DevBuf d_pk, d_unp;  // packed path: packed P and J; unpacked rows of B for DF-K
   DevBuf d_syrk;       // partial tiles of the hand-written exchange product
   DevBuf d_bcp;        // tile partials of B_Q . P from the packed A = B C
+  DevBuf d_ct;         // split BF16 C^T of the packed split-BF16 A = B C
   // FP64 exchange product by the hand-written MFMA kernel (HFG_DF_SYRK=0: rocBLAS dgemm)
   const bool use_syrk = [] {
     const char* e = std::getenv("HFG_DF_SYRK");
     return !(e && std::atoi(e) == 0);
   }();
-  // packed factor, FP64: A = B C by the hand-written kernel straight from the
-  // packed rows (HFG_DF_BCPACK=0: unpacked chunks and rocBLAS dgemm)
+  // packed factor, FP64 and split BF16: A = B C by the hand-written kernels
+  // straight from the packed rows (HFG_DF_BCPACK=0: unpacked chunks and
+  // rocBLAS GEMMs)
   const bool use_bcpack = [] {
     const char* e = std::getenv("HFG_DF_BCPACK");
     return !(e && std::atoi(e) == 0);
@@ -750,21 +754,31 @@ struct DfEngine {
       hipLaunchKernelGGL(k_split_bf16<double>, g, b, 0, stream, d_B.as<double>(), d_Bh.as<__hip_bfloat16>(),
                          d_Bl.as<__hip_bfloat16>(), bsz);
     }
+    // packed factor, split BF16: the hi and lo operands of the symmetric
+    // product straight from the packed rows (no unpack, FP32 A or permute)
+    const bool direct = prec == 2 && packed && use_bcpack;
     if ((size_t)o != work_o32) {
       d_C32.alloc(csz * 4);
       d_Ch.alloc(csz * 2);
       d_Cl.alloc(csz * 2);
-      d_A32.alloc(asz * 4);
-      d_At32.alloc(asz * 4);
+      d_A32.free();
+      d_At32.free();
       d_Ah.alloc(asz * 2);
       d_Al.alloc(asz * 2);
       d_K32.alloc(nn * 4);
       work_o32 = o;
     }
+    if (!direct && !d_A32.p) {
+      d_A32.alloc(asz * 4);
+      d_At32.alloc(asz * 4);
+    }
     float* A = d_A32.as<float>();
     float* At = d_At32.as<float>();
     float* K = d_K32.as<float>();
-    if (prec == 1) {
+    if (direct) {
+      bc_packed_bf16x3(d_B.as<double>(), npk, rank, n, dC, o, (unsigned short*)d_Ah.p, (unsigned short*)d_Al.p,
+                       (long long)rank * o, d_ct, stream);
+    } else if (prec == 1) {
       hipLaunchKernelGGL(k_to_f32, g, b, 0, stream, dC, d_C32.as<float>(), csz);
       if (!packed)
         gemm_ex(false, false, rank * n, o, n, d_B32.p, n, d_C32.p, o, A, o, 1.f, 0.f, rocblas_datatype_f32_r);
@@ -789,7 +803,8 @@ struct DfEngine {
           three(Wh, Wl, qn * n, A + (size_t)q0 * n * o);
         });
     }
-    hipLaunchKernelGGL(k_permute_qmi_f32, dim3((asz + 255) / 256), dim3(256), 0, stream, A, At, rank, n, o);
+    if (!direct)
+      hipLaunchKernelGGL(k_permute_qmi_f32, dim3((asz + 255) / 256), dim3(256), 0, stream, A, At, rank, n, o);
     (void)K;
     const int nb = nsplit(), qb = (rank + nb - 1) / nb, nbe = (rank + qb - 1) / qb, full = rank / qb;
     const int kc = qb * o, ld = rank * o;
@@ -818,8 +833,9 @@ struct DfEngine {
     if (prec == 1) {
       split_gemm(At, At, 0.f, rocblas_datatype_f32_r, 4);
     } else {
-      hipLaunchKernelGGL(k_split_bf16<float>, g, b, 0, stream, At, d_Ah.as<__hip_bfloat16>(),
-                         d_Al.as<__hip_bfloat16>(), asz);
+      if (!direct)
+        hipLaunchKernelGGL(k_split_bf16<float>, g, b, 0, stream, At, d_Ah.as<__hip_bfloat16>(),
+                           d_Al.as<__hip_bfloat16>(), asz);
       const auto bf = rocblas_datatype_bf16_r;
       split_gemm(d_Ah.p, d_Ah.p, 0.f, bf, 2);
       split_gemm(d_Ah.p, d_Al.p, 1.f, bf, 2);

@@ -286,6 +286,175 @@ __global__ __launch_bounds__(NT) void k_bc_packed_f64(const double* __restrict__
   }
 }
 
+// ---- split-BF16 form: A = B C with each operand x = hi + lo (hi = bf16(x),
+// lo = bf16(x - hi), from the FP32 value) and three BF16 products hi hi +
+// hi lo + lo hi accumulated in FP32, then A itself split into the hi and lo
+// BF16 operands of the symmetric product.
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// FP32 -> BF16 bits, round to nearest even (finite values), as __float2bfloat16
+__device__ __forceinline__ unsigned bf16_bits(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_value(unsigned h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ void split_bf16(float f, unsigned& h, unsigned& l) {
+  h = bf16_bits(f);
+  l = bf16_bits(f - bf16_value(h));
+}
+
+constexpr int SM = 128, SN = 160, SK = 32, SP = 40;  // tile rows, columns, contraction step, LDS row (BF16)
+constexpr int SBUF = (2 * SM + 2 * SN) * SP;         // BF16 per LDS buffer: B hi, B lo, C hi, C lo
+
+// C (n x o, FP64) -> hi and lo parts of C^T, [oc][nk] BF16, zero past (o, n)
+__global__ void k_split_ct(const double* __restrict__ C, int n, int o, int nk, int oc, unsigned short* __restrict__ Th,
+                           unsigned short* __restrict__ Tl) {
+  const size_t tot = (size_t)oc * nk;
+  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < tot; x += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(x / nk), k = (int)(x - (size_t)i * nk);
+    unsigned h = 0, l = 0;
+    if (i < o && k < n) split_bf16((float)C[(size_t)k * o + i], h, l);
+    Th[x] = (unsigned short)h;
+    Tl[x] = (unsigned short)l;
+  }
+}
+
+struct SplitRegs {
+  double b[8];       // B[m][k0 + 8 g + c], c < 8
+  uint4 ch[2], cl[2];  // 8 BF16 of C^T rows (chunks t and t + 512 of the 640)
+};
+
+__device__ __forceinline__ bool split_above(int m0, int k0) { return k0 >= m0 + SM; }
+
+__device__ __forceinline__ void split_load(const double* __restrict__ B, int n, const unsigned short* __restrict__ Th,
+                                           const unsigned short* __restrict__ Tl, int nk, int m0, int i0, int k0,
+                                           int t, SplitRegs& r) {
+  if (split_above(m0, k0)) {  // packed row k, consecutive m across lanes
+    const int m = m0 + (t & 127), g = t >> 7;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int k = k0 + 8 * g + c;
+      r.b[c] = (k < n && m < n) ? B[(size_t)k * (k + 1) / 2 + m] : 0.0;
+    }
+  } else if (k0 + SK <= m0) {  // packed row m, consecutive k
+    const int m = m0 + (t >> 2), g = t & 3;
+    const size_t base = (size_t)m * (m + 1) / 2;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int k = k0 + 8 * g + c;
+      r.b[c] = (k < n && m < n) ? B[base + k] : 0.0;
+    }
+  } else {
+    const int m = m0 + (t >> 2), g = t & 3;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int k = k0 + 8 * g + c;
+      const size_t hi = m > k ? m : k, lo = m > k ? k : m;
+      r.b[c] = (k < n && m < n) ? B[hi * (hi + 1) / 2 + lo] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int cid = t + 512 * h;
+    if (h == 0 || t < 128) {
+      const size_t at = (size_t)(i0 + (cid >> 2)) * nk + k0 + 8 * (cid & 3);
+      r.ch[h] = *(const uint4*)(Th + at);
+      r.cl[h] = *(const uint4*)(Tl + at);
+    }
+  }
+}
+
+__device__ __forceinline__ void split_store(int m0, int k0, int t, const SplitRegs& r, unsigned short* S) {
+  unsigned short *Bh = S, *Bl = S + SM * SP, *Ch = S + 2 * SM * SP, *Cl = Ch + SN * SP;
+  const bool above = split_above(m0, k0);
+  const int mm = above ? (t & 127) : (t >> 2), g = above ? (t >> 7) : (t & 3);
+  unsigned h[8], l[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) split_bf16((float)r.b[c], h[c], l[c]);
+  *(uint4*)(Bh + mm * SP + 8 * g) = uint4{h[0] | h[1] << 16, h[2] | h[3] << 16, h[4] | h[5] << 16, h[6] | h[7] << 16};
+  *(uint4*)(Bl + mm * SP + 8 * g) = uint4{l[0] | l[1] << 16, l[2] | l[3] << 16, l[4] | l[5] << 16, l[6] | l[7] << 16};
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    const int cid = t + 512 * x;
+    if (x == 0 || t < 128) {
+      *(uint4*)(Ch + (cid >> 2) * SP + 8 * (cid & 3)) = r.ch[x];
+      *(uint4*)(Cl + (cid >> 2) * SP + 8 * (cid & 3)) = r.cl[x];
+    }
+  }
+}
+
+// The split-BF16 A = B C from the packed factor: tiles, panel cases and
+// pipeline as k_bc_packed_f64 (128 x 160 per 512-thread workgroup, waves
+// 4 x 2 of 32 x 80), a 32-wide contraction step per v_mfma_f32_16x16x32_bf16;
+// B split while it is staged, C^T pre-split ([oc][nk], k contiguous, so each
+// lane's 8 contraction values are one 16-byte LDS read for either operand).
+// Output: the hi and lo BF16 parts of A at Ah/Al[m][Q o + i] (row stride ldt).
+__global__ __launch_bounds__(512) void k_bc_packed_bf16x3(const double* __restrict__ Bp, size_t np, int n,
+                                                          const unsigned short* __restrict__ Th,
+                                                          const unsigned short* __restrict__ Tl, int nk, int o,
+                                                          unsigned short* __restrict__ Ah,
+                                                          unsigned short* __restrict__ Al, long long ldt) {
+  __shared__ uint4 lds[2][SBUF / 8];
+  const int m0 = blockIdx.x * SM, i0 = blockIdx.y * SN, q = blockIdx.z;
+  const double* B = Bp + (size_t)q * np;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1, rq = lane & 15, g = lane >> 4;
+  f4 acc[2][5];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 5; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  SplitRegs r;
+  split_load(B, n, Th, Tl, nk, m0, i0, 0, t, r);
+  split_store(m0, 0, t, r, (unsigned short*)lds[0]);
+  if (SK < n) split_load(B, n, Th, Tl, nk, m0, i0, SK, t, r);
+  __syncthreads();
+  for (int k0 = 0, cur = 0; k0 < n; k0 += SK, cur ^= 1) {
+    if (k0 + SK < n) {
+      split_store(m0, k0 + SK, t, r, (unsigned short*)lds[cur ^ 1]);
+      if (k0 + 2 * SK < n) split_load(B, n, Th, Tl, nk, m0, i0, k0 + 2 * SK, t, r);
+    }
+    const unsigned short* S = (const unsigned short*)lds[cur];
+    const unsigned short *Bh = S, *Bl = S + SM * SP, *Ch = S + 2 * SM * SP, *Cl = Ch + SN * SP;
+    bf8 ah[2], al[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      ah[a] = *(const bf8*)(Bh + (32 * wm + 16 * a + rq) * SP + 8 * g);
+      al[a] = *(const bf8*)(Bl + (32 * wm + 16 * a + rq) * SP + 8 * g);
+    }
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      const bf8 ch = *(const bf8*)(Ch + (80 * wn + 16 * b + rq) * SP + 8 * g);
+      const bf8 cl = *(const bf8*)(Cl + (80 * wn + 16 * b + rq) * SP + 8 * g);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ch, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], cl, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ch, acc[a][b], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // D of the FP32 16 x 16 forms: lane l, register x holds row 4 (l / 16) + x, column l % 16
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 5; ++b)
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int m = m0 + 32 * wm + 16 * a + 4 * g + x;
+        const int i = i0 + 80 * wn + 16 * b + rq;
+        if (m < n && i < o) {
+          unsigned h, l;
+          split_bf16(acc[a][b][x], h, l);
+          const size_t at = (size_t)m * ldt + (size_t)q * o + i;
+          Ah[at] = (unsigned short)h;
+          Al[at] = (unsigned short)l;
+        }
+      }
+}
+
 // gamma[Q] = alpha * (sum of the np tile partials of row Q, in order)
 __global__ void k_part_reduce(const double* __restrict__ part, int np, int rank, double alpha,
                               double* __restrict__ gamma) {
@@ -324,6 +493,28 @@ void bc_packed_f64(const double* Bp, size_t np, int rank, int n, const double* C
   if (gamma) {
     hipLaunchKernelGGL(k_part_reduce, dim3((unsigned)((rank + 255) / 256)), dim3(256), 0, st, part, (int)(gx * gy),
                        rank, alpha, gamma);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+// The split-BF16 A = B C from the packed factor (rank rows of stride np),
+// written as the hi and lo BF16 parts Ah/Al[m][Q o + i] (row stride ldt >=
+// rank o); ct holds the split C^T.  On stream st.
+void bc_packed_bf16x3(const double* Bp, size_t np, int rank, int n, const double* C, int o, unsigned short* Ah,
+                      unsigned short* Al, long long ldt, DevBuf& ct, hipStream_t st) {
+  const int nk = (n + SK - 1) / SK * SK, oc = (o + SN - 1) / SN * SN;
+  const size_t tot = (size_t)oc * nk;
+  if (ct.bytes < 2 * tot * 2) ct.alloc(2 * tot * 2);
+  unsigned short* Th = ct.as<unsigned short>();
+  unsigned short* Tl = Th + tot;
+  hipLaunchKernelGGL(k_split_ct, dim3((unsigned)std::min<size_t>(2048, (tot + 255) / 256)), dim3(256), 0, st, C, n, o,
+                     nk, oc, Th, Tl);
+  HIP_CHECK(hipGetLastError());
+  const unsigned gx = (unsigned)((n + SM - 1) / SM), gy = (unsigned)(oc / SN);
+  for (int q0 = 0; q0 < rank; q0 += 65535) {
+    const int nq = std::min(65535, rank - q0);
+    hipLaunchKernelGGL(k_bc_packed_bf16x3, dim3(gx, gy, (unsigned)nq), dim3(512), 0, st, Bp + (size_t)q0 * np, np, n,
+                       Th, Tl, nk, o, Ah + (size_t)q0 * o, Al + (size_t)q0 * o, ldt);
     HIP_CHECK(hipGetLastError());
   }
 }
