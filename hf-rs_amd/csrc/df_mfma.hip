@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <vector>
 
 #include "engine.h"
@@ -136,12 +137,69 @@ __global__ void k_syrk_reduce(const double* __restrict__ part, int nslice, int n
   }
 }
 
-// A_Q = B_Q C for the rows Q = q0 .. q0 + gridDim.z - 1 of the packed factor
+// ---- Work order of the packed A = B C kernels.  Each packed element of B_Q
+// is read by two workgroups: the one of its row's 128-row tile and, as the
+// transposed element, the one of its column's tile.  Two choices make the
+// second read a cache hit:
+// - XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs by
+//   linear id, so linear id L = 8 j + x is given work item off(x) + j: each
+//   XCD takes a contiguous run of items, the tiles of one auxiliary row run
+//   together on one XCD and share its L2;
+// - tournament order over the k-tiles (the circle method on M = T or T + 1
+//   players, M odd, plus a fixed player F = M): in round r tile i multiplies
+//   k-tile (2 r - i) mod M and the tile named there multiplies k-tile i, so
+//   both reads of a 128 x 128 packed block fall in the same round; tile i
+//   meets F in round i and takes its diagonal block then, a round against
+//   the padding player (T even) is skipped.
+// Measured on (H2O)32 / cc-pVTZ (ms a launch, neither / XCD map / tournament
+// / both): FP64 111.8 / 111.3 / 119.6 / 119.2, split BF16 52.9 / 47.3 / 44.3
+// / 43.3.  The split-BF16 kernel, bound by the traffic of B, takes both; the
+// FP64 kernel, bound by the matrix cores, only the XCD map.
+constexpr int BC_XCD_MAP = 1, BC_TOURNAMENT = 2;
+struct WorkItem {
+  int mt, it, q;
+};
+__device__ __forceinline__ WorkItem work_item(int gx, int gy, long long total, int flags) {
+  const long long L = blockIdx.x, per = total / 8, rem = total % 8;
+  const int x = (int)(L % 8);
+  const long long w = (flags & BC_XCD_MAP) ? x * per + (x < rem ? x : rem) + L / 8 : L;
+  WorkItem r;
+  r.mt = (int)(w % gx);
+  const long long rest = w / gx;
+  r.it = (int)(rest % gy);
+  r.q = (int)(rest / gy);
+  return r;
+}
+
+template <int KS>
+struct PanelOrder {
+  static constexpr int TS = 128, PPT = TS / KS;  // tile width, panels a tile
+  int mt, T, M, n, tour;
+  __device__ PanelOrder(int mt_, int n_, int flags)
+      : mt(mt_), T((n_ + TS - 1) / TS), M(T | 1), n(n_), tour(flags & BC_TOURNAMENT) {}
+  __device__ int count() const { return tour ? M * PPT : (n + KS - 1) / KS; }
+  // k0 of panel s (tournament: s = r PPT + j), or -1 where there is none
+  __device__ int k0(int s) const {
+    if (!tour) return s * KS;
+    const int r = s / PPT, j = s - r * PPT;
+    int t = r == mt ? mt : ((2 * r - mt) % M + M) % M;
+    if (t >= T) return -1;
+    const int k = t * TS + j * KS;
+    return k < n ? k : -1;
+  }
+  __device__ int next(int s) const {
+    for (++s; s < count(); ++s)
+      if (k0(s) >= 0) break;
+    return s;
+  }
+};
+
+// A_Q = B_Q C for every row Q of the packed factor
 // (B_Q symmetric n x n, lower triangle row-major, packed row stride np; C
 // n x o row-major), written as At[m][Q o + i] (the exchange product's
 // operand, row stride ldt), and with part, the tile's sum of A_Q . C
-// (B_Q . P for P = C C^T without another pass over B).  One workgroup of NT threads
-// per 128 x 160 tile of A_Q (all 160 columns of a 160-orbital C in one tile,
+// (B_Q . P for P = C C^T without another pass over B).  One workgroup of NT
+// threads per 128 x 160 tile of A_Q, in the work order above (all 160 columns of a 160-orbital C in one tile,
 // so that B is read once).  Per 16-wide contraction step the B panel comes
 // from the packed rows -- along packed row k where the panel lies above the
 // diagonal (k > every m of the tile), along packed row m below it, by
@@ -151,6 +209,7 @@ __global__ void k_syrk_reduce(const double* __restrict__ part, int nslice, int n
 // Waves (NT / 128) x 2, each 128 / (NT / 128) rows x 80 columns of 16 x 16
 // MFMA blocks.
 constexpr int AM = 128, AN = 160, AK = 16;
+static_assert(AM == PanelOrder<AK>::TS, "tiles of the panel order are the row tiles");
 constexpr int APB = AK + 1, APC = AN + 1;
 
 template <int NT>
@@ -215,10 +274,11 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_bc_packed_f64(const double* __restrict__ Bp, size_t np, int n,
                                                       const double* __restrict__ C, int o,
                                                       double* __restrict__ At, long long ldt,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part, int gx, int gy, long long total, int flags) {
   constexpr int WR = AM / (NT / 128), BA = WR / 16;  // rows per wave, MFMA blocks along them
   __shared__ double Bs[2][AM * APB], Cs[2][AK * APC];
-  const int m0 = blockIdx.x * AM, i0 = blockIdx.y * AN, q = blockIdx.z;
+  const WorkItem wi = work_item(gx, gy, total, flags);
+  const int m0 = wi.mt * AM, i0 = wi.it * AN, q = wi.q;
   const double* B = Bp + (size_t)q * np;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
@@ -227,19 +287,23 @@ __global__ __launch_bounds__(NT) void k_bc_packed_f64(const double* __restrict__
   for (int a = 0; a < BA; ++a)
 #pragma unroll
     for (int b = 0; b < 5; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  // buffer k0 / AK & 1 holds panel k0; the registers hold panel k0 + AK.  A
-  // step stores the registers into the other buffer (whose readers all passed
-  // the previous barrier), loads the panel after, multiplies, and meets the
-  // other waves at one barrier.
+  // Panels in tournament order: panel s in buffer cur, the registers hold
+  // panel s1.  A step stores the registers into the other buffer (whose
+  // readers all passed the previous barrier), loads panel s2, multiplies, and
+  // meets the other waves at one barrier.
+  const PanelOrder<AK> po(wi.mt, n, flags);
+  const int S = po.count();
+  int s = po.next(-1), s1 = po.next(s);
   BcRegs<NT> r;
-  bc_load<NT>(B, n, C, o, m0, i0, 0, t, r);
-  bc_store<NT>(m0, 0, t, r, Bs[0], Cs[0]);
-  if (AK < n) bc_load<NT>(B, n, C, o, m0, i0, AK, t, r);
+  bc_load<NT>(B, n, C, o, m0, i0, po.k0(s), t, r);
+  bc_store<NT>(m0, po.k0(s), t, r, Bs[0], Cs[0]);
+  if (s1 < S) bc_load<NT>(B, n, C, o, m0, i0, po.k0(s1), t, r);
   __syncthreads();
-  for (int k0 = 0, cur = 0; k0 < n; k0 += AK, cur ^= 1) {
-    if (k0 + AK < n) {
-      bc_store<NT>(m0, k0 + AK, t, r, Bs[cur ^ 1], Cs[cur ^ 1]);
-      if (k0 + 2 * AK < n) bc_load<NT>(B, n, C, o, m0, i0, k0 + 2 * AK, t, r);
+  for (int cur = 0; s < S; cur ^= 1) {
+    const int s2 = s1 < S ? po.next(s1) : S;
+    if (s1 < S) {
+      bc_store<NT>(m0, po.k0(s1), t, r, Bs[cur ^ 1], Cs[cur ^ 1]);
+      if (s2 < S) bc_load<NT>(B, n, C, o, m0, i0, po.k0(s2), t, r);
     }
     const double* bs = Bs[cur];
     const double* cs = Cs[cur];
@@ -257,10 +321,12 @@ __global__ __launch_bounds__(NT) void k_bc_packed_f64(const double* __restrict__
         for (int b = 0; b < 5; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
     __syncthreads();
+    s = s1;
+    s1 = s2;
   }
   // A_Q[m][i] -> At[m][Q o + i]; with part, this tile's sum of A_Q . C
   double* Aq = At + (size_t)q * o;
-  double s = 0.0;
+  double sum = 0.0;
 #pragma unroll
   for (int a = 0; a < BA; ++a)
 #pragma unroll
@@ -271,18 +337,18 @@ __global__ __launch_bounds__(NT) void k_bc_packed_f64(const double* __restrict__
         const int i = i0 + 80 * wn + 16 * b + (lane & 15);
         if (m < n && i < o) {
           Aq[(size_t)m * ldt + i] = acc[a][b][x];
-          if (part) s += acc[a][b][x] * C[(size_t)m * o + i];
+          if (part) sum += acc[a][b][x] * C[(size_t)m * o + i];
         }
       }
   if (part) {  // fixed-order block sum
     __shared__ double red[NT];
-    red[t] = s;
+    red[t] = sum;
     __syncthreads();
     for (int h = NT / 2; h > 0; h >>= 1) {
       if (t < h) red[t] += red[t + h];
       __syncthreads();
     }
-    if (t == 0) part[(size_t)q * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+    if (t == 0) part[(size_t)q * gx * gy + wi.it * gx + wi.mt] = red[0];
   }
 }
 
@@ -305,6 +371,7 @@ __device__ __forceinline__ void split_bf16(float f, unsigned& h, unsigned& l) {
 }
 
 constexpr int SM = 128, SN = 160, SK = 32, SP = 40;  // tile rows, columns, contraction step, LDS row (BF16)
+static_assert(SM == PanelOrder<SK>::TS, "tiles of the panel order are the row tiles");
 constexpr int SBUF = (2 * SM + 2 * SN) * SP;         // BF16 per LDS buffer: B hi, B lo, C hi, C lo
 
 // C (n x o, FP64) -> hi and lo parts of C^T, [oc][nk] BF16, zero past (o, n)
@@ -394,9 +461,11 @@ __global__ __launch_bounds__(512) void k_bc_packed_bf16x3(const double* __restri
                                                           const unsigned short* __restrict__ Th,
                                                           const unsigned short* __restrict__ Tl, int nk, int o,
                                                           unsigned short* __restrict__ Ah,
-                                                          unsigned short* __restrict__ Al, long long ldt) {
+                                                          unsigned short* __restrict__ Al, long long ldt, int gx,
+                                                          int gy, long long total, int flags) {
   __shared__ uint4 lds[2][SBUF / 8];
-  const int m0 = blockIdx.x * SM, i0 = blockIdx.y * SN, q = blockIdx.z;
+  const WorkItem wi = work_item(gx, gy, total, flags);
+  const int m0 = wi.mt * SM, i0 = wi.it * SN, q = wi.q;
   const double* B = Bp + (size_t)q * np;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1, rq = lane & 15, g = lane >> 4;
@@ -405,15 +474,19 @@ __global__ __launch_bounds__(512) void k_bc_packed_bf16x3(const double* __restri
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 5; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  const PanelOrder<SK> po(wi.mt, n, flags);
+  const int S = po.count();
+  int s = po.next(-1), s1 = po.next(s);
   SplitRegs r;
-  split_load(B, n, Th, Tl, nk, m0, i0, 0, t, r);
-  split_store(m0, 0, t, r, (unsigned short*)lds[0]);
-  if (SK < n) split_load(B, n, Th, Tl, nk, m0, i0, SK, t, r);
+  split_load(B, n, Th, Tl, nk, m0, i0, po.k0(s), t, r);
+  split_store(m0, po.k0(s), t, r, (unsigned short*)lds[0]);
+  if (s1 < S) split_load(B, n, Th, Tl, nk, m0, i0, po.k0(s1), t, r);
   __syncthreads();
-  for (int k0 = 0, cur = 0; k0 < n; k0 += SK, cur ^= 1) {
-    if (k0 + SK < n) {
-      split_store(m0, k0 + SK, t, r, (unsigned short*)lds[cur ^ 1]);
-      if (k0 + 2 * SK < n) split_load(B, n, Th, Tl, nk, m0, i0, k0 + 2 * SK, t, r);
+  for (int cur = 0; s < S; cur ^= 1) {
+    const int s2 = s1 < S ? po.next(s1) : S;
+    if (s1 < S) {
+      split_store(m0, po.k0(s1), t, r, (unsigned short*)lds[cur ^ 1]);
+      if (s2 < S) split_load(B, n, Th, Tl, nk, m0, i0, po.k0(s2), t, r);
     }
     const unsigned short* S = (const unsigned short*)lds[cur];
     const unsigned short *Bh = S, *Bl = S + SM * SP, *Ch = S + 2 * SM * SP, *Cl = Ch + SN * SP;
@@ -435,6 +508,8 @@ __global__ __launch_bounds__(512) void k_bc_packed_bf16x3(const double* __restri
       }
     }
     __syncthreads();
+    s = s1;
+    s1 = s2;
   }
   // D of the FP32 16 x 16 forms: lane l, register x holds row 4 (l / 16) + x, column l % 16
 #pragma unroll
@@ -481,15 +556,13 @@ void bc_packed_f64(const double* Bp, size_t np, int rank, int n, const double* C
     if (scratch.bytes < need) scratch.alloc(need);
     part = scratch.as<double>();
   }
-  for (int q0 = 0; q0 < rank; q0 += 65535) {
-    const int nq = std::min(65535, rank - q0);
-    double* pq = part ? part + (size_t)q0 * gx * gy : nullptr;
-    // 512 threads (2 waves a SIMD) measured 5 % faster than 256 (1 wave a
-    // SIMD, 160 accumulator registers) on (H2O)32 / cc-pVTZ
-    hipLaunchKernelGGL(k_bc_packed_f64<512>, dim3(gx, gy, (unsigned)nq), dim3(512), 0, st, Bp + (size_t)q0 * np, np, n,
-                       C, o, At + (size_t)q0 * o, ldt, pq);
-    HIP_CHECK(hipGetLastError());
-  }
+  const long long total = (long long)gx * gy * rank;
+  if (total > INT32_MAX) throw HfgError{HFG_UNSUPPORTED, "packed A = B C: too many tiles for one launch"};
+  // 512 threads (2 waves a SIMD) measured 5 % faster than 256 (1 wave a SIMD,
+  // 160 accumulator registers) on (H2O)32 / cc-pVTZ
+  hipLaunchKernelGGL(k_bc_packed_f64<512>, dim3((unsigned)total), dim3(512), 0, st, Bp, np, n, C, o, At, ldt, part,
+                     (int)gx, (int)gy, total, BC_XCD_MAP);
+  HIP_CHECK(hipGetLastError());
   if (gamma) {
     hipLaunchKernelGGL(k_part_reduce, dim3((unsigned)((rank + 255) / 256)), dim3(256), 0, st, part, (int)(gx * gy),
                        rank, alpha, gamma);
@@ -511,12 +584,11 @@ void bc_packed_bf16x3(const double* Bp, size_t np, int rank, int n, const double
                      nk, oc, Th, Tl);
   HIP_CHECK(hipGetLastError());
   const unsigned gx = (unsigned)((n + SM - 1) / SM), gy = (unsigned)(oc / SN);
-  for (int q0 = 0; q0 < rank; q0 += 65535) {
-    const int nq = std::min(65535, rank - q0);
-    hipLaunchKernelGGL(k_bc_packed_bf16x3, dim3(gx, gy, (unsigned)nq), dim3(512), 0, st, Bp + (size_t)q0 * np, np, n,
-                       Th, Tl, nk, o, Ah + (size_t)q0 * o, Al + (size_t)q0 * o, ldt);
-    HIP_CHECK(hipGetLastError());
-  }
+  const long long total = (long long)gx * gy * rank;
+  if (total > INT32_MAX) throw HfgError{HFG_UNSUPPORTED, "packed A = B C: too many tiles for one launch"};
+  hipLaunchKernelGGL(k_bc_packed_bf16x3, dim3((unsigned)total), dim3(512), 0, st, Bp, np, n, Th, Tl, nk, o, Ah, Al,
+                     ldt, (int)gx, (int)gy, total, BC_XCD_MAP | BC_TOURNAMENT);
+  HIP_CHECK(hipGetLastError());
 }
 
 // Contraction slices for an n x n output and contraction length L: enough
