@@ -7,7 +7,8 @@ kept eigenvalues amplify the integrals' last-digit differences to ~1e-10),
 the packed layout equal
 to the unpacked one (FP64 1e-12; FP32 and split-BF16 exchange within their
 stated bounds of the FP64 result), and the DF-RHF energy of water in cc-pVTZ
-within 1e-8 Eh of the oracle's with the iteration count within 1.
+within 1e-8 Eh of the oracle's with the iteration count within 1, also
+with split-BF16 exchange before the final FP64 builds (water dimer).
 HFG_DF_PACKED=1 selects the packed layout for these small systems; the
 engine picks it by itself when the unpacked tensors would not fit in HBM."""
 import os
@@ -108,3 +109,22 @@ def test_df_rhf_cc_pvtz_water_matches_oracle(hf, packed):
     assert r["converged"] and o.converged
     assert abs(r["total_energy"] - o.total_energy) < 1e-8
     assert abs(r["iterations"] - o.iterations) <= 1
+
+
+def test_df_rhf_packed_split_bf16_reaches_the_fp64_energy(hf):
+    """Packed factor with split-BF16 exchange (the packed split-BF16 A = B C
+    kernel) in the early iterations, FP64 builds at the end: the water-dimer
+    cc-pVTZ energy equals the oracle's FP64 DF-RHF energy to 1e-8 Eh (no
+    reference counterpart; the reference is FP64 throughout)."""
+    mol = M.water_cluster((2, 1, 1), seed=2)
+    ob, oa = M.basis_text("cc-pvtz"), M.basis_text("def2-universal-jkfit")
+    with layout(True):
+        r = hf.df_rhf(hf.Basis.from_g94(ob, mol.symbols, mol.xyz), hf.Basis.from_g94(oa, mol.symbols, mol.xyz),
+                      mol.zxyz, k_precision="bf16x3")
+    o, _, _, _, _ = O.df_rhf(O.Basis.build(ob, mol.symbols, mol.xyz), O.Basis.build(oa, mol.symbols, mol.xyz),
+                             mol.zxyz)
+    print("packed DF-RHF dimer cc-pVTZ bf16x3: E %.12f vs %.12f, iters %d vs %d" % (
+        r["total_energy"], o.total_energy, r["iterations"], o.iterations))
+    assert r["converged"] and o.converged
+    assert abs(r["total_energy"] - o.total_energy) < 1e-8
+    assert r["iterations"] <= o.iterations + 4
