@@ -208,13 +208,13 @@ struct PanelOrder {
 // current one is multiplied out of the other LDS buffer, one barrier a step.
 // Waves (NT / 128) x 2, each 128 / (NT / 128) rows x 80 columns of 16 x 16
 // MFMA blocks.
-constexpr int AM = 128, AN = 160, AK = 16;
+constexpr int AM = 128, AN = 160, AK = 32;
 static_assert(AM == PanelOrder<AK>::TS, "tiles of the panel order are the row tiles");
 constexpr int APB = AK + 1, APC = AN + 1;
 
 template <int NT>
 struct BcRegs {
-  static constexpr int LR = NT / 16;         // lanes along a panel row
+  static constexpr int LR = NT / AK;         // lanes along a panel row
   static constexpr int NB = AM * AK / NT;    // B values per thread
   static constexpr int NC = AN * AK / NT;    // C values per thread
   double b[NB], c[NC];
@@ -233,17 +233,17 @@ __device__ __forceinline__ void bc_load(const double* __restrict__ B, int n, con
       r.b[c] = (k < n && m < n) ? B[base + m] : 0.0;
     }
   } else if (k0 + AK <= m0) {
-    const int k = k0 + (t & 15);
+    const int k = k0 + t % AK;
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-      const int m = m0 + (t >> 4) + LR * c;
+      const int m = m0 + t / AK + LR * c;
       r.b[c] = (k < n && m < n) ? B[(size_t)m * (m + 1) / 2 + k] : 0.0;
     }
   } else {
-    const int k = k0 + (t & 15);
+    const int k = k0 + t % AK;
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-      const int m = m0 + (t >> 4) + LR * c;
+      const int m = m0 + t / AK + LR * c;
       const size_t hi = m > k ? m : k, lo = m > k ? k : m;
       r.b[c] = (k < n && m < n) ? B[hi * (hi + 1) / 2 + lo] : 0.0;
     }
@@ -264,7 +264,7 @@ __device__ __forceinline__ void bc_store(int m0, int k0, int t, const BcRegs<NT>
     for (int c = 0; c < NB; ++c) Bs[(t % LR + LR * c) * APB + t / LR] = r.b[c];
   } else {
 #pragma unroll
-    for (int c = 0; c < NB; ++c) Bs[((t >> 4) + LR * c) * APB + (t & 15)] = r.b[c];
+    for (int c = 0; c < NB; ++c) Bs[(t / AK + LR * c) * APB + t % AK] = r.b[c];
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) Cs[(t / LR) * APC + t % LR + LR * c] = r.c[c];
