@@ -32,23 +32,24 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int ST = 128;     // output tile
-constexpr int KB = 16;      // contraction step staged in LDS
+constexpr int KB = 32;      // contraction step staged in LDS
 constexpr int PAD = KB + 1;
-constexpr int WT = ST / 2;  // a wave's quarter of the tile
-constexpr int NB = WT / 16; // 16 x 16 MFMA blocks per wave side
+constexpr int SYRK_NT = 512;  // 8 waves, 4 x 2 over the tile
+constexpr int WTI = ST / 4, WTJ = ST / 2;  // a wave's rows and columns of the tile
+constexpr int NBI = WTI / 16, NBJ = WTJ / 16;  // 16 x 16 MFMA blocks per wave side
 
 // The thread's share of the two ST x KB panels at contraction offset k0:
-// row t / 2, columns 8 (t % 2) .. +7 of each (zero beyond n rows or kend).
+// row t / 4, columns 8 (t % 4) .. +7 of each (zero beyond n rows or kend).
 struct Panel {
   double a[8], b[8];
 };
 __device__ __forceinline__ void load_panel(const double* __restrict__ X, int n, long long ldx, int i0, int j0,
                                            long long k0, long long kend, bool vec, Panel& p) {
-  const int t = threadIdx.x, r = t >> 1, c0 = (t & 1) * 8;
+  const int t = threadIdx.x, r = t >> 2, c0 = (t & 3) * 8;
   const int ra = i0 + r, rb = j0 + r;
   const long long k = k0 + c0;
   if (vec && k + 8 <= kend) {
-    // 16-byte loads: rows start 16-byte aligned when ldx is even, k0 is a multiple of 16
+    // 16-byte loads: rows start 16-byte aligned when ldx is even, k0 is a multiple of KB
 #pragma unroll
     for (int c = 0; c < 8; c += 2) {
       const double2 va = ra < n ? *(const double2*)(X + (size_t)ra * ldx + k + c) : double2{0.0, 0.0};
@@ -68,10 +69,10 @@ __device__ __forceinline__ void load_panel(const double* __restrict__ X, int n, 
   }
 }
 
-__global__ __launch_bounds__(256) void k_syrk_f64(const double* __restrict__ X, int n, long long L, long long ldx,
-                                                  const int2* __restrict__ tiles, int ntile, long long kslice,
-                                                  double* __restrict__ part) {
-  __shared__ double As[ST * PAD], Bs[ST * PAD];
+__global__ __launch_bounds__(SYRK_NT) void k_syrk_f64(const double* __restrict__ X, int n, long long L,
+                                                      long long ldx, const int2* __restrict__ tiles, int ntile,
+                                                      long long kslice, double* __restrict__ part) {
+  __shared__ double As[2][ST * PAD], Bs[2][ST * PAD];
   const int2 tile = tiles[blockIdx.x % ntile];
   const int s = blockIdx.x / ntile;
   const int i0 = tile.x * ST, j0 = tile.y * ST;
@@ -79,47 +80,63 @@ __global__ __launch_bounds__(256) void k_syrk_f64(const double* __restrict__ X, 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wi = w >> 1, wj = w & 1;
   const bool vec = (ldx & 1) == 0;
-  d4 acc[NB][NB];
+  d4 acc[NBI][NBJ];
 #pragma unroll
-  for (int a = 0; a < NB; ++a)
+  for (int a = 0; a < NBI; ++a)
 #pragma unroll
-    for (int b = 0; b < NB; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  const int r = t >> 1, c0 = (t & 1) * 8;
-  Panel cur;
-  if (kbeg < kend) load_panel(X, n, ldx, i0, j0, kbeg, kend, vec, cur);
-  for (long long k0 = kbeg; k0 < kend; k0 += KB) {
+    for (int b = 0; b < NBJ; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const int r = t >> 2, c0 = (t & 3) * 8;
+  auto store = [&](const Panel& p, int buf) {
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      As[r * PAD + c0 + c] = cur.a[c];
-      Bs[r * PAD + c0 + c] = cur.b[c];
+      As[buf][r * PAD + c0 + c] = p.a[c];
+      Bs[buf][r * PAD + c0 + c] = p.b[c];
     }
-    __syncthreads();
-    // the next panels' global loads are in flight during this step's MFMAs
-    if (k0 + KB < kend) load_panel(X, n, ldx, i0, j0, k0 + KB, kend, vec, cur);
+  };
+  // buffer cur holds the step's panels, the registers the next step's; a step
+  // stores them into the other buffer (whose readers all passed the previous
+  // barrier), loads the step after, multiplies, and meets the other waves at
+  // one barrier
+  Panel nxt;
+  if (kbeg < kend) {
+    load_panel(X, n, ldx, i0, j0, kbeg, kend, vec, nxt);
+    store(nxt, 0);
+    if (kbeg + KB < kend) load_panel(X, n, ldx, i0, j0, kbeg + KB, kend, vec, nxt);
+  }
+  __syncthreads();
+  int cur = 0;
+  for (long long k0 = kbeg; k0 < kend; k0 += KB, cur ^= 1) {
+    if (k0 + KB < kend) {
+      store(nxt, cur ^ 1);
+      if (k0 + 2 * KB < kend) load_panel(X, n, ldx, i0, j0, k0 + 2 * KB, kend, vec, nxt);
+    }
+    const double* as = As[cur];
+    const double* bs = Bs[cur];
 #pragma unroll
     for (int kk = 0; kk < KB; kk += 4) {
       const int kq = kk + (lane >> 4), rq = lane & 15;
-      double av[NB], bv[NB];
+      double av[NBI], bv[NBJ];
 #pragma unroll
-      for (int a = 0; a < NB; ++a) av[a] = As[(WT * wi + 16 * a + rq) * PAD + kq];
+      for (int a = 0; a < NBI; ++a) av[a] = as[(WTI * wi + 16 * a + rq) * PAD + kq];
 #pragma unroll
-      for (int b = 0; b < NB; ++b) bv[b] = Bs[(WT * wj + 16 * b + rq) * PAD + kq];
+      for (int b = 0; b < NBJ; ++b) bv[b] = bs[(WTJ * wj + 16 * b + rq) * PAD + kq];
 #pragma unroll
-      for (int a = 0; a < NB; ++a)
+      for (int a = 0; a < NBI; ++a)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < NBJ; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
     __syncthreads();
   }
   double* out = part + (size_t)s * n * n;
 #pragma unroll
-  for (int a = 0; a < NB; ++a)
+  for (int a = 0; a < NBI; ++a)
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
+    for (int b = 0; b < NBJ; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int i = i0 + WT * wi + 16 * a + (lane >> 4) + 4 * q;
-        const int j = j0 + WT * wj + 16 * b + (lane & 15);
+        const int i = i0 + WTI * wi + 16 * a + (lane >> 4) + 4 * q;
+        const int j = j0 + WTJ * wj + 16 * b + (lane & 15);
         if (i < n && j < n) out[(size_t)i * n + j] = acc[a][b][q];
       }
 }
@@ -618,8 +635,8 @@ void syrk_f64(const double* X, int n, long long L, long long ldx, double alpha, 
     for (int b = a; b < nt; ++b) h.push_back(make_int2(a, b));
   HIP_CHECK(hipMemcpyAsync(tiles, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice, st));
   const long long kslice = ((L + S - 1) / S + KB - 1) / KB * KB;
-  hipLaunchKernelGGL(k_syrk_f64, dim3((unsigned)(upper * S)), dim3(256), 0, st, X, n, L, ldx, tiles, upper, kslice,
-                     part);
+  hipLaunchKernelGGL(k_syrk_f64, dim3((unsigned)(upper * S)), dim3(SYRK_NT), 0, st, X, n, L, ldx, tiles, upper,
+                     kslice, part);
   HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_syrk_reduce, dim3(2048), dim3(256), 0, st, part, S, n, alpha, K);
   HIP_CHECK(hipGetLastError());
