@@ -117,6 +117,57 @@ __global__ void k_pack_sym2(const double* __restrict__ X, int n, double* __restr
   Xp[ij] = (ii == j ? 1.0 : 2.0) * X[(size_t)ii * n + j];
 }
 
+// d[Q] = sum_j B[Q][j] v[j] over the nc columns of rows Q (row stride ld), four
+// rows a workgroup so that each load of v serves four rows; 16-byte loads
+// where rows and v are 16-byte aligned; per row a fixed-order sum (strided
+// lanes, then a tree over the workgroup).  B P of DF-J (the transposed GEMV).
+constexpr int RD_ROWS = 4, RD_NT = 512;
+__global__ __launch_bounds__(RD_NT) void k_rowdot(const double* __restrict__ B, size_t ld, size_t nc, int rows,
+                                                  const double* __restrict__ v, double* __restrict__ d) {
+  __shared__ double red[RD_ROWS][RD_NT];
+  const int q0 = blockIdx.x * RD_ROWS, t = threadIdx.x;
+  double s[RD_ROWS];
+  const double* row[RD_ROWS];
+#pragma unroll
+  for (int r = 0; r < RD_ROWS; ++r) {
+    s[r] = 0.0;
+    row[r] = B + (size_t)(q0 + r < rows ? q0 + r : q0) * ld;  // a missing row repeats the first, unused
+  }
+  const bool vec = (ld % 2 == 0) && ((uintptr_t)B % 16 == 0) && ((uintptr_t)v % 16 == 0);
+  if (vec) {
+    const size_t n2 = nc / 2;
+    const double2* v2 = (const double2*)v;
+    for (size_t x = t; x < n2; x += RD_NT) {
+      const double2 b = v2[x];
+#pragma unroll
+      for (int r = 0; r < RD_ROWS; ++r) {
+        const double2 a = ((const double2*)row[r])[x];
+        s[r] = fma(a.x, b.x, s[r]);
+        s[r] = fma(a.y, b.y, s[r]);
+      }
+    }
+    if ((nc & 1) && t == 0)
+#pragma unroll
+      for (int r = 0; r < RD_ROWS; ++r) s[r] = fma(row[r][nc - 1], v[nc - 1], s[r]);
+  } else {
+    for (size_t x = t; x < nc; x += RD_NT) {
+      const double b = v[x];
+#pragma unroll
+      for (int r = 0; r < RD_ROWS; ++r) s[r] = fma(row[r][x], b, s[r]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RD_ROWS; ++r) red[r][t] = s[r];
+  __syncthreads();
+  for (int h = RD_NT / 2; h > 0; h >>= 1) {
+    if (t < h)
+#pragma unroll
+      for (int r = 0; r < RD_ROWS; ++r) red[r][t] += red[r][t + h];
+    __syncthreads();
+  }
+  if (t < RD_ROWS && q0 + t < rows) d[q0 + t] = red[t][0];
+}
+
 // Packed -> symmetric n x n.
 __global__ void k_unpack_sym(const double* __restrict__ Xp, int n, double* __restrict__ X) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,6 +357,11 @@ struct DfEngine {
   // packed factor, FP64 and split BF16: A = B C by the hand-written kernels
   // straight from the packed rows (HFG_DF_BCPACK=0: unpacked chunks and
   // rocBLAS GEMMs)
+  // d = B P by k_rowdot (HFG_DF_ROWDOT=0: rocBLAS dgemv)
+  const bool use_rowdot = [] {
+    const char* e = std::getenv("HFG_DF_ROWDOT");
+    return !(e && std::atoi(e) == 0);
+  }();
   const bool use_bcpack = [] {
     const char* e = std::getenv("HFG_DF_BCPACK");
     return !(e && std::atoi(e) == 0);
@@ -695,7 +751,11 @@ struct DfEngine {
     if (fused)
       bc_packed_f64(d_B.as<double>(), npk, rank, n, dC, o, d_At.as<double>(), (long long)rank * o, 2.0, dq.p, d_bcp,
                     stream);
-    else
+    else if (use_rowdot) {
+      hipLaunchKernelGGL(k_rowdot, dim3((unsigned)((rank + RD_ROWS - 1) / RD_ROWS)), dim3(RD_NT), 0, stream,
+                         d_B.as<double>(), bc, bc, rank, pv, dq.p);
+      HIP_CHECK(hipGetLastError());
+    } else
       RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_transpose, (rocblas_int)bc, rank, &one, d_B.as<double>(),
                              (rocblas_int)bc, pv, 1, &zero, dq.p, 1));
     RB_CHECK(rocblas_dgemv(bl->h, rocblas_operation_none, (rocblas_int)bc, rank, &one, d_B.as<double>(),
