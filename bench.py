@@ -588,8 +588,8 @@ def main():
                            {"trace": tdir, "pmc": pdirs, "copy": cdirs}, last=args.profile_last)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this engine's workloads are single-GPU (DESIGN.md describes how the path "
-              "would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this engine's workloads are single-GPU; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     hf = load_pkg()
     sys.path.insert(0, PKG)
